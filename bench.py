@@ -1,0 +1,183 @@
+"""SGEMM 8192^3 on one MI355X: BF16x3 split-precision MFMA vs plain FP32 MFMA,
+each with its error against an FP64 gemm of the same inputs.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--variants NN,NT,TN,TT]
+
+A step is one SGEMM C := A*B (alpha=1, beta=0, column-major, m=n=k=8192,
+A/B iid N(0,1)) with operands resident in HBM.  The headline value is the
+BF16x3 N,N throughput; the plain-FP32 kernel and the other transpose
+variants are timed the same way and reported alongside.  Kernel time comes
+from HIP events recorded on the stream the kernels are launched on
+(torch's current stream, which the library is handed).  One JSON line is
+printed on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+N = 8192
+FLOPS = 2.0 * N * N * N
+PEAK_BF16 = 2516.6  # TFLOP/s, dense bf16 MFMA, MI355X
+PEAK_FP32 = 157.3   # TFLOP/s, fp32 (matrix = vector), MI355X
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_gemm(mb, torch, ta, tb, A, B, C, prec, steps, warmup):
+    for _ in range(warmup):
+        mb.gemm(ta, tb, N, N, N, 1.0, A, N, B, N, 0.0, C, N, precision=prec)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s.record()
+    for _ in range(steps):
+        rc = mb.gemm(ta, tb, N, N, N, 1.0, A, N, B, N, 0.0, C, N, precision=prec)
+        assert rc == 0
+    e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return s.elapsed_time(e) / steps, wall / steps * 1e3
+
+
+def errors(torch, C, C64, ABabs):
+    d = (C.double() - C64).abs()
+    return {
+        "max_abs": float(d.max()),
+        "rel_frobenius": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(C64)),
+        "max_normwise": float((d / ABabs).max()),  # max |dC_ij| / (|A||B|)_ij
+    }
+
+
+def cpu_baseline(args):
+    """Restated mojoBLAS gemm_v11 (oracle/baseline.c) on the host cores, on the
+    same 8192^3 N,N workload, repeated until >= args.cpu_seconds."""
+    import oracle
+
+    a = np.random.default_rng(1).standard_normal(N * N).astype(np.float32)
+    b = np.random.default_rng(2).standard_normal(N * N).astype(np.float32)
+    c = np.zeros(N * N, np.float32)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.base_sgemm("N", "N", N, N, N, 1.0, a, N, b, N, 0.0, c, N)
+        reps += 1
+        el = time.perf_counter() - t0
+        log("cpu baseline rep %d: %.1f s" % (reps, el))
+        if el >= args.cpu_seconds:
+            break
+    tf = reps * FLOPS / el / 1e12
+    return {
+        "value": round(tf, 4),
+        "unit": "TFLOP/s",
+        "cores": oracle.base_threads(),
+        "kind": "port",
+        "sample": "%d x SGEMM N,N 8192^3 (alpha=1, beta=0) through the restated mojoBLAS gemm_v11 blocking "
+                  "(TK=1024, MC=256, 16x16 SIMD outer-product micro-kernel), %.1f s" % (reps, el),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--variants", default="NN,NT,TN,TT")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the level-3 path runs on a single MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mojoblas_amd as mb
+
+    torch.cuda.set_device(0)
+    g = torch.Generator(device="cuda").manual_seed(1234)
+    A = torch.randn(N * N, device="cuda", generator=g)
+    B = torch.randn(N * N, device="cuda", generator=g)
+    C = torch.empty(N * N, device="cuda")
+    variants = [v.strip().upper() for v in args.variants.split(",") if v.strip()]
+    results = {}
+    for prec in ("bf16x3", "fp32"):
+        for v in variants:
+            ms, wall_ms = time_gemm(mb, torch, v[0], v[1], A, B, C, prec, args.steps, args.warmup)
+            results[(prec, v)] = ms
+            log("%-6s %s: %.3f ms  %.1f TFLOP/s (wall %.3f ms)" % (prec, v, ms, FLOPS / ms / 1e9, wall_ms))
+
+    err = {}
+    if not args.no_errors:
+        # FP64 gemm of the same inputs (torch fp64 on the GPU), itself checked
+        # against the fp64 oracle on sampled entries.
+        A64 = A.view(N, N).t().double()  # column-major buffer -> logical matrix
+        B64 = B.view(N, N).t().double()
+        C64 = A64 @ B64
+        ABabs = A64.abs() @ B64.abs()
+        import oracle
+
+        rng = np.random.default_rng(5)
+        rows, cols = rng.integers(0, N, 256), rng.integers(0, N, 256)
+        ref = oracle.sgemm_sample("N", "N", N, 1.0, A.cpu().numpy(), N, B.cpu().numpy(), N, 0.0, None, rows, cols)
+        chk = C64[torch.from_numpy(rows).cuda(), torch.from_numpy(cols).cuda()].cpu().numpy()
+        assert np.allclose(chk, ref, rtol=1e-12, atol=1e-9), "fp64 reference disagrees with the oracle"
+        for prec in ("bf16x3", "fp32"):
+            mb.gemm("N", "N", N, N, N, 1.0, A, N, B, N, 0.0, C, N, precision=prec)
+            torch.cuda.synchronize()
+            err[prec] = errors(torch, C.view(N, N).t(), C64, ABabs)
+            log(prec, "error vs fp64:", err[prec])
+        del A64, B64, C64, ABabs
+
+    ms_bf = results[("bf16x3", "NN")]
+    tf_bf = FLOPS / ms_bf / 1e9
+    tf_fp = FLOPS / results[("fp32", "NN")] / 1e9
+    peak = PEAK_BF16 / 3.0
+    line = {
+        "metric": "SGEMM TFLOP/s @8192^3: BF16x3-MFMA vs plain FP32 at stated FP64-oracle error",
+        "value": round(tf_bf, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_bf, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (bf16x3 split on bf16 MFMA, fp32 accumulate)",
+        "data": "synthetic: A, B iid N(0,1) fp32 (torch cuda Generator seed 1234), column-major, resident in HBM",
+        "config": {"workload": "SGEMM N,N m=n=k=8192 lda=ldb=ldc=8192 alpha=1 beta=0", "m": N, "n": N, "k": N},
+        "fp32_tflops": round(tf_fp, 3),
+        "bf16x3_over_fp32": round(tf_bf / tf_fp, 3),
+        "variants_tflops": {"%s_%s" % (p, v): round(FLOPS / ms / 1e9, 3) for (p, v), ms in results.items()},
+        "errors_vs_fp64": err,
+        "error_ratio_bf16x3_over_fp32": (
+            {k: round(err["bf16x3"][k] / err["fp32"][k], 3) for k in err["fp32"]} if err else None),
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(tf_bf, 3),
+            "peak": round(peak, 1),
+            "peak_basis": "MI355X dense bf16 MFMA 2516.6 TFLOP/s / 3 bf16 products per fp32 product",
+            "unit": "TFLOP/s",
+            "frac": round(tf_bf / peak, 4),
+            "traffic": None,
+            "kernel": "gemm_bf16x3_kernel<false,false,...>",
+            "fp32_kernel_frac": round(tf_fp / PEAK_FP32, 4),
+        },
+    }
+    if not args.no_cpu:
+        del A, B, C
+        torch.cuda.empty_cache()
+        line["cpu_baseline"] = cpu_baseline(args)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

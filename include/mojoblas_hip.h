@@ -1,0 +1,83 @@
+/*
+ * mojoblas_hip: C-ABI of the MI355X (gfx950) level-3 BLAS hot path.
+ *
+ * Drop-in boundary for mojoBLAS's src/level3 Mojo functions
+ * (docs/reference.md:264-327).  A Mojo shim keeps the reference signature
+ * and calls the matching entry here through `external_call` (bindings in
+ * INTEGRATION.md); the Python layer (mojoblas_amd) binds the same symbols
+ * with ctypes.
+ *
+ * Conventions (all entries):
+ *  - Column-major; element (i,j) of X is X[i + j*ldx]; dimensions int64.
+ *  - Flags are single chars with the reference's accepted spellings
+ *    ('N'/'n', 'T'/'t', 'C'/'c' [= 'T' for real data], 'L'/'l', 'R'/'r',
+ *    'U'/'u').  The reference compares whole strings (gemm.mojo:180-197);
+ *    callers pass s[0] only when len(s) == 1 and an invalid char otherwise,
+ *    which reproduces the same info code.
+ *  - Return value: 0 = success; > 0 = Netlib info (the 1-based index of the
+ *    first invalid argument, numbered as in the reference's validation
+ *    blocks), in which case no output is touched and no GPU work is issued;
+ *    < 0 = runtime error (MBH_E_*), message in mbh_last_error().
+ *  - `_host` entries take host pointers (what the reference's BLASPtr holds,
+ *    src/type_aliases.mojo:15-17): they copy to HBM, run, copy back and
+ *    synchronise; the returned buffers are final.
+ *  - `_dev` entries take device pointers and a hipStream_t passed as void*
+ *    (NULL = the default stream); they are stream-ordered and asynchronous.
+ *  - Quick returns follow BLAS: m==0 or n==0 returns at once; alpha==0 or
+ *    k==0 only scales C by beta (beta==0 writes zeros without reading C).
+ */
+#ifndef MOJOBLAS_HIP_H
+#define MOJOBLAS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic used for an fp32 gemm (and the gemms inside the level-3
+ * recasts). */
+enum mbh_precision {
+    MBH_PREC_FP32 = 0,     /* fp32 MFMA (v_mfma_f32_32x32x2_f32), fp32 accumulate */
+    MBH_PREC_BF16X3 = 1,   /* split x = hi + lo (bf16 each); hi*hi + hi*lo + lo*hi
+                              on bf16 MFMA, fp32 accumulate */
+    MBH_PREC_FP8 = 2,      /* opt-in low accuracy: e4m3 with one fp32 scale per
+                              128x128 tile of A and of B, fp32 accumulate */
+    MBH_PREC_BF16 = 3      /* plain bf16 operands, fp32 accumulate (batched
+                              "bf16 input" mode) */
+};
+
+#define MBH_E_HIP (-1)        /* a HIP runtime call failed */
+#define MBH_E_ALLOC (-2)      /* device or host allocation failed */
+#define MBH_E_PRECISION (-3)  /* precision value not valid for this entry */
+#define MBH_E_ARG (-4)        /* argument outside what the entry supports
+                                 (not a Netlib info case) */
+#define MBH_E_SINGULAR (-5)   /* reserved */
+
+/* Library */
+const char* mbh_version(void);
+const char* mbh_last_error(void);                 /* thread-local message */
+int mbh_set_default_precision(int precision);     /* for _host entries; returns the old value */
+int mbh_get_default_precision(void);
+int mbh_device_sync(void);
+
+/* ---- gemm: C := alpha*op(A)*op(B) + beta*C ------------------------------
+ * Replaces gemm[dtype] (src/level3/gemm.mojo:56-124) for every transpose
+ * variant; info codes as gemm.mojo:179-213 (1,2,3,4,5,8,10,13). */
+int mbh_sgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                   const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
+                   float* c, int64_t ldc);
+int mbh_dgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha,
+                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                   double* c, int64_t ldc);
+int mbh_sgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                  const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
+                  float* c, int64_t ldc, int precision, void* stream);
+int mbh_dgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha,
+                  const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                  double* c, int64_t ldc, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOJOBLAS_HIP_H */

@@ -1,0 +1,61 @@
+"""Loader for libmojoblas_hip.so (the C-ABI declared in include/mojoblas_hip.h).
+
+The library is built in-tree (mojoblas_amd/csrc/Makefile) and loaded from
+this directory only; a missing library is an error, never a fallback.
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmojoblas_hip.so")
+ROOT = os.path.dirname(HERE)
+HEADER = os.path.join(ROOT, "include", "mojoblas_hip.h")
+
+_lib = None
+
+i64 = ctypes.c_int64
+f32 = ctypes.c_float
+f64 = ctypes.c_double
+vp = ctypes.c_void_p
+ch = ctypes.c_char
+i32 = ctypes.c_int
+
+# Argument lists of every exported entry, in header order.
+SIGNATURES = {
+    "mbh_version": ([], ctypes.c_char_p),
+    "mbh_last_error": ([], ctypes.c_char_p),
+    "mbh_set_default_precision": ([i32], i32),
+    "mbh_get_default_precision": ([], i32),
+    "mbh_device_sync": ([], i32),
+    "mbh_sgemm_host": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64], i32),
+    "mbh_dgemm_host": ([ch, ch, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64], i32),
+    "mbh_sgemm_dev": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, i32, vp], i32),
+    "mbh_dgemm_dev": ([ch, ch, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp], i32),
+}
+
+
+def build(jobs=8):
+    """Compile libmojoblas_hip.so for gfx950 with hipcc (no GPU needed)."""
+    subprocess.check_call(["make", "-s", "-j%d" % jobs, "-C", os.path.join(HERE, "csrc")])
+
+
+def lib():
+    """The loaded C-ABI library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libmojoblas_hip.so is not built (%s); run mojoblas_amd._lib.build() or "
+                "__graft_entry__.build() first" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().mbh_last_error().decode()

@@ -1,0 +1,254 @@
+// C-ABI entry points (include/mojoblas_hip.h): argument validation with the
+// reference's Netlib info codes, quick returns, host-pointer staging, and
+// dispatch to the kernel launchers.  No entry runs any GPU work before its
+// arguments are validated.
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/mojoblas_hip.h"
+#include "abi_internal.hpp"
+#include "dispatch.hpp"
+
+namespace mbh {
+
+static thread_local std::string g_err;
+static int g_default_prec = -1;
+static std::mutex g_prec_mu;
+
+void set_error(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    set_error("%s: %s", where, hipGetErrorString(e));
+    return MBH_E_HIP;
+}
+
+bool flag_n(char c) { return c == 'N' || c == 'n'; }
+bool flag_t(char c) { return c == 'T' || c == 't' || c == 'C' || c == 'c'; }
+bool flag_l(char c) { return c == 'L' || c == 'l'; }
+bool flag_r(char c) { return c == 'R' || c == 'r'; }
+bool flag_u(char c) { return c == 'U' || c == 'u'; }
+static int64_t max1(int64_t v) { return v > 1 ? v : 1; }
+
+// gemm.mojo:179-213
+int check_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc) {
+    if (!flag_n(ta) && !flag_t(ta)) return 1;
+    if (!flag_n(tb) && !flag_t(tb)) return 2;
+    if (m < 0) return 3;
+    if (n < 0) return 4;
+    if (k < 0) return 5;
+    if (lda < max1(flag_n(ta) ? m : k)) return 8;
+    if (ldb < max1(flag_n(tb) ? k : n)) return 10;
+    if (ldc < max1(m)) return 13;
+    return 0;
+}
+
+// trsm.mojo:62-89 and trmm.mojo:62-89 (identical numbering)
+int check_trxm(char side, char uplo, char ta, char diag, int64_t m, int64_t n, int64_t lda, int64_t ldb) {
+    if (!flag_l(side) && !flag_r(side)) return 1;
+    if (!flag_u(uplo) && !flag_l(uplo)) return 2;
+    if (!flag_n(ta) && !flag_t(ta)) return 3;
+    if (!flag_u(diag) && !flag_n(diag)) return 4;
+    if (m < 0) return 5;
+    if (n < 0) return 6;
+    if (lda < max1(flag_l(side) ? m : n)) return 9;
+    if (ldb < max1(m)) return 11;
+    return 0;
+}
+
+// syrk.mojo:60-83
+int check_syrk(char uplo, char trans, int64_t n, int64_t k, int64_t lda, int64_t ldc) {
+    if (!flag_u(uplo) && !flag_l(uplo)) return 1;
+    if (!flag_n(trans) && !flag_t(trans)) return 2;
+    if (n < 0) return 3;
+    if (k < 0) return 4;
+    if (lda < max1(flag_n(trans) ? n : k)) return 7;
+    if (ldc < max1(n)) return 10;
+    return 0;
+}
+
+// syr2k.mojo: same order as syrk with ldb at position 9 and ldc at 12
+int check_syr2k(char uplo, char trans, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc) {
+    if (!flag_u(uplo) && !flag_l(uplo)) return 1;
+    if (!flag_n(trans) && !flag_t(trans)) return 2;
+    if (n < 0) return 3;
+    if (k < 0) return 4;
+    if (lda < max1(flag_n(trans) ? n : k)) return 7;
+    if (ldb < max1(flag_n(trans) ? n : k)) return 9;
+    if (ldc < max1(n)) return 12;
+    return 0;
+}
+
+// symm.mojo:68-86
+int check_symm(char side, char uplo, int64_t m, int64_t n, int64_t lda, int64_t ldb, int64_t ldc) {
+    if (!flag_l(side) && !flag_r(side)) return 1;
+    if (!flag_u(uplo) && !flag_l(uplo)) return 2;
+    if (m < 0) return 3;
+    if (n < 0) return 4;
+    if (lda < max1(flag_l(side) ? m : n)) return 7;
+    if (ldb < max1(m)) return 9;
+    if (ldc < max1(m)) return 12;
+    return 0;
+}
+
+int default_precision() {
+    std::lock_guard<std::mutex> g(g_prec_mu);
+    if (g_default_prec < 0) {
+        g_default_prec = MBH_PREC_FP32;
+        const char* e = getenv("MOJOBLAS_HIP_PRECISION");
+        if (e) {
+            if (!strcmp(e, "bf16x3") || !strcmp(e, "BF16X3")) g_default_prec = MBH_PREC_BF16X3;
+            else if (!strcmp(e, "fp8") || !strcmp(e, "FP8")) g_default_prec = MBH_PREC_FP8;
+        }
+    }
+    return g_default_prec;
+}
+
+bool gemm_precision_ok(int prec) { return prec == MBH_PREC_FP32 || prec == MBH_PREC_BF16X3; }
+
+// ---- device staging for the host-pointer entries ----------------------------
+// Grow-only per-slot device buffers, guarded by one mutex that is held for the
+// whole host call (host entries are synchronous).
+struct Slot {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+static std::mutex g_stage_mu;
+static Slot g_slots[8];
+
+hipError_t slot_get(int i, size_t bytes, void** out) {
+    Slot& s = g_slots[i];
+    if (bytes == 0) bytes = 16;
+    if (s.cap < bytes) {
+        if (s.p) {
+            hipError_t e = hipFree(s.p);
+            if (e != hipSuccess) return e;
+            s.p = nullptr;
+            s.cap = 0;
+        }
+        hipError_t e = hipMalloc(&s.p, bytes);
+        if (e != hipSuccess) return e;
+        s.cap = bytes;
+    }
+    *out = s.p;
+    return hipSuccess;
+}
+
+std::unique_lock<std::mutex> stage_lock() { return std::unique_lock<std::mutex>(g_stage_mu); }
+
+// Copy an r x c column-major host matrix (leading dim ld) to a packed device
+// buffer (leading dim max(1, r)), and back.
+hipError_t put_matrix(void* dev, const void* host, int64_t r, int64_t c, int64_t ld, size_t es) {
+    if (r <= 0 || c <= 0) return hipSuccess;
+    return hipMemcpy2D(dev, (size_t)r * es, host, (size_t)ld * es, (size_t)r * es, (size_t)c, hipMemcpyHostToDevice);
+}
+hipError_t get_matrix(void* host, const void* dev, int64_t r, int64_t c, int64_t ld, size_t es) {
+    if (r <= 0 || c <= 0) return hipSuccess;
+    return hipMemcpy2D(host, (size_t)ld * es, dev, (size_t)r * es, (size_t)r * es, (size_t)c, hipMemcpyDeviceToHost);
+}
+
+template <typename T>
+static int gemm_host(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, const T* b,
+                     int64_t ldb, T beta, T* c, int64_t ldc, int prec) {
+    int info = check_gemm(ta, tb, m, n, k, lda, ldb, ldc);
+    if (info) return info;
+    if (m == 0 || n == 0) return 0;
+    const bool at = !flag_n(ta), bt = !flag_n(tb);
+    const bool need_ab = !(alpha == T(0) || k == 0);
+    const int64_t ar = at ? k : m, ac = at ? m : k, br = bt ? n : k, bc = bt ? k : n;
+    auto lk = stage_lock();
+    void *dA = nullptr, *dB = nullptr, *dC = nullptr;
+    hipError_t e;
+    const size_t es = sizeof(T);
+    if ((e = slot_get(2, (size_t)m * n * es, &dC)) != hipSuccess) return hip_fail(e, "hipMalloc(C)");
+    if (need_ab) {
+        if ((e = slot_get(0, (size_t)ar * ac * es, &dA)) != hipSuccess) return hip_fail(e, "hipMalloc(A)");
+        if ((e = slot_get(1, (size_t)br * bc * es, &dB)) != hipSuccess) return hip_fail(e, "hipMalloc(B)");
+        if ((e = put_matrix(dA, a, ar, ac, lda, es)) != hipSuccess) return hip_fail(e, "copy A");
+        if ((e = put_matrix(dB, b, br, bc, ldb, es)) != hipSuccess) return hip_fail(e, "copy B");
+    }
+    if (beta != T(0))
+        if ((e = put_matrix(dC, c, m, n, ldc, es)) != hipSuccess) return hip_fail(e, "copy C");
+    if constexpr (sizeof(T) == 4)
+        e = sgemm(at, bt, m, n, k, alpha, (const float*)dA, max1(ar), (const float*)dB, max1(br), beta, (float*)dC, m,
+                  prec, nullptr);
+    else
+        e = dgemm(at, bt, m, n, k, alpha, (const double*)dA, max1(ar), (const double*)dB, max1(br), beta,
+                  (double*)dC, m, nullptr);
+    if (e != hipSuccess) return hip_fail(e, "gemm launch");
+    if ((e = get_matrix(c, dC, m, n, ldc, es)) != hipSuccess) return hip_fail(e, "copy C back");
+    return 0;
+}
+
+}  // namespace mbh
+
+using namespace mbh;
+
+extern "C" {
+
+const char* mbh_version(void) { return "mojoblas_hip 0.1 (gfx950)"; }
+const char* mbh_last_error(void) { return g_err.c_str(); }
+
+int mbh_set_default_precision(int precision) {
+    int old = default_precision();
+    if (!gemm_precision_ok(precision)) {
+        set_error("mbh_set_default_precision: invalid precision %d", precision);
+        return MBH_E_PRECISION;
+    }
+    std::lock_guard<std::mutex> g(g_prec_mu);
+    g_default_prec = precision;
+    return old;
+}
+int mbh_get_default_precision(void) { return default_precision(); }
+
+int mbh_device_sync(void) {
+    hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? 0 : hip_fail(e, "hipDeviceSynchronize");
+}
+
+int mbh_sgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float* a,
+                   int64_t lda, const float* b, int64_t ldb, float beta, float* c, int64_t ldc) {
+    return gemm_host<float>(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, default_precision());
+}
+
+int mbh_dgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                   int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc) {
+    return gemm_host<double>(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, MBH_PREC_FP32);
+}
+
+int mbh_sgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float* a,
+                  int64_t lda, const float* b, int64_t ldb, float beta, float* c, int64_t ldc, int precision,
+                  void* stream) {
+    int info = check_gemm(transa, transb, m, n, k, lda, ldb, ldc);
+    if (info) return info;
+    if (!gemm_precision_ok(precision)) {
+        set_error("mbh_sgemm_dev: invalid precision %d", precision);
+        return MBH_E_PRECISION;
+    }
+    if (m == 0 || n == 0) return 0;
+    hipError_t e = sgemm(!flag_n(transa), !flag_n(transb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, precision,
+                         (hipStream_t)stream);
+    return e == hipSuccess ? 0 : hip_fail(e, "mbh_sgemm_dev");
+}
+
+int mbh_dgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                  int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc, void* stream) {
+    int info = check_gemm(transa, transb, m, n, k, lda, ldb, ldc);
+    if (info) return info;
+    if (m == 0 || n == 0) return 0;
+    hipError_t e = dgemm(!flag_n(transa), !flag_n(transb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                         (hipStream_t)stream);
+    return e == hipSuccess ? 0 : hip_fail(e, "mbh_dgemm_dev");
+}
+
+}  // extern "C"

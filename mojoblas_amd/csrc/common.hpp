@@ -1,0 +1,76 @@
+// Shared device-side definitions for the gfx950 level-3 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mbh {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+
+enum Prec { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP8 = 2, PREC_BF16 = 3 };
+
+// Epilogue store masks (syrk/syr2k write one triangle of C; the trsm
+// adjoint keeps tril of dB*X^T).
+enum TriMask { TRI_NONE = 0, TRI_LOWER = 1, TRI_UPPER = 2 };
+
+template <typename T>
+struct GemmParams {
+    int64_t m, n, k;
+    const T* A;
+    int64_t lda;
+    const T* B;
+    int64_t ldb;
+    T* C;
+    int64_t ldc;
+    T alpha, beta;
+    int tiles_m, tiles_n;
+    int tri;      // TriMask applied to (row, col) of C
+    int a_vec;    // 16-byte global loads of A allowed (pointer and lda aligned)
+    int b_vec;    // same for B
+    int tri_tiles;  // grid enumerates only tiles touching the `tri` triangle
+};
+
+// Maps the workgroup id to a C macro-tile.
+//  * XCD-aware: the hardware deals workgroups round-robin to the 8 XCDs, so
+//    workgroup w runs on XCD w%8.  Renumbering w -> (w%8)*(nwg/8) + w/8 gives
+//    each XCD a contiguous run of tile ids, so the tiles sharing A row-panels
+//    and B column-panels share one L2.
+//  * Grouped order inside that run: GROUP tile-rows are walked column by
+//    column, so concurrently resident tiles reuse the same A and B panels.
+__device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+    int nwg = (int)gridDim.x;
+    int w = (int)blockIdx.x;
+    if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
+    constexpr int GROUP = 8;
+    int per_group = GROUP * tiles_n;
+    int group = w / per_group;
+    int first_m = group * GROUP;
+    int gsize = min(tiles_m - first_m, GROUP);
+    int in_group = w - group * per_group;
+    tm = first_m + in_group % gsize;
+    tn = in_group / gsize;
+}
+
+// Lower-triangle tile enumeration for syrk-style launches: tile id t ->
+// (tm, tn) with tn <= tm, row-major over the lower block triangle.
+__device__ inline void tri_tile_coords(int t, int& tm, int& tn) {
+    // tm = floor((sqrt(8t+1)-1)/2), fixed up for rounding.
+    int r = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    tm = r;
+    tn = t - r * (r + 1) / 2;
+}
+
+__device__ inline bool tri_keep(int tri, int64_t row, int64_t col) {
+    return tri == TRI_NONE || (tri == TRI_LOWER ? row >= col : row <= col);
+}
+
+}  // namespace mbh

@@ -1,0 +1,172 @@
+// Launchers for the macro-tile gemm kernels and the beta-scale kernel.
+#include <climits>
+
+#include "dispatch.hpp"
+#include "gemm_fp.hpp"
+#include "gemm_packed.hpp"
+#include "pack.hpp"
+
+namespace mbh {
+
+namespace {
+
+template <typename T>
+bool vec_ok(const T* p, int64_t ld) {
+    constexpr int VEC = 16 / sizeof(T);
+    return ((uintptr_t)p % 16 == 0) && (ld % VEC == 0);
+}
+
+template <typename T, int BM, int BN>
+bool grid_for(GemmParams<T>& p, bool tri_grid, dim3& grid) {
+    int64_t tm = (p.m + BM - 1) / BM, tn = (p.n + BN - 1) / BN;
+    int64_t g = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
+    if (g <= 0 || g > INT_MAX || tm > INT_MAX || tn > INT_MAX) return false;
+    p.tiles_m = (int)tm;
+    p.tiles_n = (int)tn;
+    p.tri_tiles = tri_grid ? 1 : 0;
+    grid = dim3((unsigned)g);
+    return true;
+}
+
+template <typename T, int BM, int BN, int BK>
+hipError_t launch_fp(bool at, bool bt, GemmParams<T> p, bool tri_grid, hipStream_t s) {
+    dim3 grid;
+    if (!grid_for<T, BM, BN>(p, tri_grid, grid)) return hipErrorInvalidValue;
+    dim3 block(256);
+    if (!at && !bt) gemm_fp_kernel<T, false, false, BM, BN, BK><<<grid, block, 0, s>>>(p);
+    else if (!at && bt) gemm_fp_kernel<T, false, true, BM, BN, BK><<<grid, block, 0, s>>>(p);
+    else if (at && !bt) gemm_fp_kernel<T, true, false, BM, BN, BK><<<grid, block, 0, s>>>(p);
+    else gemm_fp_kernel<T, true, true, BM, BN, BK><<<grid, block, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+// Split-precision path: pack op(A) and op(B)^T into bf16 planes, then run the
+// packed macro-tile kernel.  256x256 tiles (8 waves) when that still gives one
+// tile per CU, else 128x128 (4 waves).
+template <int NPROD, int BT_, int WM_, int WN_>
+hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, bool tri_grid, hipStream_t s) {
+    constexpr int BT = BT_, PK = 32;
+    PackedGemmParams p{};
+    const int64_t tm = (g.m + BT - 1) / BT, tn = (g.n + BT - 1) / BT;
+    const int64_t ktiles = (g.k + PK - 1) / PK;
+    const int64_t ntile = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
+    if (ntile <= 0 || ntile > INT_MAX || ktiles > INT_MAX) return hipErrorInvalidValue;
+    const int planes = (NPROD == 3) ? 2 : 1;
+    const size_t a_el = (size_t)tm * BT * ktiles * PK, b_el = (size_t)tn * BT * ktiles * PK;
+    void* ws = nullptr;
+    hipError_t e = workspace(s, (a_el + b_el) * planes * sizeof(uint16_t), &ws);
+    if (e != hipSuccess) return e;
+    uint16_t* base = (uint16_t*)ws;
+    PackParams pa{g.A, g.lda, g.m, g.k, (int)ktiles, base, planes == 2 ? base + a_el : nullptr, g.a_vec};
+    uint16_t* bbase = base + a_el * planes;
+    PackParams pb{g.B, g.ldb, g.n, g.k, (int)ktiles, bbase, planes == 2 ? bbase + b_el : nullptr, g.b_vec};
+    dim3 pblock(256);
+    dim3 ga((unsigned)(tm * BT / 64), (unsigned)(ktiles * PK / 32));
+    dim3 gb((unsigned)(tn * BT / 64), (unsigned)(ktiles * PK / 32));
+    // op(A) rows = m: k-contiguous iff A is transposed.
+    if (at) pack_split_kernel<true, BT, PK><<<ga, pblock, 0, s>>>(pa);
+    else pack_split_kernel<false, BT, PK><<<ga, pblock, 0, s>>>(pa);
+    // op(B)^T rows = n: k-contiguous iff B is NOT transposed.
+    if (!bt) pack_split_kernel<true, BT, PK><<<gb, pblock, 0, s>>>(pb);
+    else pack_split_kernel<false, BT, PK><<<gb, pblock, 0, s>>>(pb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    p.m = g.m;
+    p.n = g.n;
+    p.ktiles = (int)ktiles;
+    p.Ah = pa.hi;
+    p.Al = pa.lo;
+    p.Bh = pb.hi;
+    p.Bl = pb.lo;
+    p.C = g.C;
+    p.ldc = g.ldc;
+    p.alpha = g.alpha;
+    p.beta = g.beta;
+    p.tiles_m = (int)tm;
+    p.tiles_n = (int)tn;
+    p.tri = g.tri;
+    p.tri_tiles = tri_grid ? 1 : 0;
+    gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)ntile), dim3(64 * WM_ * WN_), 0, s>>>(p);
+    return hipGetLastError();
+}
+
+template <int NPROD>
+hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, bool tri_grid, hipStream_t s) {
+    const int64_t t256 = ((g.m + 255) / 256) * ((g.n + 255) / 256);
+    if (t256 >= 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s);
+    return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s);
+}
+
+template <typename T>
+__global__ void scale_kernel(int64_t m, int64_t n, T beta, T* C, int64_t ldc, int tri) {
+    const int64_t total = m * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % m, j = idx / m;
+        if (!tri_keep(tri, i, j)) continue;
+        T* c = C + i + j * ldc;
+        *c = (beta == T(0)) ? T(0) : beta * *c;
+    }
+}
+
+template <typename T>
+GemmParams<T> make_params(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                          int64_t ldb, T beta, T* C, int64_t ldc, int tri) {
+    GemmParams<T> p{};
+    p.m = m;
+    p.n = n;
+    p.k = k;
+    p.A = A;
+    p.lda = lda;
+    p.B = B;
+    p.ldb = ldb;
+    p.C = C;
+    p.ldc = ldc;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.tri = tri;
+    p.a_vec = vec_ok(A, lda);
+    p.b_vec = vec_ok(B, ldb);
+    return p;
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t scale(int64_t m, int64_t n, T beta, T* C, int64_t ldc, hipStream_t s, int tri) {
+    if (m <= 0 || n <= 0 || beta == T(1)) return hipSuccess;
+    int64_t blocks = (m * n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    scale_kernel<T><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(m, n, beta, C, ldc, tri);
+    return hipGetLastError();
+}
+template hipError_t scale<float>(int64_t, int64_t, float, float*, int64_t, hipStream_t, int);
+template hipError_t scale<double>(int64_t, int64_t, double, double*, int64_t, hipStream_t, int);
+
+hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                 const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s, int tri,
+                 bool tri_grid) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (alpha == 0.0f || k == 0) return scale<float>(m, n, beta, C, ldc, s, tri);
+    GemmParams<float> p = make_params<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+    switch (prec) {
+        case PREC_FP32:
+            return launch_fp<float, 128, 128, 16>(at, bt, p, tri_grid, s);
+        case PREC_BF16X3:
+            return launch_packed<3>(at, bt, p, tri_grid, s);
+        case PREC_BF16:
+            return launch_packed<1>(at, bt, p, tri_grid, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                 const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s, int tri,
+                 bool tri_grid) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (alpha == 0.0 || k == 0) return scale<double>(m, n, beta, C, ldc, s, tri);
+    GemmParams<double> p = make_params<double>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+    return launch_fp<double, 64, 64, 16>(at, bt, p, tri_grid, s);
+}
+
+}  // namespace mbh

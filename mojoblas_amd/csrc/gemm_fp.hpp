@@ -1,0 +1,222 @@
+// Plain FP32 / FP64 macro-tile gemm on the gfx950 matrix cores.
+//
+//   C := alpha*op(A)*op(B) + beta*C, column-major, any lda/ldb/ldc,
+//   op = N or T per operand (template AT / BT), edge tiles zero-filled.
+//
+// Layout in LDS: each operand panel is stored k-major, [BK][rows + pad], so an
+// MFMA operand fetch is one ds_read_b32/b64 per lane with the 32 (or 16)
+// lanes of a row group reading consecutive addresses.  Global loads are
+// 16-byte vectors along whichever index is contiguous in HBM: along m/n for
+// A 'N' / B 'T' (stored straight into the LDS row), along k for A 'T' /
+// B 'N' (scattered into 4 LDS rows; the row pad is chosen so those 4-byte
+// writes from one half-wave hit 32 distinct banks).
+//
+// MFMA: fp32 uses v_mfma_f32_32x32x2_f32 (32x32 output, 16 regs/lane),
+// fp64 uses v_mfma_f64_16x16x4_f64 (16x16 output, 4 regs/lane).  The operands
+// are passed swapped (B fragment as the instruction's A) so the instruction
+// computes the C^T block: each accumulator register then holds 32 (16)
+// consecutive ROWS of one C column across lanes, which makes the column-major
+// epilogue store coalesced.
+//
+// Pipeline: two LDS buffers; the next k-tile's global loads are issued into
+// registers before the current tile's MFMAs and written to the other buffer
+// after them, one barrier per k-tile.
+#pragma once
+#include "common.hpp"
+
+namespace mbh {
+
+template <typename T>
+struct MfmaFp;
+
+template <>
+struct MfmaFp<float> {
+    static constexpr int MT = 32;   // output tile edge
+    static constexpr int KPI = 2;   // k per instruction
+    static constexpr int NACC = 16; // accumulator registers per lane
+    typedef f32x16 acc_t;
+    __device__ static inline acc_t mma(float x, float y, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct MfmaFp<double> {
+    static constexpr int MT = 16;
+    static constexpr int KPI = 4;
+    static constexpr int NACC = 4;
+    typedef f64x4 acc_t;
+    __device__ static inline acc_t mma(double x, double y, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+    }
+};
+
+// Row of the MFMA output held in register r of lane `lane` (with the swapped
+// operands this is the C column offset; the C row offset is lane % MT).
+//   v_mfma_f32_32x32x*  : 8*(r/4) + 4*(lane/32) + r%4
+//   v_mfma_f64_16x16x4  : 4*r + lane/16   (measured on gfx950: the f64 form
+//                         interleaves the four lane groups row by row)
+template <int MT>
+__device__ inline int acc_col(int r, int lane) {
+    if (MT == 32) return (r >> 2) * 8 + 4 * (lane >> 5) + (r & 3);
+    return 4 * r + (lane >> 4);
+}
+
+// Global -> register staging of one operand panel (ROWS x BK of op(X)).
+// KCONTIG: element (row, kk) lives at src[kk + row*ld]; else src[row + kk*ld].
+template <typename T, bool KCONTIG, int ROWS, int BK, int NT>
+struct PanelLoader {
+    static constexpr int VEC = 16 / sizeof(T);  // elements per 16-byte load
+    static constexpr int NV = ROWS * BK / VEC / NT;
+    static_assert(NV * VEC * NT == ROWS * BK, "panel not divisible by the thread block");
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
+    vec_t r[NV];
+
+    __device__ inline void load(const T* __restrict__ src, int64_t ld, int64_t row0, int64_t nrows, int64_t k0,
+                                int64_t ktot, bool vec_ok, int tid) {
+        const bool full = vec_ok && (row0 + ROWS <= nrows) && (k0 + BK <= ktot);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int idx = tid + NT * v;
+            int row, kk;
+            if (KCONTIG) {
+                row = idx / (BK / VEC);
+                kk = (idx % (BK / VEC)) * VEC;
+            } else {
+                kk = idx / (ROWS / VEC);
+                row = (idx % (ROWS / VEC)) * VEC;
+            }
+            if (full) {
+                const T* p = KCONTIG ? src + (k0 + kk) + (row0 + row) * ld : src + (row0 + row) + (k0 + kk) * ld;
+                r[v] = *reinterpret_cast<const vec_t*>(p);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const int64_t gr = row0 + row + (KCONTIG ? 0 : e);
+                    const int64_t gk = k0 + kk + (KCONTIG ? e : 0);
+                    T x = T(0);
+                    if (gr < nrows && gk < ktot) x = KCONTIG ? src[gk + gr * ld] : src[gr + gk * ld];
+                    r[v][e] = x;
+                }
+            }
+        }
+    }
+
+    // Register -> LDS ([BK][S] layout).
+    template <int S>
+    __device__ inline void store(T* lds, int tid) const {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int idx = tid + NT * v;
+            if (KCONTIG) {
+                const int row = idx / (BK / VEC);
+                const int kk = (idx % (BK / VEC)) * VEC;
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) lds[(kk + e) * S + row] = r[v][e];
+            } else {
+                const int kk = idx / (ROWS / VEC);
+                const int row = (idx % (ROWS / VEC)) * VEC;
+                *reinterpret_cast<vec_t*>(lds + kk * S + row) = r[v];
+            }
+        }
+    }
+};
+
+// LDS row stride for a panel: a multiple of the vector width when rows are
+// written as vectors; rows + 2 (fp32) / rows + 1 (fp64) when written as
+// scattered scalars (4 consecutive kk from 8 rows land in distinct banks).
+template <typename T, bool KCONTIG, int ROWS>
+constexpr int lds_stride() {
+    return KCONTIG ? ROWS + (sizeof(T) == 4 ? 2 : 1) : ROWS + 16 / (int)sizeof(T);
+}
+
+template <typename T, bool AT, bool BT, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_fp_kernel(GemmParams<T> p) {
+    typedef MfmaFp<T> M;
+    constexpr int NT = 256;
+    constexpr int WM = BM / 2, WN = BN / 2;  // 2 x 2 waves
+    constexpr int TM = WM / M::MT, TN = WN / M::MT;
+    constexpr bool AK = AT;    // A 'T': op(A)(i,k) = A[k + i*lda] is k-contiguous
+    constexpr bool BK_ = !BT;  // B 'N': op(B)(k,j) = B[k + j*ldb] is k-contiguous
+    constexpr int SA = lds_stride<T, AK, BM>();
+    constexpr int SB = lds_stride<T, BK_, BN>();
+    __shared__ __attribute__((aligned(16))) T As[2][BK * SA];
+    __shared__ __attribute__((aligned(16))) T Bs[2][BK * SB];
+
+    int tm, tn;
+    if (p.tri_tiles) tri_tile_coords((int)blockIdx.x, tm, tn);
+    else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+
+    PanelLoader<T, AK, BM, BK, NT> la;
+    PanelLoader<T, BK_, BN, BK, NT> lb;
+    typename M::acc_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < M::NACC; ++r) acc[i][j][r] = T(0);
+
+    const int nk = (int)((p.k + BK - 1) / BK);
+    la.load(p.A, p.lda, m0, p.m, 0, p.k, p.a_vec, tid);
+    lb.load(p.B, p.ldb, n0, p.n, 0, p.k, p.b_vec, tid);
+    la.template store<SA>(As[0], tid);
+    lb.template store<SB>(Bs[0], tid);
+    __syncthreads();
+
+    constexpr int G = 64 / M::MT;  // lane groups = k values per instruction
+    const int lrow = lane % M::MT, lgrp = lane / M::MT;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) {
+            la.load(p.A, p.lda, m0, p.m, (int64_t)(kt + 1) * BK, p.k, p.a_vec, tid);
+            lb.load(p.B, p.ldb, n0, p.n, (int64_t)(kt + 1) * BK, p.k, p.b_vec, tid);
+        }
+        const T* Ab = As[cur] + lgrp * SA + wm0 + lrow;
+        const T* Bb = Bs[cur] + lgrp * SB + wn0 + lrow;
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += G) {
+            T a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = Ab[kk * SA + i * M::MT];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b[j] = Bb[kk * SB + j * M::MT];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) acc[i][j] = M::mma(b[j], a[i], acc[i][j]);
+        }
+        if (kt + 1 < nk) {
+            la.template store<SA>(As[cur ^ 1], tid);
+            lb.template store<SB>(Bs[cur ^ 1], tid);
+        }
+        __syncthreads();
+    }
+
+    // Epilogue: register r of tile (i,j) holds C(row, col) with
+    // row = m0 + wm0 + i*MT + lane%MT, col = n0 + wn0 + j*MT + acc_col(r, lane).
+    const T alpha = p.alpha, beta = p.beta;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int64_t row = m0 + wm0 + i * M::MT + lrow;
+        if (row >= p.m) continue;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+#pragma unroll
+            for (int r = 0; r < M::NACC; ++r) {
+                const int64_t col = n0 + wn0 + j * M::MT + acc_col<M::MT>(r, lane);
+                if (col < p.n && tri_keep(p.tri, row, col)) {
+                    T* cp = p.C + row + col * p.ldc;
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * *cp;
+                    *cp = v;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace mbh

@@ -1,0 +1,99 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+header declares, and rejects invalid arguments with the reference's Netlib
+info codes before any GPU call (so these run without a GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from mojoblas_amd import _lib
+import mojoblas_amd as mb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(mbh_\w+)\s*\(", text))
+    return names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = _lib.lib()
+    declared = header_symbols()
+    assert declared, "no declarations found"
+    for name in sorted(declared):
+        assert hasattr(L, name), "missing export " + name
+    # and the Python binding knows the signature of each
+    assert declared <= set(_lib.SIGNATURES), sorted(declared - set(_lib.SIGNATURES))
+
+
+def test_version_string():
+    assert b"gfx950" in _lib.lib().mbh_version()
+
+
+# (args, info) for gemm, gemm.mojo:179-213.
+GEMM_INFO = [
+    ((b"X", b"N", 2, 2, 2, 2, 2, 2), 1),
+    ((b"N", b"\x00", 2, 2, 2, 2, 2, 2), 2),
+    ((b"N", b"N", -1, 2, 2, 2, 2, 2), 3),
+    ((b"N", b"N", 2, -1, 2, 2, 2, 2), 4),
+    ((b"N", b"N", 2, 2, -1, 2, 2, 2), 5),
+    ((b"N", b"N", 2, 2, 2, 1, 2, 2), 8),
+    ((b"t", b"N", 2, 2, 3, 2, 3, 2), 8),
+    ((b"N", b"N", 2, 2, 2, 2, 1, 2), 10),
+    ((b"N", b"c", 2, 3, 2, 2, 2, 2), 10),
+    ((b"N", b"N", 2, 2, 2, 2, 2, 1), 13),
+]
+
+
+@pytest.mark.parametrize("args,info", GEMM_INFO)
+@pytest.mark.parametrize("entry", ["host_s", "host_d", "dev_s", "dev_d"])
+def test_gemm_info_codes_without_gpu(args, info, entry):
+    L = _lib.lib()
+    ta, tb, m, n, k, lda, ldb, ldc = args
+    dt = np.float32 if entry.endswith("s") else np.float64
+    a = np.full(16, 7, dt)
+    b = np.full(16, 7, dt)
+    c = np.full(16, 7, dt)
+    P = lambda x: ctypes.c_void_p(x.ctypes.data)
+    if entry == "host_s":
+        rc = L.mbh_sgemm_host(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc)
+    elif entry == "host_d":
+        rc = L.mbh_dgemm_host(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc)
+    elif entry == "dev_s":
+        rc = L.mbh_sgemm_dev(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc, 0, None)
+    else:
+        rc = L.mbh_dgemm_dev(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc, None)
+    assert rc == info
+    assert (c == 7).all() and (a == 7).all() and (b == 7).all()
+
+
+def test_python_mirror_prints_info_like_the_reference(capsys):
+    a = np.zeros(4, np.float32)
+    assert mb.gemm("No", "N", 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a.copy(), 2) == 1
+    assert "gemm: Info 1" in capsys.readouterr().out
+    assert mb.gemm("N", "N", 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a.copy(), 1) == 13
+    assert "gemm: Info 13" in capsys.readouterr().out
+
+
+def test_zero_dimension_quick_return_touches_nothing():
+    # m == 0 returns before any GPU call (gemm.mojo:215-216)
+    c = np.full(4, 7, np.float32)
+    assert mb.gemm("N", "N", 0, 2, 2, 1.0, c, 1, c, 2, 0.0, c, 1) == 0
+    assert (c == 7).all()
+
+
+def test_invalid_precision_rejected():
+    L = _lib.lib()
+    a = np.zeros(4, np.float32)
+    rc = L.mbh_sgemm_dev(b"N", b"N", 2, 2, 2, 1.0, ctypes.c_void_p(a.ctypes.data), 2,
+                         ctypes.c_void_p(a.ctypes.data), 2, 0.0, ctypes.c_void_p(a.ctypes.data), 2, 99, None)
+    assert rc == -3
+    assert "precision" in _lib.last_error()

@@ -1,0 +1,161 @@
+"""GPU parity of gemm (all four transposes, alpha/beta, lda/ldb/ldc, edge
+tiles) against the fp64 oracle, through the C-ABI (device and host entries).
+
+Bound, componentwise: |C - C_oracle| <= (e_rep + 2 k u)|alpha| |A||B| + 4u|beta C|
+with u = 2^-24 and e_rep = 0 (fp32) or 3*2^-18 (bf16x3, see helpers.rep_error).
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers as H
+
+torch = pytest.importorskip("torch")
+import mojoblas_amd as mb
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "level3_kat.json")))
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def run_dev(ta, tb, m, n, k, alpha, abuf, lda, bbuf, ldb, beta, cbuf, ldc, prec):
+    A, B, C = dev(abuf), dev(bbuf), dev(cbuf)
+    rc = mb.gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, precision=prec)
+    torch.cuda.synchronize()
+    assert rc == 0
+    return C.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", [t for t in KAT["tests"] if t["routine"] == "gemm"], ids=lambda t: t["name"])
+@pytest.mark.parametrize("entry", ["dev", "host"])
+def test_known_answers(case, entry):
+    dt = np.float32 if case["dtype"] == "float32" else np.float64
+    arrays = {k: np.array(v, dtype=dt) for k, v in case["arrays"].items()}
+    if entry == "dev":
+        arrays = {k: dev(v) for k, v in arrays.items()}
+    args = [arrays[a["array"]] if isinstance(a, dict) else a for a in case["args"]]
+    assert mb.gemm(*args) == 0
+    torch.cuda.synchronize()
+    for e in case["expects"]:
+        got = float(arrays[e["array"]][e["index"]])
+        assert abs(got - e["value"]) <= e["atol"] + e["rtol"] * abs(e["value"]), (case["name"], e, got)
+
+
+SHAPES = [(1, 1, 1), (7, 13, 5), (128, 128, 16), (129, 65, 33), (64, 300, 257), (300, 257, 190)]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
+def test_gemm_vs_oracle(ta, tb, prec):
+    rng = np.random.default_rng(11)
+    for idx, (m, n, k) in enumerate(SHAPES):
+        pad = idx % 3  # exercises odd leading dimensions (scalar load path)
+        alpha, beta = [(1.0, 0.0), (1.5, 0.5), (-0.75, 1.0)][idx % 3]
+        ar, ac = (m, k) if ta == "N" else (k, m)
+        br, bc = (k, n) if tb == "N" else (n, k)
+        lda, ldb, ldc = ar + pad, br + pad, m + pad
+        abuf, amat = H.col_major(rng, ar, ac, lda)
+        bbuf, bmat = H.col_major(rng, br, bc, ldb)
+        cbuf, cmat = H.col_major(rng, m, n, ldc, fill=np.nan if beta == 0 else None)
+        got = run_dev(ta, tb, m, n, k, alpha, abuf, lda, bbuf, ldb, beta, cbuf, ldc, prec)
+        want = H.oracle_gemm(ta, tb, m, n, k, alpha, abuf, lda, bbuf, ldb, beta, np.nan_to_num(cbuf), ldc)
+        bound = H.gemm_bound(prec, k, alpha, H.abs_product(ta, tb, amat, bmat), beta, np.nan_to_num(cmat))
+        err = np.abs(H.logical(got, m, n, ldc).astype(np.float64) - want)
+        assert (err <= bound).all(), (m, n, k, ta, tb, prec, float(err.max()), float((err / bound).max()))
+        # padding rows of C untouched
+        full = got[: ldc * n].reshape(n, ldc).T
+        assert np.isnan(full[m:]).all()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_config1_sgemm_512(prec):
+    """SGEMM N,N 512^3, lda=ldb=ldc=512, alpha=1.5, beta=0.5, A/B/C U[-1,1]."""
+    m = n = k = 512
+    a = oracle.fill_f32(m * k, 101, "uniform")
+    b = oracle.fill_f32(k * n, 102, "uniform")
+    c = oracle.fill_f32(m * n, 103, "uniform")
+    got = run_dev("N", "N", m, n, k, 1.5, a, m, b, k, 0.5, c, m, prec)
+    want = H.oracle_gemm("N", "N", m, n, k, 1.5, a, m, b, k, 0.5, c, m)
+    amat = a.reshape(k, m).T.astype(np.float64)
+    bmat = b.reshape(n, k).T.astype(np.float64)
+    bound = H.gemm_bound(prec, k, 1.5, np.abs(amat) @ np.abs(bmat), 0.5, c.reshape(n, m).T)
+    err = np.abs(got.reshape(n, m).T - want)
+    assert (err <= bound).all(), float((err / bound).max())
+    # the host-pointer entry gives the same result bit for bit
+    ch = c.copy()
+    assert mb.gemm("N", "N", m, n, k, 1.5, a, m, b, k, 0.5, ch, m, precision=prec) == 0
+    assert np.array_equal(ch, got)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_deterministic_and_beta_zero_does_not_read_c(prec):
+    rng = np.random.default_rng(12)
+    m, n, k = 257, 130, 300
+    abuf, _ = H.col_major(rng, m, k, m)
+    bbuf, _ = H.col_major(rng, k, n, k)
+    c1 = run_dev("N", "N", m, n, k, 1.0, abuf, m, bbuf, k, 0.0, np.full(m * n, np.nan, np.float32), m, prec)
+    c2 = run_dev("N", "N", m, n, k, 1.0, abuf, m, bbuf, k, 0.0, np.zeros(m * n, np.float32), m, prec)
+    assert np.isfinite(c1).all()
+    assert np.array_equal(c1, c2)
+
+
+def test_alpha_zero_scales_only():
+    m, n, k = 33, 17, 9
+    a = np.full(m * k, np.nan, np.float32)  # not read when alpha == 0
+    c = np.arange(m * n, dtype=np.float32)
+    got = run_dev("N", "N", m, n, k, 0.0, a, m, a, k, 2.0, c, m, "fp32")
+    assert np.array_equal(got, 2.0 * c)
+    got = run_dev("N", "N", m, n, k, 0.0, a, m, a, k, 0.0, c, m, "fp32")
+    assert (got == 0).all()
+
+
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
+def test_dgemm_vs_oracle(ta, tb):
+    rng = np.random.default_rng(13)
+    for m, n, k in [(5, 3, 7), (64, 64, 64), (100, 77, 130)]:
+        ar, ac = (m, k) if ta == "N" else (k, m)
+        br, bc = (k, n) if tb == "N" else (n, k)
+        abuf, amat = H.col_major(rng, ar, ac, ar + 1, dtype=np.float64)
+        bbuf, bmat = H.col_major(rng, br, bc, br, dtype=np.float64)
+        cbuf, cmat = H.col_major(rng, m, n, m, dtype=np.float64)
+        got = run_dev(ta, tb, m, n, k, 1.25, abuf, ar + 1, bbuf, br, -0.5, cbuf, m, None)
+        want = cbuf.copy()
+        oracle.dgemm(ta, tb, m, n, k, 1.25, abuf, ar + 1, bbuf, br, -0.5, want, m)
+        bound = 4 * k * 2.0 ** -53 * (1.25 * H.abs_product(ta, tb, amat, bmat) + 0.5 * np.abs(cmat)) + 1e-300
+        assert (np.abs(got.reshape(n, m).T - want.reshape(n, m).T) <= bound).all()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
+def test_large_sampled(ta, tb, prec):
+    """2048 x 1536 x 2304 (ragged against the 128-tiles) on the vector path,
+    priced on 4096 sampled entries by the oracle; plus linearity in alpha."""
+    m, n, k = 2048 + 64, 1536 + 32, 2304
+    g = torch.Generator(device="cuda").manual_seed(14)
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    A = torch.randn(ar * ac, device="cuda", generator=g)
+    B = torch.randn(br * bc, device="cuda", generator=g)
+    C = torch.empty(m * n, device="cuda")
+    assert mb.gemm(ta, tb, m, n, k, 1.0, A, ar, B, br, 0.0, C, m, precision=prec) == 0
+    C2 = torch.empty_like(C)
+    assert mb.gemm(ta, tb, m, n, k, -2.0, A, ar, B, br, 0.0, C2, m, precision=prec) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(C2, -2.0 * C)  # alpha applied exactly once, in the epilogue
+    rng = np.random.default_rng(15)
+    rows = rng.integers(0, m, 4096)
+    cols = rng.integers(0, n, 4096)
+    a, b, c = A.cpu().numpy(), B.cpu().numpy(), C.cpu().numpy()
+    want = oracle.sgemm_sample(ta, tb, k, 1.0, a, ar, b, br, 0.0, None, rows, cols)
+    absab = oracle.sgemm_sample_abs(ta, tb, k, a, ar, b, br, rows, cols)
+    got = c[rows + cols * m].astype(np.float64)
+    bound = H.gemm_bound(prec, k, 1.0, absab, 0.0, 0.0)
+    assert (np.abs(got - want) <= bound).all(), float((np.abs(got - want) / bound).max())
