@@ -77,6 +77,78 @@ int mbh_dgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, dou
                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                   double* c, int64_t ldc, void* stream);
 
+/* ---- trsm: B := alpha*inv(op(A))*B (side L) or alpha*B*inv(op(A)) (side R)
+ * Replaces trsm[dtype] (src/level3/trsm.mojo:20-275; info codes :62-89:
+ * 1,2,3,4,5,6,9,11).  BLAS semantics, including side R with trans T (the
+ * reference solves the untransposed system there, trsm.mojo:223-273).
+ * fp32: `precision` selects the gemm of the off-diagonal updates; the 128x128
+ * diagonal blocks are inverted and applied in fp64; refine = 1 adds one step
+ * of iterative refinement with an fp64 residual (0 = none).  The _host entry
+ * uses the default precision and refine = 1. */
+int mbh_strsm_host(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float alpha,
+                   const float* a, int64_t lda, float* b, int64_t ldb);
+int mbh_dtrsm_host(char side, char uplo, char transa, char diag, int64_t m, int64_t n, double alpha,
+                   const double* a, int64_t lda, double* b, int64_t ldb);
+int mbh_strsm_dev(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float alpha,
+                  const float* a, int64_t lda, float* b, int64_t ldb, int precision, int refine,
+                  void* stream);
+int mbh_dtrsm_dev(char side, char uplo, char transa, char diag, int64_t m, int64_t n, double alpha,
+                  const double* a, int64_t lda, double* b, int64_t ldb, void* stream);
+
+/* ---- trmm: B := alpha*op(A)*B (side L) or alpha*B*op(A) (side R) --------
+ * Replaces trmm[dtype] (src/level3/trmm.mojo:20-305; info codes :62-89);
+ * alpha == 0 zeroes B without reading A (trmm.mojo:94-98). */
+int mbh_strmm_host(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float alpha,
+                   const float* a, int64_t lda, float* b, int64_t ldb);
+int mbh_dtrmm_host(char side, char uplo, char transa, char diag, int64_t m, int64_t n, double alpha,
+                   const double* a, int64_t lda, double* b, int64_t ldb);
+int mbh_strmm_dev(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float alpha,
+                  const float* a, int64_t lda, float* b, int64_t ldb, int precision, void* stream);
+int mbh_dtrmm_dev(char side, char uplo, char transa, char diag, int64_t m, int64_t n, double alpha,
+                  const double* a, int64_t lda, double* b, int64_t ldb, void* stream);
+
+/* ---- syrk: C := alpha*A*A^T + beta*C (trans N) or alpha*A^T*A + beta*C ---
+ * Replaces syrk[dtype] (src/level3/syrk.mojo:20-200; info codes :60-83:
+ * 1,2,3,4,7,10); only the `uplo` triangle of C is read or written. */
+int mbh_ssyrk_host(char uplo, char trans, int64_t n, int64_t k, float alpha, const float* a,
+                   int64_t lda, float beta, float* c, int64_t ldc);
+int mbh_dsyrk_host(char uplo, char trans, int64_t n, int64_t k, double alpha, const double* a,
+                   int64_t lda, double beta, double* c, int64_t ldc);
+int mbh_ssyrk_dev(char uplo, char trans, int64_t n, int64_t k, float alpha, const float* a,
+                  int64_t lda, float beta, float* c, int64_t ldc, int precision, void* stream);
+int mbh_dsyrk_dev(char uplo, char trans, int64_t n, int64_t k, double alpha, const double* a,
+                  int64_t lda, double beta, double* c, int64_t ldc, void* stream);
+
+/* ---- syr2k: C := alpha*(A*B^T + B*A^T) + beta*C (trans N) or
+ * alpha*(A^T*B + B^T*A) + beta*C.  Replaces syr2k[dtype]
+ * (src/level3/syr2k.mojo:20-190; info codes 1,2,3,4,7,9,12). */
+int mbh_ssyr2k_host(char uplo, char trans, int64_t n, int64_t k, float alpha, const float* a,
+                    int64_t lda, const float* b, int64_t ldb, float beta, float* c, int64_t ldc);
+int mbh_dsyr2k_host(char uplo, char trans, int64_t n, int64_t k, double alpha, const double* a,
+                    int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc);
+int mbh_ssyr2k_dev(char uplo, char trans, int64_t n, int64_t k, float alpha, const float* a,
+                   int64_t lda, const float* b, int64_t ldb, float beta, float* c, int64_t ldc,
+                   int precision, void* stream);
+int mbh_dsyr2k_dev(char uplo, char trans, int64_t n, int64_t k, double alpha, const double* a,
+                   int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc,
+                   void* stream);
+
+/* ---- symm: C := alpha*A*B + beta*C (side L) or alpha*B*A + beta*C (side R),
+ * A symmetric, read from its `uplo` triangle.  Replaces symm[dtype]
+ * (src/level3/symm.mojo:20-241; info codes :68-86: 1,2,3,4,7,9,12); the
+ * full alpha*temp2 term is applied even where B(l,j) == 0 (the reference
+ * skips it, symm.mojo:114,133). */
+int mbh_ssymm_host(char side, char uplo, int64_t m, int64_t n, float alpha, const float* a,
+                   int64_t lda, const float* b, int64_t ldb, float beta, float* c, int64_t ldc);
+int mbh_dsymm_host(char side, char uplo, int64_t m, int64_t n, double alpha, const double* a,
+                   int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc);
+int mbh_ssymm_dev(char side, char uplo, int64_t m, int64_t n, float alpha, const float* a,
+                  int64_t lda, const float* b, int64_t ldb, float beta, float* c, int64_t ldc,
+                  int precision, void* stream);
+int mbh_dsymm_dev(char side, char uplo, int64_t m, int64_t n, double alpha, const double* a,
+                  int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,6 +139,72 @@ def gemm(trans_a, trans_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, precisi
     return _finish("gemm", rc)
 
 
+def _call(name, prefix_args, ptr_args, tail_args, operands, precision, stream, host_extra=(), dev_extra_float=None,
+          dev_extra_double=()):
+    """Route one level-3 call to the s/d, host/dev entry of the C-ABI."""
+    L = _lib.lib()
+    kind, dt = _common(*operands)
+    t = "s" if dt == "float32" else "d"
+    fn = getattr(L, "mbh_%s%s_%s" % (t, name, kind))
+    args = list(prefix_args) + list(ptr_args) + list(tail_args)
+    if kind == "dev":
+        extra = (list(dev_extra_float) if t == "s" else list(dev_extra_double)) + [_stream(stream)]
+        return _finish(name, fn(*(args + extra)))
+    if t == "s" and precision is not None:
+        old = L.mbh_set_default_precision(_prec(precision))
+        try:
+            return _finish(name, fn(*args))
+        finally:
+            if old >= 0:
+                L.mbh_set_default_precision(old)
+    return _finish(name, fn(*args))
+
+
+def trsm(side, uplo, trans_a, diag, m, n, alpha, a, lda, b, ldb, precision=None, refine=1, stream=None):
+    """Solve op(A) X = alpha B (side L) or X op(A) = alpha B (side R); X
+    overwrites B (src/level3/trsm.mojo:20-275, BLAS semantics).
+
+    float32: `precision` ("fp32" default, "bf16x3") selects the gemm of the
+    off-diagonal updates; diagonal 128x128 blocks are solved in fp64;
+    refine=1 adds one fp64-residual refinement step (device entry; the host
+    entry always refines once).
+    """
+    ptrs = [_p(a), lda, _p(b), ldb]
+    return _call("trsm", [_flag(side), _flag(uplo), _flag(trans_a), _flag(diag), m, n, alpha], [],
+                 ptrs, (a, b), precision, stream, dev_extra_float=[_prec(precision), int(refine)])
+
+
+def trmm(side, uplo, trans_a, diag, m, n, alpha, a, lda, b, ldb, precision=None, stream=None):
+    """B := alpha*op(A)*B (side L) or alpha*B*op(A) (side R), A triangular
+    (src/level3/trmm.mojo:20-305)."""
+    ptrs = [_p(a), lda, _p(b), ldb]
+    return _call("trmm", [_flag(side), _flag(uplo), _flag(trans_a), _flag(diag), m, n, alpha], [],
+                 ptrs, (a, b), precision, stream, dev_extra_float=[_prec(precision)])
+
+
+def syrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, precision=None, stream=None):
+    """C := alpha*A*A^T + beta*C (trans N) or alpha*A^T*A + beta*C on the
+    `uplo` triangle (src/level3/syrk.mojo:20-200)."""
+    return _call("syrk", [_flag(uplo), _flag(trans), n, k, alpha], [], [_p(a), lda, beta, _p(c), ldc], (a, c),
+                 precision, stream, dev_extra_float=[_prec(precision)])
+
+
+def syr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc, precision=None, stream=None):
+    """C := alpha*(A*B^T + B*A^T) + beta*C (trans N) or alpha*(A^T*B + B^T*A)
+    + beta*C on the `uplo` triangle (src/level3/syr2k.mojo:20-190)."""
+    return _call("syr2k", [_flag(uplo), _flag(trans), n, k, alpha], [],
+                 [_p(a), lda, _p(b), ldb, beta, _p(c), ldc], (a, b, c), precision, stream,
+                 dev_extra_float=[_prec(precision)])
+
+
+def symm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, precision=None, stream=None):
+    """C := alpha*A*B + beta*C (side L) or alpha*B*A + beta*C (side R), A
+    symmetric from its `uplo` triangle (src/level3/symm.mojo:20-241)."""
+    return _call("symm", [_flag(side), _flag(uplo), m, n, alpha], [],
+                 [_p(a), lda, _p(b), ldb, beta, _p(c), ldc], (a, b, c), precision, stream,
+                 dev_extra_float=[_prec(precision)])
+
+
 # The reference exports its gemm generations under their own names
 # (src/level3/__init__.mojo:29-39); here they are the one GPU gemm.
 gemm_v3 = gemm_v6 = gemm_v7 = gemm_v8 = gemm_v9 = gemm_v10 = gemm_v11 = gemm_v12 = gemm

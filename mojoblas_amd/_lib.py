@@ -32,6 +32,26 @@ SIGNATURES = {
     "mbh_dgemm_host": ([ch, ch, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64], i32),
     "mbh_sgemm_dev": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, i32, vp], i32),
     "mbh_dgemm_dev": ([ch, ch, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp], i32),
+    "mbh_strsm_host": ([ch, ch, ch, ch, i64, i64, f32, vp, i64, vp, i64], i32),
+    "mbh_dtrsm_host": ([ch, ch, ch, ch, i64, i64, f64, vp, i64, vp, i64], i32),
+    "mbh_strsm_dev": ([ch, ch, ch, ch, i64, i64, f32, vp, i64, vp, i64, i32, i32, vp], i32),
+    "mbh_dtrsm_dev": ([ch, ch, ch, ch, i64, i64, f64, vp, i64, vp, i64, vp], i32),
+    "mbh_strmm_host": ([ch, ch, ch, ch, i64, i64, f32, vp, i64, vp, i64], i32),
+    "mbh_dtrmm_host": ([ch, ch, ch, ch, i64, i64, f64, vp, i64, vp, i64], i32),
+    "mbh_strmm_dev": ([ch, ch, ch, ch, i64, i64, f32, vp, i64, vp, i64, i32, vp], i32),
+    "mbh_dtrmm_dev": ([ch, ch, ch, ch, i64, i64, f64, vp, i64, vp, i64, vp], i32),
+    "mbh_ssyrk_host": ([ch, ch, i64, i64, f32, vp, i64, f32, vp, i64], i32),
+    "mbh_dsyrk_host": ([ch, ch, i64, i64, f64, vp, i64, f64, vp, i64], i32),
+    "mbh_ssyrk_dev": ([ch, ch, i64, i64, f32, vp, i64, f32, vp, i64, i32, vp], i32),
+    "mbh_dsyrk_dev": ([ch, ch, i64, i64, f64, vp, i64, f64, vp, i64, vp], i32),
+    "mbh_ssyr2k_host": ([ch, ch, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64], i32),
+    "mbh_dsyr2k_host": ([ch, ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64], i32),
+    "mbh_ssyr2k_dev": ([ch, ch, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, i32, vp], i32),
+    "mbh_dsyr2k_dev": ([ch, ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp], i32),
+    "mbh_ssymm_host": ([ch, ch, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64], i32),
+    "mbh_dsymm_host": ([ch, ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64], i32),
+    "mbh_ssymm_dev": ([ch, ch, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, i32, vp], i32),
+    "mbh_dsymm_dev": ([ch, ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp], i32),
 }
 
 

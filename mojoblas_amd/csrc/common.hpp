@@ -20,22 +20,29 @@ enum Prec { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP8 = 2, PREC_BF16 = 3 };
 // adjoint keeps tril of dB*X^T).
 enum TriMask { TRI_NONE = 0, TRI_LOWER = 1, TRI_UPPER = 2 };
 
-template <typename T>
-struct GemmParams {
+// Arguments of the register/LDS-staged MFMA gemm.  Operands may have their
+// own storage types (TA, TB, TC) while the MFMA runs in TS: the trsm solver
+// multiplies fp32 data by fp64 block inverses and forms fp64 residuals of
+// fp32 operands this way.
+template <typename TA, typename TB, typename TC, typename TS>
+struct GemmArgs {
     int64_t m, n, k;
-    const T* A;
+    const TA* A;
     int64_t lda;
-    const T* B;
+    const TB* B;
     int64_t ldb;
-    T* C;
+    TC* C;
     int64_t ldc;
-    T alpha, beta;
+    TS alpha, beta;
     int tiles_m, tiles_n;
-    int tri;      // TriMask applied to (row, col) of C
-    int a_vec;    // 16-byte global loads of A allowed (pointer and lda aligned)
-    int b_vec;    // same for B
-    int tri_tiles;  // grid enumerates only tiles touching the `tri` triangle
+    int tri;        // TriMask applied to (row, col) of C
+    int a_vec;      // 16-byte global loads of A allowed (pointer and lda aligned)
+    int b_vec;      // same for B
+    int tri_tiles;  // 0: full grid; 1/2: grid enumerates only lower/upper block-triangle tiles
+    int krange;     // KRange
 };
+template <typename T>
+using GemmParams = GemmArgs<T, T, T, T>;
 
 // Maps the workgroup id to a C macro-tile.
 //  * XCD-aware: the hardware deals workgroups round-robin to the 8 XCDs, so
@@ -58,15 +65,44 @@ __device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
     tn = in_group / gsize;
 }
 
-// Lower-triangle tile enumeration for syrk-style launches: tile id t ->
-// (tm, tn) with tn <= tm, row-major over the lower block triangle.
-__device__ inline void tri_tile_coords(int t, int& tm, int& tn) {
-    // tm = floor((sqrt(8t+1)-1)/2), fixed up for rounding.
+// Triangle tile enumeration for syrk-style launches: tile id t -> (r, c) with
+// c <= r, row-major over the lower block triangle.  mode 1 returns (tm, tn) =
+// (r, c) (lower triangle of C), mode 2 returns (c, r) (upper triangle).
+__device__ inline void tri_tile_coords(int mode, int t, int& tm, int& tn) {
     int r = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
     while ((r + 1) * (r + 2) / 2 <= t) ++r;
     while (r * (r + 1) / 2 > t) --r;
-    tm = r;
-    tn = t - r * (r + 1) / 2;
+    const int c = t - r * (r + 1) / 2;
+    if (mode == 2) {
+        tm = c;
+        tn = r;
+    } else {
+        tm = r;
+        tn = c;
+    }
+}
+
+// k-range restriction for triangular operands (trmm, trsm residuals): a tile
+// only visits the k-tiles where op(A) or op(B) can be nonzero.
+enum KRange {
+    KR_ALL = 0,
+    KR_LE_ROW = 1,  // op(A) lower: A(i,k) = 0 for k > i      -> k < (tm+1)*BM
+    KR_GE_ROW = 2,  // op(A) upper: A(i,k) = 0 for k < i      -> k >= tm*BM
+    KR_GE_COL = 3,  // op(B) lower: B(k,j) = 0 for k < j      -> k >= tn*BN
+    KR_LE_COL = 4   // op(B) upper: B(k,j) = 0 for k > j      -> k < (tn+1)*BN
+};
+
+__device__ inline void kt_range(int mode, int tm, int tn, int BM, int BN, int BK, int nk, int& kt0, int& kt1) {
+    kt0 = 0;
+    kt1 = nk;
+    switch (mode) {
+        case KR_LE_ROW: kt1 = min(nk, ((tm + 1) * BM) / BK); break;
+        case KR_GE_ROW: kt0 = (tm * BM) / BK; break;
+        case KR_GE_COL: kt0 = (tn * BN) / BK; break;
+        case KR_LE_COL: kt1 = min(nk, ((tn + 1) * BN) / BK); break;
+        default: break;
+    }
+    if (kt0 > nk) kt0 = nk;
 }
 
 __device__ inline bool tri_keep(int tri, int64_t row, int64_t col) {

@@ -16,27 +16,27 @@ bool vec_ok(const T* p, int64_t ld) {
     return ((uintptr_t)p % 16 == 0) && (ld % VEC == 0);
 }
 
-template <typename T, int BM, int BN>
-bool grid_for(GemmParams<T>& p, bool tri_grid, dim3& grid) {
+template <typename P, int BM, int BN>
+bool grid_for(P& p, int tri_grid, dim3& grid) {
     int64_t tm = (p.m + BM - 1) / BM, tn = (p.n + BN - 1) / BN;
     int64_t g = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
     if (g <= 0 || g > INT_MAX || tm > INT_MAX || tn > INT_MAX) return false;
     p.tiles_m = (int)tm;
     p.tiles_n = (int)tn;
-    p.tri_tiles = tri_grid ? 1 : 0;
+    p.tri_tiles = tri_grid;
     grid = dim3((unsigned)g);
     return true;
 }
 
-template <typename T, int BM, int BN, int BK>
-hipError_t launch_fp(bool at, bool bt, GemmParams<T> p, bool tri_grid, hipStream_t s) {
+template <typename T, typename TA, typename TB, typename TC, int BM, int BN, int BK>
+hipError_t launch_fp(bool at, bool bt, GemmArgs<TA, TB, TC, T> p, int tri_grid, hipStream_t s) {
     dim3 grid;
-    if (!grid_for<T, BM, BN>(p, tri_grid, grid)) return hipErrorInvalidValue;
+    if (!grid_for<GemmArgs<TA, TB, TC, T>, BM, BN>(p, tri_grid, grid)) return hipErrorInvalidValue;
     dim3 block(256);
-    if (!at && !bt) gemm_fp_kernel<T, false, false, BM, BN, BK><<<grid, block, 0, s>>>(p);
-    else if (!at && bt) gemm_fp_kernel<T, false, true, BM, BN, BK><<<grid, block, 0, s>>>(p);
-    else if (at && !bt) gemm_fp_kernel<T, true, false, BM, BN, BK><<<grid, block, 0, s>>>(p);
-    else gemm_fp_kernel<T, true, true, BM, BN, BK><<<grid, block, 0, s>>>(p);
+    if (!at && !bt) gemm_fp_kernel<T, TA, TB, TC, false, false, BM, BN, BK><<<grid, block, 0, s>>>(p);
+    else if (!at && bt) gemm_fp_kernel<T, TA, TB, TC, false, true, BM, BN, BK><<<grid, block, 0, s>>>(p);
+    else if (at && !bt) gemm_fp_kernel<T, TA, TB, TC, true, false, BM, BN, BK><<<grid, block, 0, s>>>(p);
+    else gemm_fp_kernel<T, TA, TB, TC, true, true, BM, BN, BK><<<grid, block, 0, s>>>(p);
     return hipGetLastError();
 }
 
@@ -44,7 +44,7 @@ hipError_t launch_fp(bool at, bool bt, GemmParams<T> p, bool tri_grid, hipStream
 // packed macro-tile kernel.  256x256 tiles (8 waves) when that still gives one
 // tile per CU, else 128x128 (4 waves).
 template <int NPROD, int BT_, int WM_, int WN_>
-hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, bool tri_grid, hipStream_t s) {
+hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
     constexpr int BT = BT_, PK = 32;
     PackedGemmParams p{};
     const int64_t tm = (g.m + BT - 1) / BT, tn = (g.n + BT - 1) / BT;
@@ -54,7 +54,7 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, bool tri_gri
     const int planes = (NPROD == 3) ? 2 : 1;
     const size_t a_el = (size_t)tm * BT * ktiles * PK, b_el = (size_t)tn * BT * ktiles * PK;
     void* ws = nullptr;
-    hipError_t e = workspace(s, (a_el + b_el) * planes * sizeof(uint16_t), &ws);
+    hipError_t e = workspace(s, 0, (a_el + b_el) * planes * sizeof(uint16_t), &ws);
     if (e != hipSuccess) return e;
     uint16_t* base = (uint16_t*)ws;
     PackParams pa{g.A, g.lda, g.m, g.k, (int)ktiles, base, planes == 2 ? base + a_el : nullptr, g.a_vec};
@@ -84,13 +84,14 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, bool tri_gri
     p.tiles_m = (int)tm;
     p.tiles_n = (int)tn;
     p.tri = g.tri;
-    p.tri_tiles = tri_grid ? 1 : 0;
+    p.tri_tiles = tri_grid;
+    p.krange = g.krange;
     gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)ntile), dim3(64 * WM_ * WN_), 0, s>>>(p);
     return hipGetLastError();
 }
 
 template <int NPROD>
-hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, bool tri_grid, hipStream_t s) {
+hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
     const int64_t t256 = ((g.m + 255) / 256) * ((g.n + 255) / 256);
     if (t256 >= 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s);
     return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s);
@@ -110,7 +111,7 @@ __global__ void scale_kernel(int64_t m, int64_t n, T beta, T* C, int64_t ldc, in
 
 template <typename T>
 GemmParams<T> make_params(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
-                          int64_t ldb, T beta, T* C, int64_t ldc, int tri) {
+                          int64_t ldb, T beta, T* C, int64_t ldc, int tri, int krange) {
     GemmParams<T> p{};
     p.m = m;
     p.n = n;
@@ -124,6 +125,7 @@ GemmParams<T> make_params(int64_t m, int64_t n, int64_t k, T alpha, const T* A, 
     p.alpha = alpha;
     p.beta = beta;
     p.tri = tri;
+    p.krange = krange;
     p.a_vec = vec_ok(A, lda);
     p.b_vec = vec_ok(B, ldb);
     return p;
@@ -144,13 +146,13 @@ template hipError_t scale<double>(int64_t, int64_t, double, double*, int64_t, hi
 
 hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
                  const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s, int tri,
-                 bool tri_grid) {
+                 int tri_grid, int krange) {
     if (m <= 0 || n <= 0) return hipSuccess;
     if (alpha == 0.0f || k == 0) return scale<float>(m, n, beta, C, ldc, s, tri);
-    GemmParams<float> p = make_params<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+    GemmParams<float> p = make_params<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, krange);
     switch (prec) {
         case PREC_FP32:
-            return launch_fp<float, 128, 128, 16>(at, bt, p, tri_grid, s);
+            return launch_fp<float, float, float, float, 128, 128, 16>(at, bt, p, tri_grid, s);
         case PREC_BF16X3:
             return launch_packed<3>(at, bt, p, tri_grid, s);
         case PREC_BF16:
@@ -162,11 +164,46 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
 
 hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
                  const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s, int tri,
-                 bool tri_grid) {
+                 int tri_grid, int krange) {
     if (m <= 0 || n <= 0) return hipSuccess;
     if (alpha == 0.0 || k == 0) return scale<double>(m, n, beta, C, ldc, s, tri);
-    GemmParams<double> p = make_params<double>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
-    return launch_fp<double, 64, 64, 16>(at, bt, p, tri_grid, s);
+    GemmParams<double> p = make_params<double>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, krange);
+    return launch_fp<double, double, double, double, 64, 64, 16>(at, bt, p, tri_grid, s);
 }
+
+template <typename TA, typename TB, typename TC>
+hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const TA* A, int64_t lda,
+                       const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, hipStream_t s, int shape,
+                       int krange) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (alpha == 0.0 || k == 0) return hipErrorInvalidValue;  // callers never ask for a pure scale here
+    GemmArgs<TA, TB, TC, double> p{};
+    p.m = m;
+    p.n = n;
+    p.k = k;
+    p.A = A;
+    p.lda = lda;
+    p.B = B;
+    p.ldb = ldb;
+    p.C = C;
+    p.ldc = ldc;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.krange = krange;
+    p.a_vec = vec_ok(A, lda);
+    p.b_vec = vec_ok(B, ldb);
+    if (shape == 1) return launch_fp<double, TA, TB, TC, 128, 64, 16>(at, bt, p, 0, s);
+    if (shape == 2) return launch_fp<double, TA, TB, TC, 64, 128, 16>(at, bt, p, 0, s);
+    return launch_fp<double, TA, TB, TC, 64, 64, 16>(at, bt, p, 0, s);
+}
+
+#define MBH_F64ACC(TA, TB, TC)                                                                                     \
+    template hipError_t gemm_f64acc<TA, TB, TC>(bool, bool, int64_t, int64_t, int64_t, double, const TA*, int64_t, \
+                                                const TB*, int64_t, double, TC*, int64_t, hipStream_t, int, int);
+MBH_F64ACC(double, float, float)
+MBH_F64ACC(float, double, float)
+MBH_F64ACC(float, float, double)
+MBH_F64ACC(double, double, double)
+#undef MBH_F64ACC
 
 }  // namespace mbh

@@ -62,17 +62,18 @@ __device__ inline int acc_col(int r, int lane) {
     return 4 * r + (lane >> 4);
 }
 
-// Global -> register staging of one operand panel (ROWS x BK of op(X)).
+// Global -> register staging of one operand panel (ROWS x BK of op(X)),
+// converting the storage type TIN to the MFMA type T on load.
 // KCONTIG: element (row, kk) lives at src[kk + row*ld]; else src[row + kk*ld].
-template <typename T, bool KCONTIG, int ROWS, int BK, int NT>
+template <typename T, typename TIN, bool KCONTIG, int ROWS, int BK, int NT>
 struct PanelLoader {
-    static constexpr int VEC = 16 / sizeof(T);  // elements per 16-byte load
+    static constexpr int VEC = 16 / sizeof(TIN);  // elements per 16-byte load
     static constexpr int NV = ROWS * BK / VEC / NT;
     static_assert(NV * VEC * NT == ROWS * BK, "panel not divisible by the thread block");
-    typedef T vec_t __attribute__((ext_vector_type(VEC)));
-    vec_t r[NV];
+    typedef TIN vin_t __attribute__((ext_vector_type(VEC)));
+    T r[NV][VEC];
 
-    __device__ inline void load(const T* __restrict__ src, int64_t ld, int64_t row0, int64_t nrows, int64_t k0,
+    __device__ inline void load(const TIN* __restrict__ src, int64_t ld, int64_t row0, int64_t nrows, int64_t k0,
                                 int64_t ktot, bool vec_ok, int tid) {
         const bool full = vec_ok && (row0 + ROWS <= nrows) && (k0 + BK <= ktot);
 #pragma unroll
@@ -87,15 +88,17 @@ struct PanelLoader {
                 row = (idx % (ROWS / VEC)) * VEC;
             }
             if (full) {
-                const T* p = KCONTIG ? src + (k0 + kk) + (row0 + row) * ld : src + (row0 + row) + (k0 + kk) * ld;
-                r[v] = *reinterpret_cast<const vec_t*>(p);
+                const TIN* p = KCONTIG ? src + (k0 + kk) + (row0 + row) * ld : src + (row0 + row) + (k0 + kk) * ld;
+                const vin_t x = *reinterpret_cast<const vin_t*>(p);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) r[v][e] = (T)x[e];
             } else {
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) {
                     const int64_t gr = row0 + row + (KCONTIG ? 0 : e);
                     const int64_t gk = k0 + kk + (KCONTIG ? e : 0);
                     T x = T(0);
-                    if (gr < nrows && gk < ktot) x = KCONTIG ? src[gk + gr * ld] : src[gr + gk * ld];
+                    if (gr < nrows && gk < ktot) x = (T)(KCONTIG ? src[gk + gr * ld] : src[gr + gk * ld]);
                     r[v][e] = x;
                 }
             }
@@ -116,42 +119,51 @@ struct PanelLoader {
             } else {
                 const int kk = idx / (ROWS / VEC);
                 const int row = (idx % (ROWS / VEC)) * VEC;
-                *reinterpret_cast<vec_t*>(lds + kk * S + row) = r[v];
+                if (sizeof(T) * VEC == 16) {
+                    typedef T vt __attribute__((ext_vector_type(VEC)));
+                    vt x;
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) x[e] = r[v][e];
+                    *reinterpret_cast<vt*>(lds + kk * S + row) = x;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) lds[kk * S + row + e] = r[v][e];
+                }
             }
         }
     }
 };
 
-// LDS row stride for a panel: a multiple of the vector width when rows are
-// written as vectors; rows + 2 (fp32) / rows + 1 (fp64) when written as
-// scattered scalars (4 consecutive kk from 8 rows land in distinct banks).
-template <typename T, bool KCONTIG, int ROWS>
+// LDS row stride for a panel: rows + the vector width when rows are written as
+// vectors; rows + 2 (fp32) / rows + 1 (fp64) when written as scattered scalars
+// (4 consecutive kk from 8 rows land in distinct banks).
+template <typename T, typename TIN, bool KCONTIG, int ROWS>
 constexpr int lds_stride() {
-    return KCONTIG ? ROWS + (sizeof(T) == 4 ? 2 : 1) : ROWS + 16 / (int)sizeof(T);
+    return KCONTIG ? ROWS + (sizeof(T) == 4 ? 2 : 1) : ROWS + 16 / (int)sizeof(TIN);
 }
 
-template <typename T, bool AT, bool BT, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void gemm_fp_kernel(GemmParams<T> p) {
+template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p) {
     typedef MfmaFp<T> M;
     constexpr int NT = 256;
     constexpr int WM = BM / 2, WN = BN / 2;  // 2 x 2 waves
     constexpr int TM = WM / M::MT, TN = WN / M::MT;
     constexpr bool AK = AT;    // A 'T': op(A)(i,k) = A[k + i*lda] is k-contiguous
     constexpr bool BK_ = !BT;  // B 'N': op(B)(k,j) = B[k + j*ldb] is k-contiguous
-    constexpr int SA = lds_stride<T, AK, BM>();
-    constexpr int SB = lds_stride<T, BK_, BN>();
+    constexpr int SA = lds_stride<T, TA, AK, BM>();
+    constexpr int SB = lds_stride<T, TB, BK_, BN>();
     __shared__ __attribute__((aligned(16))) T As[2][BK * SA];
     __shared__ __attribute__((aligned(16))) T Bs[2][BK * SB];
 
     int tm, tn;
-    if (p.tri_tiles) tri_tile_coords((int)blockIdx.x, tm, tn);
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
     else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
 
-    PanelLoader<T, AK, BM, BK, NT> la;
-    PanelLoader<T, BK_, BN, BK, NT> lb;
+    PanelLoader<T, TA, AK, BM, BK, NT> la;
+    PanelLoader<T, TB, BK_, BN, BK, NT> lb;
     typename M::acc_t acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -160,18 +172,21 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmParams<T> p) {
 #pragma unroll
             for (int r = 0; r < M::NACC; ++r) acc[i][j][r] = T(0);
 
-    const int nk = (int)((p.k + BK - 1) / BK);
-    la.load(p.A, p.lda, m0, p.m, 0, p.k, p.a_vec, tid);
-    lb.load(p.B, p.ldb, n0, p.n, 0, p.k, p.b_vec, tid);
-    la.template store<SA>(As[0], tid);
-    lb.template store<SB>(Bs[0], tid);
+    int kt0, kt1;
+    kt_range(p.krange, tm, tn, BM, BN, BK, (int)((p.k + BK - 1) / BK), kt0, kt1);
+    if (kt0 < kt1) {
+        la.load(p.A, p.lda, m0, p.m, (int64_t)kt0 * BK, p.k, p.a_vec, tid);
+        lb.load(p.B, p.ldb, n0, p.n, (int64_t)kt0 * BK, p.k, p.b_vec, tid);
+        la.template store<SA>(As[0], tid);
+        lb.template store<SB>(Bs[0], tid);
+    }
     __syncthreads();
 
     constexpr int G = 64 / M::MT;  // lane groups = k values per instruction
     const int lrow = lane % M::MT, lgrp = lane / M::MT;
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) {
+    for (int kt = kt0; kt < kt1; ++kt) {
+        const int cur = (kt - kt0) & 1;
+        if (kt + 1 < kt1) {
             la.load(p.A, p.lda, m0, p.m, (int64_t)(kt + 1) * BK, p.k, p.a_vec, tid);
             lb.load(p.B, p.ldb, n0, p.n, (int64_t)(kt + 1) * BK, p.k, p.b_vec, tid);
         }
@@ -189,7 +204,7 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmParams<T> p) {
 #pragma unroll
                 for (int j = 0; j < TN; ++j) acc[i][j] = M::mma(b[j], a[i], acc[i][j]);
         }
-        if (kt + 1 < nk) {
+        if (kt + 1 < kt1) {
             la.template store<SA>(As[cur ^ 1], tid);
             lb.template store<SB>(Bs[cur ^ 1], tid);
         }
@@ -209,10 +224,10 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmParams<T> p) {
             for (int r = 0; r < M::NACC; ++r) {
                 const int64_t col = n0 + wn0 + j * M::MT + acc_col<M::MT>(r, lane);
                 if (col < p.n && tri_keep(p.tri, row, col)) {
-                    T* cp = p.C + row + col * p.ldc;
+                    TC* cp = p.C + row + col * p.ldc;
                     T v = alpha * acc[i][j][r];
-                    if (beta != T(0)) v += beta * *cp;
-                    *cp = v;
+                    if (beta != T(0)) v += beta * (T)*cp;
+                    *cp = (TC)v;
                 }
             }
         }

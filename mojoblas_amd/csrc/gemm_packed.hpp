@@ -37,7 +37,8 @@ struct PackedGemmParams {
     float alpha, beta;
     int tiles_m, tiles_n;
     int tri;
-    int tri_tiles;
+    int tri_tiles;  // 0 full grid, 1/2 lower/upper block triangle
+    int krange;     // KRange (k-tiles visited per tile)
 };
 
 __device__ inline int lds_swz(int row) { return ((row >> 1) ^ (row >> 2)) & 3; }
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     __shared__ uint4 Bs[2][NPL][BN * 4];
 
     int tm, tn;
-    if (p.tri_tiles) tri_tile_coords((int)blockIdx.x, tm, tn);
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
     else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
@@ -107,17 +108,20 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    const int nk = p.ktiles;
-    gload(0);
-    lstore(0);
+    int kt0, kt1;
+    kt_range(p.krange, tm, tn, BM, BN, BK, p.ktiles, kt0, kt1);
+    if (kt0 < kt1) {
+        gload(kt0);
+        lstore(0);
+    }
     __syncthreads();
 
     // Per-lane fragment addressing: row offset lane&31, chunk 2*ks + lane/32,
     // swizzle depends only on lane bits 1..3 (row offsets are multiples of 32).
     const int lrow = lane & 31, lgrp = lane >> 5, lsw = lds_swz(lrow);
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
+    for (int kt = kt0; kt < kt1; ++kt) {
+        const int cur = (kt - kt0) & 1;
+        if (kt + 1 < kt1) gload(kt + 1);
 #pragma unroll
         for (int ks = 0; ks < BK / 16; ++ks) {
             const int ch = (2 * ks + lgrp) ^ lsw;
@@ -145,7 +149,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
                 }
         }
-        if (kt + 1 < nk) lstore(cur ^ 1);
+        if (kt + 1 < kt1) lstore(cur ^ 1);
         __syncthreads();
     }
 

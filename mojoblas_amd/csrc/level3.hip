@@ -1,0 +1,375 @@
+// Level-3 routines recast onto the MFMA gemm kernels.
+//
+//   trsm  recursive blocked solve: the triangular dimension is halved (at
+//         multiples of NB = 128) until a leaf is one 128x128 diagonal block;
+//         a leaf applies the fp64 inverse of that block with an fp64-MFMA
+//         product, every off-diagonal update is one large gemm on the fp32
+//         (or BF16x3) matrix cores.  For fp32 data one step of iterative
+//         refinement follows: R = alpha*B - op(A)*X formed in fp64 from the
+//         fp32 operands, op(A)*D = R solved the same way, X += D.
+//   trmm  op(A) materialized as a full triangle (zeros, unit diagonal), one
+//         gemm whose tiles visit only the k-tiles where op(A) is nonzero, into
+//         scratch, then copied back (trmm is in place).
+//   syrk  gemm(A, A^T) over the lower/upper block-triangle tile grid with a
+//         triangle-masked store.  syr2k: two such gemms.
+//   symm  A symmetrized into scratch, then one gemm.
+// Semantics follow the reference's docstrings (BLAS): src/level3/trsm.mojo,
+// trmm.mojo, syrk.mojo, syr2k.mojo, symm.mojo.
+#include <climits>
+
+#include "common.hpp"
+#include "dispatch.hpp"
+#include "level3.hpp"
+
+namespace mbh {
+
+namespace {
+
+constexpr int NB = 128;  // diagonal block of the trsm leaves
+
+inline dim3 grid_for_elems(int64_t count) {
+    int64_t b = (count + 255) / 256;
+    if (b > 16384) b = 16384;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+// out(i,j) = element (i,j) of the `upper`/lower triangle of A (zeros outside,
+// 1 on the diagonal when unit), n x n.
+template <typename Tin, typename Tout>
+__global__ void tri_copy_kernel(int64_t n, const Tin* A, int64_t lda, int upper, int unit, Tout* out,
+                                int64_t ldo) {
+    const int64_t total = n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % n, j = idx / n;
+        Tout v = Tout(0);
+        if (i == j) v = unit ? Tout(1) : (Tout)A[i + j * lda];
+        else if (upper ? (i < j) : (i > j)) v = (Tout)A[i + j * lda];
+        out[i + j * ldo] = v;
+    }
+}
+
+// out = A with the unreferenced triangle mirrored from the stored one.
+template <typename T>
+__global__ void sym_copy_kernel(int64_t n, const T* A, int64_t lda, int upper, T* out, int64_t ldo) {
+    const int64_t total = n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % n, j = idx / n;
+        const bool stored = upper ? (i <= j) : (i >= j);
+        out[i + j * ldo] = stored ? A[i + j * lda] : A[j + i * lda];
+    }
+}
+
+// out(i,j) = (Tout)(alpha * in(i,j)), m x n.
+template <typename Tin, typename Tout>
+__global__ void copy_kernel(int64_t m, int64_t n, const Tin* in, int64_t ldi, Tout* out, int64_t ldo,
+                            double alpha) {
+    const int64_t total = m * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % m, j = idx / m;
+        const Tin x = in[i + j * ldi];
+        out[i + j * ldo] = (alpha == 1.0) ? (Tout)x : (Tout)(alpha * (double)x);
+    }
+}
+
+// out(i,j) += d(i,j)
+template <typename T>
+__global__ void add_kernel(int64_t m, int64_t n, const T* d, int64_t ldd, T* out, int64_t ldo) {
+    const int64_t total = m * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % m, j = idx / m;
+        out[i + j * ldo] += d[i + j * ldd];
+    }
+}
+
+// Inverses of the NB x NB diagonal blocks of op(A) (op(A) is eff_lower /
+// upper triangular, trans = op is A^T), computed in fp64: one workgroup of NB
+// threads per block, thread j solving op(A)_bb x = e_j by substitution.  The
+// running columns live in LDS ([row][col], col contiguous, so the NB threads
+// read one row conflict-free); row i of the block is staged in LDS once per
+// step and read as a broadcast.  Output: inv + b*NB*NB, column-major, ld NB,
+// zero-padded for a short last block.
+template <typename Tin>
+__global__ __launch_bounds__(NB) void tri_inv_kernel(int64_t m, const Tin* A, int64_t lda, int eff_lower, int trans,
+                                                     int unit, double* inv) {
+    __shared__ double x[NB][NB + 1];
+    __shared__ double lrow[NB];
+    const int b = blockIdx.x, j = threadIdx.x;
+    const int64_t r0 = (int64_t)b * NB;
+    const int nb = (int)((m - r0 < NB) ? (m - r0) : NB);
+    auto opA = [&](int i, int k) -> double {
+        const int64_t gi = r0 + i, gk = r0 + k;
+        return trans ? (double)A[gk + gi * lda] : (double)A[gi + gk * lda];
+    };
+    for (int i = 0; i < NB; ++i) x[i][j] = 0.0;
+    for (int step = 0; step < nb; ++step) {
+        const int i = eff_lower ? step : nb - 1 - step;
+        __syncthreads();
+        if (j < nb) {
+            const bool in_tri = eff_lower ? (j <= i) : (j >= i);
+            lrow[j] = (j == i) ? (unit ? 1.0 : opA(i, i)) : (in_tri ? opA(i, j) : 0.0);
+        }
+        __syncthreads();
+        if (j < nb) {
+            double s = (i == j) ? 1.0 : 0.0;
+            if (eff_lower) {
+                for (int k = 0; k < i; ++k) s -= lrow[k] * x[k][j];
+            } else {
+                for (int k = i + 1; k < nb; ++k) s -= lrow[k] * x[k][j];
+            }
+            x[i][j] = s / lrow[i];
+        }
+    }
+    __syncthreads();
+    double* out = inv + (size_t)b * NB * NB;
+    for (int i = 0; i < NB; ++i) out[i + (size_t)j * NB] = (i < nb && j < nb) ? x[i][j] : 0.0;
+}
+
+template <typename T>
+hipError_t xgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                 int64_t ldb, T beta, T* C, int64_t ldc, int prec, hipStream_t s, int tri = 0, int tri_grid = 0,
+                 int krange = 0);
+template <>
+hipError_t xgemm<float>(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                        const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s,
+                        int tri, int tri_grid, int krange) {
+    return sgemm(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, s, tri, tri_grid, krange);
+}
+template <>
+hipError_t xgemm<double>(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                         int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int, hipStream_t s,
+                         int tri, int tri_grid, int krange) {
+    return dgemm(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, tri_grid, krange);
+}
+
+#define MBH_TRY(x)                          \
+    do {                                    \
+        hipError_t e__ = (x);               \
+        if (e__ != hipSuccess) return e__;  \
+    } while (0)
+
+// In-place solve of op(A) X = B (left) or X op(A) = B (right) for B already
+// scaled by alpha; `inv` holds the fp64 inverses of op(A)'s diagonal blocks.
+template <typename T>
+struct TriSolver {
+    bool left, eff_lower, trans;
+    int64_t m, n;
+    const T* A;
+    int64_t lda;
+    const double* inv;
+    int prec;
+    hipStream_t s;
+
+    // Pointer to op(A)(i, k) for a gemm operand with trans flag `trans`.
+    const T* opA(int64_t i, int64_t k) const { return trans ? A + k + i * lda : A + i + k * lda; }
+
+    static int64_t split(int64_t len) {
+        int64_t h = (len / 2 + NB - 1) / NB * NB;
+        return h >= len ? len - NB : h;
+    }
+
+    hipError_t leaf(T* B, int64_t ldb, int64_t r0, int64_t len) const {
+        const double* iv = inv + (size_t)(r0 / NB) * NB * NB;
+        if (left)  // X_b = inv_b * B_b  (len x n), one 128-row tile row: in place is safe
+            return gemm_f64acc<double, T, T>(false, false, len, n, len, 1.0, iv, NB, B + r0, ldb, 0.0, B + r0, ldb, s,
+                                             1);
+        // X_b = B_b * inv_b  (m x len), one 128-column tile column: in place is safe
+        return gemm_f64acc<T, double, T>(false, false, m, len, len, 1.0, B + r0 * ldb, ldb, iv, NB, 0.0,
+                                         B + r0 * ldb, ldb, s, 2);
+    }
+
+    hipError_t solve(T* B, int64_t ldb, int64_t r0, int64_t len) const {
+        if (len <= NB) return leaf(B, ldb, r0, len);
+        const int64_t h = split(len), r1 = r0 + h, len2 = len - h;
+        if (left) {
+            if (eff_lower) {  // X1 first; B2 -= op(A)21 X1
+                MBH_TRY(solve(B, ldb, r0, h));
+                MBH_TRY(xgemm<T>(trans, false, len2, n, h, T(-1), opA(r1, r0), lda, B + r0, ldb, T(1), B + r1, ldb,
+                                 prec, s));
+                return solve(B, ldb, r1, len2);
+            }
+            MBH_TRY(solve(B, ldb, r1, len2));  // X2 first; B1 -= op(A)12 X2
+            MBH_TRY(xgemm<T>(trans, false, h, n, len2, T(-1), opA(r0, r1), lda, B + r1, ldb, T(1), B + r0, ldb, prec,
+                             s));
+            return solve(B, ldb, r0, h);
+        }
+        if (!eff_lower) {  // X op(A), op(A) upper: X1 first; B2 -= X1 op(A)12
+            MBH_TRY(solve(B, ldb, r0, h));
+            MBH_TRY(xgemm<T>(false, trans, m, len2, h, T(-1), B + r0 * ldb, ldb, opA(r0, r1), lda, T(1),
+                             B + r1 * ldb, ldb, prec, s));
+            return solve(B, ldb, r1, len2);
+        }
+        MBH_TRY(solve(B, ldb, r1, len2));  // op(A) lower: X2 first; B1 -= X2 op(A)21
+        MBH_TRY(xgemm<T>(false, trans, m, h, len2, T(-1), B + r1 * ldb, ldb, opA(r1, r0), lda, T(1), B + r0 * ldb,
+                         ldb, prec, s));
+        return solve(B, ldb, r0, h);
+    }
+};
+
+template <typename T>
+hipError_t launch_scale_tri(int64_t m, int64_t n, T beta, T* C, int64_t ldc, hipStream_t s, int tri) {
+    return scale<T>(m, n, beta, C, ldc, s, tri);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// trsm (trsm.mojo:20-275, BLAS semantics)
+
+template <typename T>
+hipError_t trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t n, T alpha, const T* A, int64_t lda,
+                T* B, int64_t ldb, int prec, int refine, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (alpha == T(0)) return scale<T>(m, n, T(0), B, ldb, s);
+    const bool left = (side == 'L' || side == 'l');
+    const bool upper = (uplo == 'U' || uplo == 'u');
+    const bool trans = !(transa == 'N' || transa == 'n');
+    const bool unit = (diag == 'U' || diag == 'u');
+    const bool eff_lower = (!upper) != trans;
+    const int64_t na = left ? m : n;
+    const int64_t nblk = (na + NB - 1) / NB;
+    const bool do_ir = (sizeof(T) == 4) && refine > 0;
+
+    // scratch: inverses | (IR) fp64 alpha*B0 | fp32 triangle | fp32 correction
+    size_t inv_b = (size_t)nblk * NB * NB * sizeof(double);
+    size_t b0_b = do_ir ? (size_t)m * n * sizeof(double) : 0;
+    size_t tri_b = do_ir ? (size_t)na * na * sizeof(T) : 0;
+    size_t d_b = do_ir ? (size_t)m * n * sizeof(T) : 0;
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    void* ws = nullptr;
+    MBH_TRY(workspace(s, 1, al(inv_b) + al(b0_b) + al(tri_b) + al(d_b), &ws));
+    char* base = (char*)ws;
+    double* inv = (double*)base;
+    double* b0 = (double*)(base + al(inv_b));
+    T* tri = (T*)(base + al(inv_b) + al(b0_b));
+    T* d = (T*)(base + al(inv_b) + al(b0_b) + al(tri_b));
+
+    tri_inv_kernel<T><<<dim3((unsigned)nblk), dim3(NB), 0, s>>>(na, A, lda, eff_lower ? 1 : 0, trans ? 1 : 0,
+                                                                unit ? 1 : 0, inv);
+    MBH_TRY(hipGetLastError());
+    if (do_ir) {
+        copy_kernel<T, double><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, B, ldb, b0, m, (double)alpha);
+        tri_copy_kernel<T, T><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0, tri,
+                                                                      na);
+        MBH_TRY(hipGetLastError());
+    }
+    if (alpha != T(1)) MBH_TRY(scale<T>(m, n, alpha, B, ldb, s));
+    TriSolver<T> ts{left, eff_lower, trans, m, n, A, lda, inv, prec, s};
+    MBH_TRY(ts.solve(B, ldb, 0, na));
+    if (!do_ir) return hipSuccess;
+
+    // R = alpha*B0 - op(T) X in fp64 (b0 becomes R); op(T) visits only its
+    // nonzero k-tiles.
+    if constexpr (sizeof(T) == 4) {
+        if (left)
+            MBH_TRY((gemm_f64acc<float, float, double>(trans, false, m, n, m, -1.0, tri, na, B, ldb, 1.0, b0, m, s, 0,
+                                                       eff_lower ? KR_LE_ROW : KR_GE_ROW)));
+        else
+            MBH_TRY((gemm_f64acc<float, float, double>(false, trans, m, n, n, -1.0, B, ldb, tri, na, 1.0, b0, m, s, 0,
+                                                       eff_lower ? KR_GE_COL : KR_LE_COL)));
+    }
+    copy_kernel<double, T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, b0, m, d, m, 1.0);
+    MBH_TRY(hipGetLastError());
+    MBH_TRY(ts.solve(d, m, 0, na));
+    add_kernel<T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, d, m, B, ldb);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// trmm (trmm.mojo:20-305, BLAS semantics): B := alpha*op(A)*B or alpha*B*op(A)
+
+template <typename T>
+hipError_t trmm(char side, char uplo, char transa, char diag, int64_t m, int64_t n, T alpha, const T* A, int64_t lda,
+                T* B, int64_t ldb, int prec, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (alpha == T(0)) return scale<T>(m, n, T(0), B, ldb, s);
+    const bool left = (side == 'L' || side == 'l');
+    const bool upper = (uplo == 'U' || uplo == 'u');
+    const bool trans = !(transa == 'N' || transa == 'n');
+    const bool unit = (diag == 'U' || diag == 'u');
+    const bool eff_lower = (!upper) != trans;
+    const int64_t na = left ? m : n;
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t tri_b = (size_t)na * na * sizeof(T), w_b = (size_t)m * n * sizeof(T);
+    void* ws = nullptr;
+    MBH_TRY(workspace(s, 1, al(tri_b) + al(w_b), &ws));
+    T* tri = (T*)ws;
+    T* w = (T*)((char*)ws + al(tri_b));
+    tri_copy_kernel<T, T><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0, tri, na);
+    MBH_TRY(hipGetLastError());
+    if (left)
+        MBH_TRY(xgemm<T>(trans, false, m, n, m, alpha, tri, na, B, ldb, T(0), w, m, prec, s, 0, 0,
+                         eff_lower ? KR_LE_ROW : KR_GE_ROW));
+    else
+        MBH_TRY(xgemm<T>(false, trans, m, n, n, alpha, B, ldb, tri, na, T(0), w, m, prec, s, 0, 0,
+                         eff_lower ? KR_GE_COL : KR_LE_COL));
+    copy_kernel<T, T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, w, m, B, ldb, 1.0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// syrk (syrk.mojo:20-200) and syr2k (syr2k.mojo:20-190)
+
+template <typename T>
+hipError_t syrk(char uplo, char trans, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, T beta, T* C,
+                int64_t ldc, int prec, hipStream_t s) {
+    if (n <= 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return hipSuccess;
+    const bool upper = (uplo == 'U' || uplo == 'u');
+    const bool tr = !(trans == 'N' || trans == 'n');
+    const int tri = upper ? TRI_UPPER : TRI_LOWER;
+    // N: C = A A^T (A n x k); T: C = A^T A (A k x n)
+    return xgemm<T>(tr, !tr, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, prec, s, tri, upper ? 2 : 1);
+}
+
+template <typename T>
+hipError_t syr2k(char uplo, char trans, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                 int64_t ldb, T beta, T* C, int64_t ldc, int prec, hipStream_t s) {
+    if (n <= 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return hipSuccess;
+    const bool upper = (uplo == 'U' || uplo == 'u');
+    const bool tr = !(trans == 'N' || trans == 'n');
+    const int tri = upper ? TRI_UPPER : TRI_LOWER, grid = upper ? 2 : 1;
+    if (alpha == T(0) || k == 0) return scale<T>(n, n, beta, C, ldc, s, tri);
+    MBH_TRY(xgemm<T>(tr, !tr, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, s, tri, grid));
+    return xgemm<T>(tr, !tr, n, n, k, alpha, B, ldb, A, lda, T(1), C, ldc, prec, s, tri, grid);
+}
+
+// ---------------------------------------------------------------------------
+// symm (symm.mojo:20-241)
+
+template <typename T>
+hipError_t symm(char side, char uplo, int64_t m, int64_t n, T alpha, const T* A, int64_t lda, const T* B,
+                int64_t ldb, T beta, T* C, int64_t ldc, int prec, hipStream_t s) {
+    if (m <= 0 || n <= 0 || (alpha == T(0) && beta == T(1))) return hipSuccess;
+    if (alpha == T(0)) return scale<T>(m, n, beta, C, ldc, s);
+    const bool left = (side == 'L' || side == 'l');
+    const bool upper = (uplo == 'U' || uplo == 'u');
+    const int64_t na = left ? m : n;
+    void* ws = nullptr;
+    MBH_TRY(workspace(s, 1, (size_t)na * na * sizeof(T), &ws));
+    T* full = (T*)ws;
+    sym_copy_kernel<T><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, full, na);
+    MBH_TRY(hipGetLastError());
+    if (left) return xgemm<T>(false, false, m, n, m, alpha, full, na, B, ldb, beta, C, ldc, prec, s);
+    return xgemm<T>(false, false, m, n, n, alpha, B, ldb, full, na, beta, C, ldc, prec, s);
+}
+
+#define MBH_L3(T)                                                                                                 \
+    template hipError_t trsm<T>(char, char, char, char, int64_t, int64_t, T, const T*, int64_t, T*, int64_t, int, \
+                                int, hipStream_t);                                                                \
+    template hipError_t trmm<T>(char, char, char, char, int64_t, int64_t, T, const T*, int64_t, T*, int64_t, int, \
+                                hipStream_t);                                                                     \
+    template hipError_t syrk<T>(char, char, int64_t, int64_t, T, const T*, int64_t, T, T*, int64_t, int,          \
+                                hipStream_t);                                                                     \
+    template hipError_t syr2k<T>(char, char, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, T, T*,    \
+                                 int64_t, int, hipStream_t);                                                      \
+    template hipError_t symm<T>(char, char, int64_t, int64_t, T, const T*, int64_t, const T*, int64_t, T, T*,     \
+                                int64_t, int, hipStream_t);
+MBH_L3(float)
+MBH_L3(double)
+#undef MBH_L3
+
+}  // namespace mbh

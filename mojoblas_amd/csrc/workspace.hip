@@ -1,10 +1,10 @@
-// Per-(device, stream) grow-only device workspace for kernels that stage
-// operands (the split-precision pack pass, blocked trsm scratch).  Growing a
+// Grow-only device workspace per (device, stream, slot) for kernels that stage
+// operands (the split-precision pack pass, level-3 driver scratch).  Growing a
 // buffer first synchronises its stream, so work still reading the old buffer
 // has finished before it is freed.
 #include <map>
 #include <mutex>
-#include <utility>
+#include <tuple>
 
 #include "dispatch.hpp"
 
@@ -16,15 +16,15 @@ struct Buf {
     size_t cap = 0;
 };
 std::mutex g_ws_mu;
-std::map<std::pair<int, hipStream_t>, Buf> g_ws;
+std::map<std::tuple<int, hipStream_t, int>, Buf> g_ws;
 }  // namespace
 
-hipError_t workspace(hipStream_t s, size_t bytes, void** out) {
+hipError_t workspace(hipStream_t s, int slot, size_t bytes, void** out) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     std::lock_guard<std::mutex> g(g_ws_mu);
-    Buf& b = g_ws[std::make_pair(dev, s)];
+    Buf& b = g_ws[std::make_tuple(dev, s, slot)];
     if (b.cap < bytes) {
         if (b.p) {
             if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;

@@ -1,0 +1,262 @@
+"""GPU parity of trsm, trmm, syrk, syr2k and symm against the fp64 oracle,
+through the C-ABI (device entries with torch tensors, host entries with numpy).
+
+Bounds:
+  trmm/syrk/syr2k/symm (products): componentwise
+      |X - X_oracle| <= (e_rep + 2 k u)|alpha| P + 4u|beta C|
+    with P the matching |A||B| product (helpers.gemm_bound);
+  trsm: normwise on the solution, max|X - X_oracle| <= tol * max|X_oracle|,
+    tol = 2e-6 for fp32 data (one fp64-residual refinement step brings the
+    fp32/BF16x3 solve to fp32 rounding level), 1e-12 for fp64 data; the test
+    matrices have diagonal U[1,2] and off-diagonal U[-1,1]/n (well conditioned).
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers as H
+
+torch = pytest.importorskip("torch")
+import mojoblas_amd as mb
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "level3_kat.json")))
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+@pytest.mark.parametrize("case", [t for t in KAT["tests"] if t["routine"] != "gemm"], ids=lambda t: t["name"])
+@pytest.mark.parametrize("entry", ["dev", "host"])
+def test_known_answers(case, entry):
+    dt = np.float32 if case["dtype"] == "float32" else np.float64
+    arrays = {k: np.array(v, dtype=dt) for k, v in case["arrays"].items()}
+    if entry == "dev":
+        arrays = {k: dev(v) for k, v in arrays.items()}
+    args = [arrays[a["array"]] if isinstance(a, dict) else a for a in case["args"]]
+    assert getattr(mb, case["routine"])(*args) == 0
+    torch.cuda.synchronize()
+    for e in case["expects"]:
+        got = float(arrays[e["array"]][e["index"]])
+        assert abs(got - e["value"]) <= e["atol"] + e["rtol"] * abs(e["value"]), (case["name"], e, got)
+
+
+def tri_operand(rng, n, upper, dtype, unit_junk=False):
+    """Stored triangular A: diag U[1,2], off-diag U[-1,1]/n in the `upper`
+    triangle, NaN in the other triangle (must never be read)."""
+    a = rng.uniform(-1, 1, (n, n)) / max(n, 1)
+    a[np.diag_indices(n)] = rng.uniform(1, 2, n)
+    mask = np.triu(np.ones((n, n), bool)) if upper else np.tril(np.ones((n, n), bool))
+    a = np.where(mask, a, np.nan)
+    if unit_junk:
+        a[np.diag_indices(n)] = np.nan  # unit diagonal is implied, never read
+    return np.asfortranarray(a).astype(dtype)
+
+
+TRSM_SIZES = [(1, 1), (5, 7), (130, 33), (300, 129), (257, 400)]
+
+
+@pytest.mark.parametrize("side,uplo,ta,diag", list(itertools.product("LR", "UL", "NT", "NU")))
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "f64"])
+def test_trsm_vs_oracle(side, uplo, ta, diag, prec):
+    rng = np.random.default_rng(21)
+    dt = np.float64 if prec == "f64" else np.float32
+    for m, n in TRSM_SIZES:
+        na = m if side == "L" else n
+        a = tri_operand(rng, na, uplo == "U", dt, unit_junk=(diag == "U"))
+        b = np.asfortranarray(rng.uniform(-1, 1, (m, n))).astype(dt)
+        alpha = 1.5
+        want = b.astype(np.float64).ravel(order="F").copy()
+        assert oracle.dtrsm(side, uplo, ta, diag, m, n, alpha, a.astype(np.float64).ravel(order="F"), na, want, m) == 0
+        A, B = dev(a.ravel(order="F")), dev(b.ravel(order="F"))
+        kw = {} if prec == "f64" else {"precision": prec}
+        assert mb.trsm(side, uplo, ta, diag, m, n, alpha, A, na, B, m, **kw) == 0
+        got = B.cpu().numpy().astype(np.float64)
+        tol = 1e-12 if prec == "f64" else 2e-6
+        err = np.abs(got - want).max() / max(np.abs(want).max(), 1e-30)
+        assert err <= tol, (m, n, side, uplo, ta, diag, prec, err)
+
+
+def test_trsm_refinement_reaches_fp32_level():
+    """Without refinement the BF16x3 updates leave ~1e-6-level error; one
+    fp64-residual step must bring it to fp32 rounding level."""
+    rng = np.random.default_rng(22)
+    m, n = 1024, 256
+    a = tri_operand(rng, m, False, np.float32)
+    b = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    want = b.astype(np.float64).ravel(order="F").copy()
+    oracle.dtrsm("L", "L", "N", "N", m, n, 1.0, a.astype(np.float64).ravel(order="F"), m, want, m)
+    errs = {}
+    for refine in (0, 1):
+        A, B = dev(a.ravel(order="F")), dev(np.asfortranarray(b).ravel(order="F"))
+        assert mb.trsm("L", "L", "N", "N", m, n, 1.0, A, m, B, m, precision="bf16x3", refine=refine) == 0
+        errs[refine] = np.abs(B.cpu().numpy() - want).max() / np.abs(want).max()
+    assert errs[1] <= 2e-7, errs
+    assert errs[1] <= errs[0], errs
+
+
+@pytest.mark.parametrize("side,uplo,ta,diag", list(itertools.product("LR", "UL", "NT", "NU")))
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "f64"])
+def test_trmm_vs_oracle(side, uplo, ta, diag, prec):
+    rng = np.random.default_rng(23)
+    dt = np.float64 if prec == "f64" else np.float32
+    for m, n in [(1, 1), (7, 5), (140, 300), (300, 129)]:
+        na = m if side == "L" else n
+        a = tri_operand(rng, na, uplo == "U", dt, unit_junk=(diag == "U"))
+        b = np.asfortranarray(rng.uniform(-1, 1, (m, n))).astype(dt)
+        alpha = -0.75
+        want = b.astype(np.float64).ravel(order="F").copy()
+        a64 = a.astype(np.float64)
+        assert oracle.dtrmm(side, uplo, ta, diag, m, n, alpha, a64.ravel(order="F"), na, want, m) == 0
+        A, B = dev(a.ravel(order="F")), dev(b.ravel(order="F"))
+        kw = {} if prec == "f64" else {"precision": prec}
+        assert mb.trmm(side, uplo, ta, diag, m, n, alpha, A, na, B, m, **kw) == 0
+        got = B.cpu().numpy().astype(np.float64)
+        t = np.nan_to_num(a64)
+        if diag == "U":
+            t[np.diag_indices(na)] = 1.0
+        t = t if ta == "N" else t.T
+        bb = np.abs(b.astype(np.float64))
+        p = (np.abs(t) @ bb) if side == "L" else (bb @ np.abs(t))
+        if prec == "f64":
+            bound = 4 * na * 2.0 ** -53 * abs(alpha) * p + 1e-300
+        else:
+            bound = H.gemm_bound(prec, na, alpha, p, 0.0, 0.0)
+        err = np.abs(got.reshape(n, m).T - want.reshape(n, m).T)
+        assert (err <= bound).all(), (m, n, side, uplo, ta, diag, prec, float((err / bound).max()))
+
+
+@pytest.mark.parametrize("uplo,trans", list(itertools.product("UL", "NT")))
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "f64"])
+def test_syrk_syr2k_vs_oracle(uplo, trans, prec):
+    rng = np.random.default_rng(24)
+    dt = np.float64 if prec == "f64" else np.float32
+    for n, k in [(1, 1), (9, 4), (200, 77), (300, 260)]:
+        ar, ac = (n, k) if trans == "N" else (k, n)
+        a = np.asfortranarray(rng.uniform(-1, 1, (ar, ac))).astype(dt)
+        b = np.asfortranarray(rng.uniform(-1, 1, (ar, ac))).astype(dt)
+        tri = np.triu(np.ones((n, n), bool)) if uplo == "U" else np.tril(np.ones((n, n), bool))
+        c0 = rng.uniform(-1, 1, (n, n))
+        c0[~tri] = np.nan  # the other triangle is neither read nor written
+        c0 = np.asfortranarray(c0).astype(dt)
+        for routine in ("syrk", "syr2k"):
+            alpha, beta = 1.25, 0.5
+            want = np.nan_to_num(c0.astype(np.float64)).ravel(order="F")
+            af, bf = a.astype(np.float64).ravel(order="F"), b.astype(np.float64).ravel(order="F")
+            if routine == "syrk":
+                assert oracle.dsyrk(uplo, trans, n, k, alpha, af, ar, beta, want, n) == 0
+            else:
+                assert oracle.dsyr2k(uplo, trans, n, k, alpha, af, ar, bf, ar, beta, want, n) == 0
+            C = dev(c0.ravel(order="F"))
+            kw = {} if prec == "f64" else {"precision": prec}
+            if routine == "syrk":
+                assert mb.syrk(uplo, trans, n, k, alpha, dev(a.ravel(order="F")), ar, beta, C, n, **kw) == 0
+            else:
+                assert mb.syr2k(uplo, trans, n, k, alpha, dev(a.ravel(order="F")), ar, dev(b.ravel(order="F")), ar,
+                                beta, C, n, **kw) == 0
+            got = C.cpu().numpy().astype(np.float64).reshape(n, n).T
+            w = want.reshape(n, n).T
+            A64 = np.abs(a.astype(np.float64) if trans == "N" else a.astype(np.float64).T)
+            B64 = np.abs(b.astype(np.float64) if trans == "N" else b.astype(np.float64).T)
+            p = A64 @ A64.T if routine == "syrk" else A64 @ B64.T + B64 @ A64.T
+            cabs = np.nan_to_num(np.abs(c0.astype(np.float64)))
+            if prec == "f64":
+                bound = 4 * max(k, 1) * 2.0 ** -53 * (alpha * p + beta * cabs) + 1e-300
+            else:
+                bound = H.gemm_bound(prec, 2 * k, alpha, p, beta, cabs)
+            err = np.abs(got - w)
+            assert (err[tri] <= bound[tri]).all(), (routine, n, k, uplo, trans, prec, float((err / bound)[tri].max()))
+            assert np.isnan(got[~tri]).all(), (routine, "other triangle written")
+
+
+@pytest.mark.parametrize("side,uplo", list(itertools.product("LR", "UL")))
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "f64"])
+def test_symm_vs_oracle(side, uplo, prec):
+    rng = np.random.default_rng(25)
+    dt = np.float64 if prec == "f64" else np.float32
+    for m, n in [(1, 1), (6, 9), (200, 130), (129, 300)]:
+        na = m if side == "L" else n
+        s = rng.uniform(-1, 1, (na, na))
+        s = s + s.T
+        stored = s.copy()
+        stored[(np.tril(np.ones((na, na)), -1) if uplo == "U" else np.triu(np.ones((na, na)), 1)) > 0] = np.nan
+        a = np.asfortranarray(stored).astype(dt)
+        b = np.asfortranarray(rng.uniform(-1, 1, (m, n))).astype(dt)
+        b[0, 0] = 0.0  # exact zero in B (the reference's zero-skip must not matter)
+        c = np.asfortranarray(rng.uniform(-1, 1, (m, n))).astype(dt)
+        alpha, beta = -2.0, 0.5
+        want = c.astype(np.float64).ravel(order="F").copy()
+        assert oracle.dsymm(side, uplo, m, n, alpha, a.astype(np.float64).ravel(order="F"), na,
+                            b.astype(np.float64).ravel(order="F"), m, beta, want, m) == 0
+        C = dev(c.ravel(order="F"))
+        kw = {} if prec == "f64" else {"precision": prec}
+        assert mb.symm(side, uplo, m, n, alpha, dev(a.ravel(order="F")), na, dev(b.ravel(order="F")), m, beta, C, m,
+                       **kw) == 0
+        got = C.cpu().numpy().astype(np.float64).reshape(n, m).T
+        sym = np.abs(s.astype(dt).astype(np.float64))
+        bb = np.abs(b.astype(np.float64))
+        p = sym @ bb if side == "L" else bb @ sym
+        if prec == "f64":
+            bound = 4 * na * 2.0 ** -53 * (2 * p + 0.5 * np.abs(c)) + 1e-300
+        else:
+            bound = H.gemm_bound(prec, na, alpha, p, beta, c.astype(np.float64))
+        err = np.abs(got - want.reshape(n, m).T)
+        assert (err <= bound).all(), (m, n, side, uplo, prec, float((err / bound).max()))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_config3_strsm_4096(prec):
+    """strsm L,L,N,N m=n=4096 (config size): A lower, off-diagonal U[-1,1]/n,
+    diagonal U[1,2], B U[-1,1].  Checked on the fp64 residual of the whole
+    solution and on 16 columns against the oracle."""
+    m = n = 4096
+    g = torch.Generator(device="cuda").manual_seed(31)
+    a = (torch.rand(m, m, device="cuda", generator=g, dtype=torch.float64) * 2 - 1) / m
+    a = torch.tril(a, -1) + torch.diag(1 + torch.rand(m, device="cuda", generator=g, dtype=torch.float64))
+    A = a.t().contiguous().float().reshape(-1)  # column-major buffer
+    b = torch.rand(n, m, device="cuda", generator=g) * 2 - 1  # row j = column j of B
+    B = b.reshape(-1).clone()
+    assert mb.trsm("L", "L", "N", "N", m, n, 1.0, A, m, B, m, precision=prec) == 0
+    torch.cuda.synchronize()
+    X = B.view(n, m).t().double()
+    L = A.view(m, m).t().double()
+    res = (L @ X - b.t().double()).abs().max().item()
+    assert res <= 1e-6, res
+    cols = np.arange(0, n, n // 16)
+    a_h = A.cpu().numpy().astype(np.float64)
+    want = b[cols].cpu().numpy().astype(np.float64).ravel()  # 16 columns, ld = m
+    oracle.dtrsm("L", "L", "N", "N", m, len(cols), 1.0, a_h, m, want, m)
+    got = X[:, torch.from_numpy(cols).cuda()].t().cpu().numpy().ravel()
+    assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_config3_ssyrk_8192x2048(prec):
+    """ssyrk L,N n=8192 k=2048, A N(0,1), alpha=1, beta=0, priced on 2048
+    sampled lower-triangle entries by the oracle; upper triangle untouched."""
+    n, k = 8192, 2048
+    g = torch.Generator(device="cuda").manual_seed(32)
+    A = torch.randn(n * k, device="cuda", generator=g)
+    C = torch.full((n * n,), float("nan"), device="cuda")
+    assert mb.syrk("L", "N", n, k, 1.0, A, n, 0.0, C, n, precision=prec) == 0
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(33)
+    i = rng.integers(0, n, 2048)
+    j = rng.integers(0, n, 2048)
+    rows, cols = np.maximum(i, j), np.minimum(i, j)
+    a = A.cpu().numpy()
+    want = oracle.sgemm_sample("N", "T", k, 1.0, a, n, a, n, 0.0, None, rows, cols)
+    absab = oracle.sgemm_sample_abs("N", "T", k, a, n, a, n, rows, cols)
+    c = C.cpu().numpy()
+    got = c[rows + cols * n].astype(np.float64)
+    bound = H.gemm_bound(prec, k, 1.0, absab, 0.0, 0.0)
+    assert (np.abs(got - want) <= bound).all()
+    up_r, up_c = np.minimum(i, j), np.maximum(i, j)
+    strict = up_r < up_c
+    assert np.isnan(c[up_r[strict] + up_c[strict] * n]).all()
