@@ -43,6 +43,57 @@ struct PackedGemmParams {
 
 __device__ inline int lds_swz(int row) { return ((row >> 1) ^ (row >> 2)) & 3; }
 
+// Epilogue of a wave's TM x TN block of 32x32 accumulators holding C^T
+// (register r of lane l is C(row0 + 32i + l%32, col0 + 32j + acc_col(r, l))).
+// Interior blocks without a triangle mask take a branch-free path.
+template <int TM, int TN>
+__device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* C, int64_t ldc, int64_t m, int64_t n,
+                                      int64_t row0, int64_t col0, int lane, float alpha, float beta, int tri) {
+    const int lrow = lane & 31;
+    const bool interior = tri == TRI_NONE && row0 + 32 * TM <= m && col0 + 32 * TN <= n;
+    if (interior) {
+        float* cb = C + (row0 + lrow) + col0 * ldc;
+        if (beta == 0.0f) {
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        cb[i * 32 + (int64_t)(j * 32 + acc_col<32>(r, lane)) * ldc] = alpha * acc[i][j][r];
+        } else {
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        float* cp = cb + i * 32 + (int64_t)(j * 32 + acc_col<32>(r, lane)) * ldc;
+                        *cp = alpha * acc[i][j][r] + beta * *cp;
+                    }
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int64_t row = row0 + i * 32 + lrow;
+        if (row >= m) continue;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t col = col0 + j * 32 + acc_col<32>(r, lane);
+                if (col < n && tri_keep(tri, row, col)) {
+                    float* cp = C + row + col * ldc;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.0f) v += beta * *cp;
+                    *cp = v;
+                }
+            }
+        }
+    }
+}
+
 template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(PackedGemmParams p) {
     constexpr int BK = 32;
@@ -65,40 +116,52 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
 
     const size_t a_tile = (size_t)BM * BK, b_tile = (size_t)BN * BK;  // bf16 elements per k-tile block
-    const uint4* gA[NPL];
-    const uint4* gB[NPL];
-    gA[0] = reinterpret_cast<const uint4*>(p.Ah + (size_t)tm * p.ktiles * a_tile);
-    gB[0] = reinterpret_cast<const uint4*>(p.Bh + (size_t)tn * p.ktiles * b_tile);
+    const uint4* __restrict__ gAh = reinterpret_cast<const uint4*>(p.Ah + (size_t)tm * p.ktiles * a_tile) + tid;
+    const uint4* __restrict__ gBh = reinterpret_cast<const uint4*>(p.Bh + (size_t)tn * p.ktiles * b_tile) + tid;
+    const uint4* __restrict__ gAl = gAh;
+    const uint4* __restrict__ gBl = gBh;
     if (NPL == 2) {
-        gA[NPL - 1] = reinterpret_cast<const uint4*>(p.Al + (size_t)tm * p.ktiles * a_tile);
-        gB[NPL - 1] = reinterpret_cast<const uint4*>(p.Bl + (size_t)tn * p.ktiles * b_tile);
+        gAl = reinterpret_cast<const uint4*>(p.Al + (size_t)tm * p.ktiles * a_tile) + tid;
+        gBl = reinterpret_cast<const uint4*>(p.Bl + (size_t)tn * p.ktiles * b_tile) + tid;
     }
+    // LDS element offsets (in uint4) of this thread's two staging chunks per
+    // plane; the same for every k-tile.  Every configuration moves exactly two
+    // 16-byte chunks per plane per thread, held in named registers (arrays
+    // indexed inside the k-loop were demoted to memory by the compiler).
+    static_assert(A_LD == 2 && B_LD == 2, "staging assumes two chunks per plane per thread");
+    const int cA0 = tid, cA1 = tid + NT;
+    const int sA0 = (cA0 >> 2) * 4 + ((cA0 & 3) ^ lds_swz(cA0 >> 2));
+    const int sA1 = (cA1 >> 2) * 4 + ((cA1 & 3) ^ lds_swz(cA1 >> 2));
+    const int sB0 = sA0, sB1 = (BN == BM) ? sA1 : ((cA1 >> 2) * 4 + ((cA1 & 3) ^ lds_swz(cA1 >> 2)));
 
-    uint4 ra[NPL][A_LD], rb[NPL][B_LD];
-    auto gload = [&](int kt) {
-#pragma unroll
-        for (int pl = 0; pl < NPL; ++pl) {
-#pragma unroll
-            for (int q = 0; q < A_LD; ++q) ra[pl][q] = gA[pl][(size_t)kt * A_CH + tid + q * NT];
-#pragma unroll
-            for (int q = 0; q < B_LD; ++q) rb[pl][q] = gB[pl][(size_t)kt * B_CH + tid + q * NT];
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int pl = 0; pl < NPL; ++pl) {
-#pragma unroll
-            for (int q = 0; q < A_LD; ++q) {
-                const int c = tid + q * NT, row = c >> 2, ch = c & 3;
-                As[buf][pl][row * 4 + (ch ^ lds_swz(row))] = ra[pl][q];
-            }
-#pragma unroll
-            for (int q = 0; q < B_LD; ++q) {
-                const int c = tid + q * NT, row = c >> 2, ch = c & 3;
-                Bs[buf][pl][row * 4 + (ch ^ lds_swz(row))] = rb[pl][q];
-            }
-        }
-    };
+    uint4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
+#define MBH_GLOAD(kt)                                             \
+    do {                                                          \
+        const size_t oa = (size_t)(kt) * A_CH, ob = (size_t)(kt) * B_CH; \
+        ah0 = gAh[oa];                                            \
+        ah1 = gAh[oa + NT];                                       \
+        bh0 = gBh[ob];                                            \
+        bh1 = gBh[ob + NT];                                       \
+        if (NPL == 2) {                                           \
+            al0 = gAl[oa];                                        \
+            al1 = gAl[oa + NT];                                   \
+            bl0 = gBl[ob];                                        \
+            bl1 = gBl[ob + NT];                                   \
+        }                                                         \
+    } while (0)
+#define MBH_LSTORE(buf)                                           \
+    do {                                                          \
+        As[buf][0][sA0] = ah0;                                    \
+        As[buf][0][sA1] = ah1;                                    \
+        Bs[buf][0][sB0] = bh0;                                    \
+        Bs[buf][0][sB1] = bh1;                                    \
+        if (NPL == 2) {                                           \
+            As[buf][NPL - 1][sA0] = al0;                          \
+            As[buf][NPL - 1][sA1] = al1;                          \
+            Bs[buf][NPL - 1][sB0] = bl0;                          \
+            Bs[buf][NPL - 1][sB1] = bl1;                          \
+        }                                                         \
+    } while (0)
 
     f32x16 acc[TM][TN];
 #pragma unroll
@@ -111,8 +174,8 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     int kt0, kt1;
     kt_range(p.krange, tm, tn, BM, BN, BK, p.ktiles, kt0, kt1);
     if (kt0 < kt1) {
-        gload(kt0);
-        lstore(0);
+        MBH_GLOAD(kt0);
+        MBH_LSTORE(0);
     }
     __syncthreads();
 
@@ -121,7 +184,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     const int lrow = lane & 31, lgrp = lane >> 5, lsw = lds_swz(lrow);
     for (int kt = kt0; kt < kt1; ++kt) {
         const int cur = (kt - kt0) & 1;
-        if (kt + 1 < kt1) gload(kt + 1);
+        if (kt + 1 < kt1) MBH_GLOAD(kt + 1);
 #pragma unroll
         for (int ks = 0; ks < BK / 16; ++ks) {
             const int ch = (2 * ks + lgrp) ^ lsw;
@@ -149,30 +212,14 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
                 }
         }
-        if (kt + 1 < kt1) lstore(cur ^ 1);
+        if (kt + 1 < kt1) MBH_LSTORE(cur ^ 1);
         __syncthreads();
     }
+#undef MBH_GLOAD
+#undef MBH_LSTORE
 
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-    const float alpha = p.alpha, beta = p.beta;
-#pragma unroll
-    for (int i = 0; i < TM; ++i) {
-        const int64_t row = m0 + wm0 + i * 32 + lrow;
-        if (row >= p.m) continue;
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t col = n0 + wn0 + j * 32 + acc_col<32>(r, lane);
-                if (col < p.n && tri_keep(p.tri, row, col)) {
-                    float* cp = p.C + row + col * p.ldc;
-                    float v = alpha * acc[i][j][r];
-                    if (beta != 0.0f) v += beta * *cp;
-                    *cp = v;
-                }
-            }
-        }
-    }
+    epilogue_32x32<TM, TN>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
 }
 
 }  // namespace mbh
