@@ -106,7 +106,7 @@ def main():
     C = torch.empty(N * N, device="cuda")
     variants = [v.strip().upper() for v in args.variants.split(",") if v.strip()]
     results = {}
-    for prec in ("bf16x3", "fp32"):
+    for prec in ("bf16x3", "fp32", "fp8"):
         for v in variants:
             ms, wall_ms = time_gemm(mb, torch, v[0], v[1], A, B, C, prec, args.steps, args.warmup)
             results[(prec, v)] = ms
@@ -127,7 +127,7 @@ def main():
         ref = oracle.sgemm_sample("N", "N", N, 1.0, A.cpu().numpy(), N, B.cpu().numpy(), N, 0.0, None, rows, cols)
         chk = C64[torch.from_numpy(rows).cuda(), torch.from_numpy(cols).cuda()].cpu().numpy()
         assert np.allclose(chk, ref, rtol=1e-12, atol=1e-9), "fp64 reference disagrees with the oracle"
-        for prec in ("bf16x3", "fp32"):
+        for prec in ("bf16x3", "fp32", "fp8"):
             mb.gemm("N", "N", N, N, N, 1.0, A, N, B, N, 0.0, C, N, precision=prec)
             torch.cuda.synchronize()
             err[prec] = errors(torch, C.view(N, N).t(), C64, ABabs)
@@ -153,6 +153,7 @@ def main():
         "data": "synthetic: A, B iid N(0,1) fp32 (torch cuda Generator seed 1234), column-major, resident in HBM",
         "config": {"workload": "SGEMM N,N m=n=k=8192 lda=ldb=ldc=8192 alpha=1 beta=0", "m": N, "n": N, "k": N},
         "fp32_tflops": round(tf_fp, 3),
+        "fp8_tflops": round(FLOPS / results[("fp8", "NN")] / 1e9, 3),
         "bf16x3_over_fp32": round(tf_bf / tf_fp, 3),
         "variants_tflops": {"%s_%s" % (p, v): round(FLOPS / ms / 1e9, 3) for (p, v), ms in results.items()},
         "errors_vs_fp64": err,

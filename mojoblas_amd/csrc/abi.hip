@@ -114,7 +114,7 @@ int default_precision() {
     return g_default_prec;
 }
 
-bool gemm_precision_ok(int prec) { return prec == MBH_PREC_FP32 || prec == MBH_PREC_BF16X3; }
+bool gemm_precision_ok(int prec) { return prec == MBH_PREC_FP32 || prec == MBH_PREC_BF16X3 || prec == MBH_PREC_FP8; }
 
 // ---- device staging for the host-pointer entries ----------------------------
 // Grow-only per-slot device buffers, guarded by one mutex that is held for the

@@ -90,9 +90,60 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     return hipGetLastError();
 }
 
+// FP8 path: per-128x128-tile scaled e4m3 pack, then the packed kernel in fp8
+// mode (64-deep k-tiles; K padded to a multiple of 128).
+template <int BT_, int WM_, int WN_>
+hipError_t run_fp8(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
+    constexpr int BT = BT_, PK = 64;
+    const int64_t tm = (g.m + BT - 1) / BT, tn = (g.n + BT - 1) / BT;
+    const int64_t kblocks = (g.k + 127) / 128, ktiles = 2 * kblocks;
+    const int64_t ntile = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
+    if (ntile <= 0 || ntile > INT_MAX || ktiles > INT_MAX) return hipErrorInvalidValue;
+    const size_t qa = (size_t)tm * BT * ktiles * PK, qb = (size_t)tn * BT * ktiles * PK;
+    const size_t na = (size_t)tm * (BT / 128) * kblocks, nb = (size_t)tn * (BT / 128) * kblocks;
+    void* ws = nullptr;
+    hipError_t e = workspace(s, 0, qa + qb + (na + nb) * sizeof(int), &ws);
+    if (e != hipSuccess) return e;
+    uint8_t* base = (uint8_t*)ws;
+    int* sa = (int*)(base + qa + qb);
+    int* sb = sa + na;
+    PackFp8Params pa{g.A, g.lda, g.m, g.k, (int)ktiles, (int)kblocks, base, sa, g.a_vec};
+    PackFp8Params pb{g.B, g.ldb, g.n, g.k, (int)ktiles, (int)kblocks, base + qa, sb, g.b_vec};
+    dim3 ga((unsigned)(tm * BT / 128), (unsigned)kblocks), gb((unsigned)(tn * BT / 128), (unsigned)kblocks);
+    if (at) pack_fp8_kernel<true, BT><<<ga, 256, 0, s>>>(pa);
+    else pack_fp8_kernel<false, BT><<<ga, 256, 0, s>>>(pa);
+    if (!bt) pack_fp8_kernel<true, BT><<<gb, 256, 0, s>>>(pb);
+    else pack_fp8_kernel<false, BT><<<gb, 256, 0, s>>>(pb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    PackedGemmParams p{};
+    p.m = g.m;
+    p.n = g.n;
+    p.ktiles = (int)ktiles;
+    p.Ah = (const uint16_t*)pa.q;
+    p.Bh = (const uint16_t*)pb.q;
+    p.C = g.C;
+    p.ldc = g.ldc;
+    p.alpha = g.alpha;
+    p.beta = g.beta;
+    p.tiles_m = (int)tm;
+    p.tiles_n = (int)tn;
+    p.tri = g.tri;
+    p.tri_tiles = tri_grid;
+    p.krange = g.krange;
+    p.sA = sa;
+    p.sB = sb;
+    p.kblocks = (int)kblocks;
+    gemm_packed_kernel<8, BT, BT, WM_, WN_><<<dim3((unsigned)ntile), dim3(64 * WM_ * WN_), 0, s>>>(p);
+    return hipGetLastError();
+}
+
 template <int NPROD>
 hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
     const int64_t t256 = ((g.m + 255) / 256) * ((g.n + 255) / 256);
+    if (NPROD == 8) {
+        if (t256 >= 256) return run_fp8<256, 2, 4>(at, bt, g, tri_grid, s);
+        return run_fp8<128, 2, 2>(at, bt, g, tri_grid, s);
+    }
     if (t256 >= 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s);
     return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s);
 }
@@ -157,6 +208,8 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
             return launch_packed<3>(at, bt, p, tri_grid, s);
         case PREC_BF16:
             return launch_packed<1>(at, bt, p, tri_grid, s);
+        case PREC_FP8:
+            return launch_packed<8>(at, bt, p, tri_grid, s);
         default:
             return hipErrorInvalidValue;
     }

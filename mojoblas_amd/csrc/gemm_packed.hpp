@@ -1,7 +1,13 @@
-// Macro-tile gemm over packed bf16 panels (see pack.hpp), fp32 accumulate.
+// Macro-tile gemm over packed low-precision panels (see pack.hpp), fp32
+// accumulate.
 //
 //   NPROD = 3: BF16x3, acc += Bh*Ah + Bl*Ah + Bh*Al   (x = hi + lo per operand)
 //   NPROD = 1: plain bf16 operands, acc += Bh*Ah
+//   NPROD = 8: e4m3 operands with one power-of-two scale per 128x128 tile of
+//              A and of B, applied by v_mfma_scale_f32_32x32x64_f8f6f4 (E8M0
+//              scale operands), so no per-slab partial sums are needed
+//
+// A k-tile is 64 bytes per row in every mode: 32 bf16 or 64 e4m3 values.
 //
 // Each k-tile of a macro-tile panel is one contiguous BM x BK (BN x BK) bf16
 // run per plane, so the global loads are 16-byte, fully coalesced and carry
@@ -39,7 +45,12 @@ struct PackedGemmParams {
     int tri;
     int tri_tiles;  // 0 full grid, 1/2 lower/upper block triangle
     int krange;     // KRange (k-tiles visited per tile)
+    const int* sA;  // fp8: E8M0 scale per 128x128 tile, [rows/128][kblocks]
+    const int* sB;
+    int kblocks;
 };
+
+typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 __device__ inline int lds_swz(int row) { return ((row >> 1) ^ (row >> 2)) & 3; }
 
@@ -96,13 +107,14 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* C, int
 
 template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(PackedGemmParams p) {
-    constexpr int BK = 32;
+    constexpr bool FP8 = (NPROD == 8);
+    constexpr int BK = FP8 ? 64 : 32;  // k per 64-byte LDS row
     constexpr int NT = 64 * WAVES_M * WAVES_N;
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
     constexpr int TM = WM / 32, TN = WN / 32;
     constexpr int NPL = (NPROD == 3) ? 2 : 1;  // planes per operand
-    constexpr int A_CH = BM * BK / 8;          // 16-byte chunks per A plane tile
-    constexpr int B_CH = BN * BK / 8;
+    constexpr int A_CH = BM * 4;               // 16-byte chunks per A plane tile (64-byte rows)
+    constexpr int B_CH = BN * 4;
     constexpr int A_LD = A_CH / NT, B_LD = B_CH / NT;  // chunks per thread per plane
     static_assert(A_LD * NT == A_CH && B_LD * NT == B_CH, "tile not divisible by the thread block");
     // LDS: [buf][plane][rows * 64 bytes] as uint4 (16 bytes)
@@ -115,14 +127,16 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
 
-    const size_t a_tile = (size_t)BM * BK, b_tile = (size_t)BN * BK;  // bf16 elements per k-tile block
-    const uint4* __restrict__ gAh = reinterpret_cast<const uint4*>(p.Ah + (size_t)tm * p.ktiles * a_tile) + tid;
-    const uint4* __restrict__ gBh = reinterpret_cast<const uint4*>(p.Bh + (size_t)tn * p.ktiles * b_tile) + tid;
+    // Panel offsets in 16-byte chunks: a k-tile block is BM (BN) rows of 64
+    // bytes in every mode.
+    const size_t a_off = (size_t)tm * p.ktiles * A_CH + tid, b_off = (size_t)tn * p.ktiles * B_CH + tid;
+    const uint4* __restrict__ gAh = reinterpret_cast<const uint4*>(p.Ah) + a_off;
+    const uint4* __restrict__ gBh = reinterpret_cast<const uint4*>(p.Bh) + b_off;
     const uint4* __restrict__ gAl = gAh;
     const uint4* __restrict__ gBl = gBh;
     if (NPL == 2) {
-        gAl = reinterpret_cast<const uint4*>(p.Al + (size_t)tm * p.ktiles * a_tile) + tid;
-        gBl = reinterpret_cast<const uint4*>(p.Bl + (size_t)tn * p.ktiles * b_tile) + tid;
+        gAl = reinterpret_cast<const uint4*>(p.Al) + a_off;
+        gBl = reinterpret_cast<const uint4*>(p.Bl) + b_off;
     }
     // LDS element offsets (in uint4) of this thread's two staging chunks per
     // plane; the same for every k-tile.  Every configuration moves exactly two
@@ -185,6 +199,31 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     for (int kt = kt0; kt < kt1; ++kt) {
         const int cur = (kt - kt0) & 1;
         if (kt + 1 < kt1) MBH_GLOAD(kt + 1);
+        if constexpr (FP8) {
+            // Lane's 32 k values: chunks 2*lgrp and 2*lgrp+1 of its row.
+            const int c0 = (2 * lgrp) ^ lsw, c1 = (2 * lgrp + 1) ^ lsw;
+            const int sa = p.sA[((tm * BM + wm0) >> 7) * p.kblocks + (kt >> 1)];
+            const int sb = p.sB[((tn * BN + wn0) >> 7) * p.kblocks + (kt >> 1)];
+            i32x8 af[TM], bfr[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                const int base = (wm0 + i * 32 + lrow) * 4;
+                const uint4 x0 = As[cur][0][base + c0], x1 = As[cur][0][base + c1];
+                af[i] = i32x8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+            }
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const int base = (wn0 + j * 32 + lrow) * 4;
+                const uint4 x0 = Bs[cur][0][base + c0], x1 = Bs[cur][0][base + c1];
+                bfr[j] = i32x8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(bfr[j], af[i], acc[i][j], 0, 0, 0, sb,
+                                                                                 0, sa);
+        } else {
 #pragma unroll
         for (int ks = 0; ks < BK / 16; ++ks) {
             const int ch = (2 * ks + lgrp) ^ lsw;
@@ -211,6 +250,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
                     }
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
                 }
+        }
         }
         if (kt + 1 < kt1) MBH_LSTORE(cur ^ 1);
         __syncthreads();

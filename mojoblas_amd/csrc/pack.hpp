@@ -101,3 +101,106 @@ __global__ __launch_bounds__(256) void pack_split_kernel(PackParams p) {
 }
 
 }  // namespace mbh
+
+namespace mbh {
+
+// FP8 pack: op(X) -> e4m3 (OCP, max 448) with one power-of-two scale per
+// 128-row x 128-k tile.  Packed like the bf16 planes (blocks of PR rows x 64
+// k, one byte per element, k fastest, zero padding), scales as E8M0 bytes
+// (exponent + 127) in a [R/128][K/128] int array: x ~= 2^(e) * q.
+// One workgroup stages its whole 128x128 fp32 tile in LDS, reduces the tile's
+// amax, picks e = ceil(log2(amax / 448)) and writes 16 quantized bytes per
+// store.
+struct PackFp8Params {
+    const float* X;
+    int64_t ld;
+    int64_t R, K;
+    int ktiles;        // packed 64-deep k-tiles (even: K padded to 128)
+    int kblocks;       // K / 128 (rounded up)
+    uint8_t* q;        // packed e4m3
+    int* scale;        // E8M0 per 128x128 tile
+    int vec;           // 16-byte loads allowed
+};
+
+__device__ inline uint32_t fp8_pack4(float a, float b, float c, float d) {
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+    return (uint32_t)w;
+}
+
+template <bool KCONTIG, int PR>
+__global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params p) {
+    constexpr int T = 128, PK = 64;
+    __shared__ float s[T][T + 1];
+    __shared__ float red[256];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * T, k0 = (int64_t)blockIdx.y * T;
+    float amax = 0.0f;
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it) {
+        const int idx = tid + 256 * it;
+        if (KCONTIG) {  // (r, k) at X[k + r*ld]: 32 float4 per row
+            const int r = idx / 32, kq = (idx % 32) * 4;
+            const int64_t gr = r0 + r, gk = k0 + kq;
+            f32x4 v = {0.f, 0.f, 0.f, 0.f};
+            if (p.vec && gr < p.R && gk + 4 <= p.K) {
+                v = *reinterpret_cast<const f32x4*>(p.X + gk + gr * p.ld);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = (gr < p.R && gk + e < p.K) ? p.X[gk + e + gr * p.ld] : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                s[r][kq + e] = v[e];
+                amax = fmaxf(amax, fabsf(v[e]));
+            }
+        } else {  // (r, k) at X[r + k*ld]: 32 float4 per k column
+            const int kk = idx / 32, rq = (idx % 32) * 4;
+            const int64_t gr = r0 + rq, gk = k0 + kk;
+            f32x4 v = {0.f, 0.f, 0.f, 0.f};
+            if (p.vec && gk < p.K && gr + 4 <= p.R) {
+                v = *reinterpret_cast<const f32x4*>(p.X + gr + gk * p.ld);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = (gr + e < p.R && gk < p.K) ? p.X[gr + e + gk * p.ld] : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                s[rq + e][kk] = v[e];
+                amax = fmaxf(amax, fabsf(v[e]));
+            }
+        }
+    }
+    red[tid] = amax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = fmaxf(red[tid], red[tid + w]);
+        __syncthreads();
+    }
+    amax = red[0];
+    // Smallest power of two s with amax / s <= 448 (e4m3 max); 1 for a zero tile.
+    int e = 0;
+    if (amax > 0.0f && __builtin_isfinite(amax)) {
+        e = (int)ceilf(log2f(amax / 448.0f));
+        if (ldexpf(448.0f, e) < amax) ++e;
+        if (e < -127) e = -127;
+        if (e > 127) e = 127;
+    }
+    const float inv = ldexpf(1.0f, -e);
+    if (tid == 0) p.scale[(r0 / T) * p.kblocks + k0 / T] = e + 127;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int idx = tid + 256 * it;
+        const int r = idx / 8, kc = (idx % 8) * 16;
+        const int64_t gr = r0 + r, gk = k0 + kc;
+        const int64_t t = gr / PR, rr = gr % PR, kt = gk / PK, kk = gk % PK;
+        uint4 w;
+        w.x = fp8_pack4(s[r][kc + 0] * inv, s[r][kc + 1] * inv, s[r][kc + 2] * inv, s[r][kc + 3] * inv);
+        w.y = fp8_pack4(s[r][kc + 4] * inv, s[r][kc + 5] * inv, s[r][kc + 6] * inv, s[r][kc + 7] * inv);
+        w.z = fp8_pack4(s[r][kc + 8] * inv, s[r][kc + 9] * inv, s[r][kc + 10] * inv, s[r][kc + 11] * inv);
+        w.w = fp8_pack4(s[r][kc + 12] * inv, s[r][kc + 13] * inv, s[r][kc + 14] * inv, s[r][kc + 15] * inv);
+        *reinterpret_cast<uint4*>(p.q + ((t * p.ktiles + kt) * PR + rr) * PK + kk) = w;
+    }
+}
+
+}  // namespace mbh

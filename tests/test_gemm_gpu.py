@@ -2,7 +2,9 @@
 tiles) against the fp64 oracle, through the C-ABI (device and host entries).
 
 Bound, componentwise: |C - C_oracle| <= (e_rep + 2 k u)|alpha| |A||B| + 4u|beta C|
-with u = 2^-24 and e_rep = 0 (fp32) or 3*2^-18 (bf16x3, see helpers.rep_error).
+with u = 2^-24 and e_rep = 0 (fp32) or 3*2^-16 (bf16x3, see helpers.rep_error).
+FP8 (e4m3, per-128x128-tile scales): bit-exact on representable inputs and
+within the per-element rounding model |x - qs| <= 2^-4|x| + 2^-10 s otherwise.
 """
 import itertools
 import json
@@ -159,3 +161,69 @@ def test_large_sampled(ta, tb, prec):
     got = c[rows + cols * m].astype(np.float64)
     bound = H.gemm_bound(prec, k, 1.0, absab, 0.0, 0.0)
     assert (np.abs(got - want) <= bound).all(), float((np.abs(got - want) / bound).max())
+
+
+def _tile_scaled_ints(rng, rows, cols, rows_tiles_first):
+    """Integers in [-8, 8] times 2^e, e in [-2, 2] drawn per 128x128 tile of
+    the (rows x cols) matrix: every value and every partial product sum is
+    exact in e4m3 (after the per-tile power-of-two scale) and in fp32."""
+    x = rng.integers(-8, 9, (rows, cols)).astype(np.float64)
+    et = rng.integers(-2, 3, ((rows + 127) // 128, (cols + 127) // 128))
+    e = np.repeat(np.repeat(et, 128, axis=0), 128, axis=1)[:rows, :cols]
+    return x * np.exp2(e)
+
+
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
+def test_fp8_exact_on_representable_inputs(ta, tb):
+    """FP8 (e4m3, per-128x128-tile power-of-two scales) is bit-exact when every
+    operand tile is representable: pins the per-tile scale lookup and layout."""
+    rng = np.random.default_rng(16)
+    m, n, k = 300, 260, 384
+    opa = _tile_scaled_ints(rng, m, k, True)   # op(A), m x k: tiles over (row, k)
+    opb = _tile_scaled_ints(rng, n, k, True).T  # op(B), k x n: tiles over (n, k)
+    a = (opa if ta == "N" else opa.T).astype(np.float32)
+    b = (opb if tb == "N" else opb.T).astype(np.float32)
+    abuf = np.asfortranarray(a).ravel(order="F")
+    bbuf = np.asfortranarray(b).ravel(order="F")
+    got = run_dev(ta, tb, m, n, k, 1.0, abuf, a.shape[0], bbuf, b.shape[0], 0.0, np.zeros(m * n, np.float32), m,
+                  "fp8")
+    want = H.oracle_gemm(ta, tb, m, n, k, 1.0, abuf, a.shape[0], bbuf, b.shape[0], 0.0, np.zeros(m * n), m)
+    assert np.array_equal(got.reshape(n, m).T.astype(np.float64), want)
+
+
+def _fp8_bound(op_rows_by_k, k):
+    """Per-element e4m3 error bound after per-128x128-tile scaling:
+    |x - q s| <= 2^-4 |x| + 2^-10 s, s = 2^ceil(log2(amax_tile / 448))."""
+    r = op_rows_by_k.shape[0]
+    d = np.empty_like(op_rows_by_k)
+    for i0 in range(0, r, 128):
+        for k0 in range(0, k, 128):
+            t = op_rows_by_k[i0:i0 + 128, k0:k0 + 128]
+            amax = np.abs(t).max()
+            s = 2.0 ** np.ceil(np.log2(amax / 448.0)) if amax > 0 else 1.0
+            d[i0:i0 + 128, k0:k0 + 128] = 2.0 ** -4 * np.abs(t) + 2.0 ** -10 * s
+    return d
+
+
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
+def test_fp8_within_rounding_bound(ta, tb):
+    rng = np.random.default_rng(17)
+    m, n, k = 257, 200, 300
+    opa = rng.standard_normal((m, k))
+    opb = rng.standard_normal((k, n))
+    a = (opa if ta == "N" else opa.T).astype(np.float32)
+    b = (opb if tb == "N" else opb.T).astype(np.float32)
+    abuf = np.asfortranarray(a).ravel(order="F")
+    bbuf = np.asfortranarray(b).ravel(order="F")
+    got = run_dev(ta, tb, m, n, k, 1.0, abuf, a.shape[0], bbuf, b.shape[0], 0.0, np.zeros(m * n, np.float32), m,
+                  "fp8")
+    want = H.oracle_gemm(ta, tb, m, n, k, 1.0, abuf, a.shape[0], bbuf, b.shape[0], 0.0, np.zeros(m * n), m)
+    oa = np.abs(opa.astype(np.float32).astype(np.float64))
+    ob = np.abs(opb.astype(np.float32).astype(np.float64))
+    da = _fp8_bound(oa, k)
+    db = _fp8_bound(ob.T, k).T
+    bound = da @ ob + oa @ db + da @ db + 2 * k * 2.0 ** -24 * (oa @ ob)
+    err = np.abs(got.reshape(n, m).T - want)
+    assert (err <= bound).all(), float((err / bound).max())
+    # and it is a low-accuracy mode: clearly less accurate than bf16x3
+    assert err.max() > 1e-3
