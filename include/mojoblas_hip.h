@@ -149,6 +149,39 @@ int mbh_dsymm_dev(char side, char uplo, int64_t m, int64_t n, double alpha, cons
                   int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc,
                   void* stream);
 
+/* ---- variable-size batched gemm (no counterpart in the reference) --------
+ * count independent problems C_p := alpha*A_p*B_p + beta*C_p (each with the
+ * semantics of gemm("N","N", m_p, n_p, k_p, ...), gemm.mojo:56-124), run as
+ * one persistent kernel that pulls C tiles from a work queue.
+ *  plan_create : host arrays of shapes and leading dimensions; validates every
+ *                problem (MBH_E_ARG names the first bad one), assigns tile
+ *                classes from the tuning table, orders the tiles longest-first
+ *                and stores the plan in HBM (allocates; once per shape list).
+ *  exec        : device arrays of count device pointers; allocates nothing and
+ *                never synchronises the host, so it can be captured in a
+ *                hipGraph and replayed.  precision MBH_PREC_FP32 (fp32 MFMA) or
+ *                MBH_PREC_BF16 (bf16 operands packed on the fly, fp32
+ *                accumulate).  Calls sharing one plan must not overlap.
+ *  plan_info   : out[8] = {count, tiles, workgroups, flops, problems on
+ *                64x64 tiles, on 128x128 tiles, pack blocks, precision}.
+ *  set_tuning  : class table (36 ints, [bucket(m)][bucket(n)], 0 = 64x64,
+ *                1 = 128x128; NULL = compiled default) and workgroups per CU
+ *                (0 = default) for plans created afterwards. */
+typedef struct mbh_vbatch_plan_s* mbh_vbatch_plan;
+int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, const int64_t* k,
+                           const int64_t* lda, const int64_t* ldb, const int64_t* ldc, int precision,
+                           mbh_vbatch_plan* plan);
+int mbh_sgemm_vbatched_exec(mbh_vbatch_plan plan, float alpha, const float* const* a,
+                            const float* const* b, float beta, float* const* c, void* stream);
+int mbh_vbatch_plan_info(mbh_vbatch_plan plan, double* out);
+int mbh_vbatch_plan_destroy(mbh_vbatch_plan plan);
+int mbh_vbatch_set_tuning(const int* class_table, int wgs_per_cu);
+/* One-shot convenience: plan_create + exec + plan_destroy (synchronous). */
+int mbh_sgemm_vbatched_dev(int64_t count, const int64_t* m, const int64_t* n, const int64_t* k,
+                           float alpha, const float* const* a, const int64_t* lda,
+                           const float* const* b, const int64_t* ldb, float beta, float* const* c,
+                           const int64_t* ldc, int precision, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,12 @@ SIGNATURES = {
     "mbh_dsymm_host": ([ch, ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64], i32),
     "mbh_ssymm_dev": ([ch, ch, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, i32, vp], i32),
     "mbh_dsymm_dev": ([ch, ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp], i32),
+    "mbh_vbatch_plan_create": ([i64, vp, vp, vp, vp, vp, vp, i32, vp], i32),
+    "mbh_sgemm_vbatched_exec": ([vp, f32, vp, vp, f32, vp, vp], i32),
+    "mbh_vbatch_plan_info": ([vp, vp], i32),
+    "mbh_vbatch_plan_destroy": ([vp], i32),
+    "mbh_vbatch_set_tuning": ([vp, i32], i32),
+    "mbh_sgemm_vbatched_dev": ([i64, vp, vp, vp, f32, vp, vp, vp, vp, f32, vp, vp, i32, vp], i32),
 }
 
 

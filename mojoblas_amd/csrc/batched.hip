@@ -1,0 +1,444 @@
+// Variable-size batched gemm: thousands of independent N,N problems
+//   C_p := alpha * A_p * B_p + beta * C_p,   p = 0..count-1,
+// with their own m, n, k and leading dimensions, run by one persistent
+// kernel.  (The reference has no batched routine; each problem has the
+// semantics of gemm("N", "N", ...), src/level3/gemm.mojo:56-124.)
+//
+// Plan (host, once per shape list): validate, give every problem a tile class
+// from the tuning table, enumerate its C tiles as work items, order the items
+// longest-first (by BM*BN*k) and copy everything to HBM; in bf16 mode also
+// lay out the packed bf16 panels and the pack work list.
+// Exec (device only, no allocation, no host synchronisation, capturable in a
+// hipGraph): a one-thread kernel resets the work counter, [bf16: one pack
+// kernel converts every problem's A and B^T to k-contiguous bf16 panels],
+// then the persistent kernel's workgroups pull items from the counter until
+// the list is exhausted.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "../../include/mojoblas_hip.h"
+#include "abi_internal.hpp"
+#include "gemm_fp.hpp"
+#include "gemm_packed.hpp"
+#include "pack.hpp"
+#include "vbatched_tuning.hpp"
+
+namespace mbh {
+
+struct VbProblem {
+    int64_t m, n, k, lda, ldb, ldc;
+    int cls, tiles_n, ktiles, pad;
+    int64_t a_off, b_off;  // bf16 mode: panel offsets (bf16 elements) in the pack buffer
+};
+
+struct VbItem {
+    int prob, tile;
+};
+
+struct VbPackBlock {
+    int prob, which;  // which: 0 = op(A) rows (m), 1 = op(B)^T rows (n)
+    int r0, k0;
+};
+
+// Tile classes: 0 = 64x64 (2x2 waves of 32x32), 1 = 128x128 (2x2 waves of 64x64).
+__host__ __device__ constexpr int vb_tile(int cls) { return cls == 0 ? 64 : 128; }
+
+namespace {
+
+__global__ void vb_reset_kernel(int* ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = 0;
+}
+
+constexpr int FP_LDS =
+    FpTileLds<float, float, float, false, false, 128, 128, 16>::TOTAL > FpTileLds<float, float, float, false, false, 64, 64, 16>::TOTAL
+        ? FpTileLds<float, float, float, false, false, 128, 128, 16>::TOTAL
+        : FpTileLds<float, float, float, false, false, 64, 64, 16>::TOTAL;
+constexpr int BF_LDS = PackedTileLds<1, 128, 128>::TOTAL;
+
+// Claims the next work item for the whole workgroup (-1 when drained).
+__device__ inline int vb_next(int* ctr, int nitems, int* slot) {
+    __syncthreads();  // everyone has consumed the previous value of *slot
+    if (threadIdx.x == 0) *slot = atomicAdd(ctr, 1);
+    __syncthreads();
+    const int it = *slot;
+    return it < nitems ? it : -1;
+}
+
+__global__ __launch_bounds__(256) void vb_gemm_f32_kernel(const VbProblem* __restrict__ probs,
+                                                          const VbItem* __restrict__ items, int nitems, int* ctr,
+                                                          float alpha, float beta, const float* const* A,
+                                                          const float* const* B, float* const* C) {
+    __shared__ __attribute__((aligned(16))) float lds[FP_LDS];
+    __shared__ int slot;
+    for (;;) {
+        const int it = vb_next(ctr, nitems, &slot);
+        if (it < 0) break;
+        const VbItem w = items[it];
+        const VbProblem pr = probs[w.prob];
+        GemmParams<float> p{};
+        p.m = pr.m;
+        p.n = pr.n;
+        p.k = pr.k;
+        p.A = A[w.prob];
+        p.lda = pr.lda;
+        p.B = B[w.prob];
+        p.ldb = pr.ldb;
+        p.C = C[w.prob];
+        p.ldc = pr.ldc;
+        p.alpha = alpha;
+        p.beta = beta;
+        p.a_vec = ((uintptr_t)p.A % 16 == 0) && (pr.lda % 4 == 0);
+        p.b_vec = ((uintptr_t)p.B % 16 == 0) && (pr.ldb % 4 == 0);
+        const int tm = w.tile / pr.tiles_n, tn = w.tile % pr.tiles_n;
+        if (pr.cls == 0)
+            gemm_fp_tile<float, float, float, float, false, false, 64, 64, 16>(p, tm, tn, lds);
+        else
+            gemm_fp_tile<float, float, float, float, false, false, 128, 128, 16>(p, tm, tn, lds);
+    }
+}
+
+// bf16 pack of every problem: block b moves 64 rows x 32 k of op(A) (m x k,
+// m-contiguous for N) or op(B)^T (n x k, k-contiguous for N) into the
+// problem's panel (tile rows of the problem's class).
+__global__ __launch_bounds__(256) void vb_pack_kernel(const VbProblem* __restrict__ probs,
+                                                      const VbPackBlock* __restrict__ blocks,
+                                                      const float* const* A, const float* const* B, uint16_t* pack) {
+    __shared__ float s[64][33];
+    const VbPackBlock blk = blocks[blockIdx.x];
+    const VbProblem pr = probs[blk.prob];
+    PackParams pp{};
+    if (blk.which == 0) {
+        pp.X = A[blk.prob];
+        pp.ld = pr.lda;
+        pp.R = pr.m;
+        pp.hi = pack + pr.a_off;
+        pp.vec = ((uintptr_t)pp.X % 16 == 0) && (pr.lda % 4 == 0);
+    } else {
+        pp.X = B[blk.prob];
+        pp.ld = pr.ldb;
+        pp.R = pr.n;
+        pp.hi = pack + pr.b_off;
+        pp.vec = ((uintptr_t)pp.X % 16 == 0) && (pr.ldb % 4 == 0);
+    }
+    pp.K = pr.k;
+    pp.ktiles = pr.ktiles;
+    pp.lo = nullptr;
+    if (blk.which == 0) {  // A 'N': element (i, k) at A[i + k*lda]
+        if (pr.cls == 0) pack_split_block<false, 64, 32>(pp, blk.r0, blk.k0, s);
+        else pack_split_block<false, 128, 32>(pp, blk.r0, blk.k0, s);
+    } else {  // B 'N': element (j, k) of B^T at B[k + j*ldb]
+        if (pr.cls == 0) pack_split_block<true, 64, 32>(pp, blk.r0, blk.k0, s);
+        else pack_split_block<true, 128, 32>(pp, blk.r0, blk.k0, s);
+    }
+}
+
+__global__ __launch_bounds__(256) void vb_gemm_bf16_kernel(const VbProblem* __restrict__ probs,
+                                                           const VbItem* __restrict__ items, int nitems, int* ctr,
+                                                           float alpha, float beta, const uint16_t* pack,
+                                                           float* const* C) {
+    __shared__ uint4 lds[BF_LDS];
+    __shared__ int slot;
+    for (;;) {
+        const int it = vb_next(ctr, nitems, &slot);
+        if (it < 0) break;
+        const VbItem w = items[it];
+        const VbProblem pr = probs[w.prob];
+        PackedGemmParams p{};
+        p.m = pr.m;
+        p.n = pr.n;
+        p.ktiles = pr.ktiles;
+        p.Ah = pack + pr.a_off;
+        p.Bh = pack + pr.b_off;
+        p.C = C[w.prob];
+        p.ldc = pr.ldc;
+        p.alpha = alpha;
+        p.beta = beta;
+        const int tm = w.tile / pr.tiles_n, tn = w.tile % pr.tiles_n;
+        if (pr.cls == 0) gemm_packed_tile<1, 64, 64, 2, 2>(p, tm, tn, lds);
+        else gemm_packed_tile<1, 128, 128, 2, 2>(p, tm, tn, lds);
+    }
+}
+
+// C_p := beta * C_p for every problem (alpha == 0: A and B are not read).
+__global__ void vb_scale_kernel(const VbProblem* __restrict__ probs, int count, float beta, float* const* C) {
+    for (int p = blockIdx.x; p < count; p += gridDim.x) {
+        const VbProblem pr = probs[p];
+        float* c = C[p];
+        for (int64_t idx = threadIdx.x; idx < pr.m * pr.n; idx += blockDim.x) {
+            const int64_t i = idx % pr.m, j = idx / pr.m;
+            float* cp = c + i + j * pr.ldc;
+            *cp = (beta == 0.0f) ? 0.0f : beta * *cp;
+        }
+    }
+}
+
+// ---- tuning state ------------------------------------------------------------
+std::mutex g_tune_mu;
+int g_class_table[VB_BUCKETS][VB_BUCKETS];
+int g_wgs_per_cu = VB_DEFAULT_WGS_PER_CU;
+bool g_tune_init = false;
+
+void tune_init_locked() {
+    if (g_tune_init) return;
+    memcpy(g_class_table, VB_DEFAULT_CLASS, sizeof(g_class_table));
+    g_tune_init = true;
+}
+
+int bucket_of(int64_t x) {
+    // 0: <=16, 1: <=32, 2: <=64, 3: <=128, 4: <=256, 5: larger
+    int b = 0;
+    int64_t lim = 16;
+    while (b < VB_BUCKETS - 1 && x > lim) {
+        ++b;
+        lim *= 2;
+    }
+    return b;
+}
+
+}  // namespace
+
+struct VbPlan {
+    int64_t count = 0;
+    int prec = 0;
+    int nitems = 0, nblocks = 0, grid = 0;
+    double flops = 0.0;
+    VbProblem* probs = nullptr;
+    VbItem* items = nullptr;
+    VbPackBlock* blocks = nullptr;
+    int* ctr = nullptr;
+    uint16_t* pack = nullptr;
+    int64_t class_count[2] = {0, 0};
+};
+
+}  // namespace mbh
+
+using namespace mbh;
+
+extern "C" {
+
+int mbh_vbatch_set_tuning(const int* class_table, int wgs_per_cu) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_init_locked();
+    if (class_table) {
+        for (int i = 0; i < VB_BUCKETS * VB_BUCKETS; ++i)
+            if (class_table[i] < 0 || class_table[i] > 1) {
+                set_error("mbh_vbatch_set_tuning: class %d at %d (classes are 0 and 1)", class_table[i], i);
+                return MBH_E_ARG;
+            }
+        memcpy(g_class_table, class_table, sizeof(g_class_table));
+    } else {
+        memcpy(g_class_table, VB_DEFAULT_CLASS, sizeof(g_class_table));
+    }
+    if (wgs_per_cu < 0 || wgs_per_cu > 8) {
+        set_error("mbh_vbatch_set_tuning: wgs_per_cu %d outside [0, 8]", wgs_per_cu);
+        return MBH_E_ARG;
+    }
+    g_wgs_per_cu = wgs_per_cu ? wgs_per_cu : VB_DEFAULT_WGS_PER_CU;
+    return 0;
+}
+
+int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, const int64_t* k, const int64_t* lda,
+                           const int64_t* ldb, const int64_t* ldc, int precision, mbh_vbatch_plan* out) {
+    if (!out) {
+        set_error("mbh_vbatch_plan_create: null output handle");
+        return MBH_E_ARG;
+    }
+    *out = nullptr;
+    if (count < 0 || count > (int64_t)1 << 30) {
+        set_error("mbh_vbatch_plan_create: count %lld out of range", (long long)count);
+        return MBH_E_ARG;
+    }
+    if (precision != MBH_PREC_FP32 && precision != MBH_PREC_BF16) {
+        set_error("mbh_vbatch_plan_create: precision %d (batched supports FP32 and BF16)", precision);
+        return MBH_E_PRECISION;
+    }
+    if (count > 0 && (!m || !n || !k || !lda || !ldb || !ldc)) {
+        set_error("mbh_vbatch_plan_create: null shape array");
+        return MBH_E_ARG;
+    }
+    // Validation with gemm("N","N") semantics per problem (gemm.mojo:179-213).
+    for (int64_t p = 0; p < count; ++p) {
+        int info = check_gemm('N', 'N', m[p], n[p], k[p], lda[p], ldb[p], ldc[p]);
+        if (info) {
+            set_error("mbh_vbatch_plan_create: problem %lld fails gemm argument %d", (long long)p, info);
+            return MBH_E_ARG;
+        }
+    }
+    int class_table[VB_BUCKETS][VB_BUCKETS];
+    int wgs;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        tune_init_locked();
+        memcpy(class_table, g_class_table, sizeof(class_table));
+        wgs = g_wgs_per_cu;
+    }
+    VbPlan* plan = new VbPlan();
+    plan->count = count;
+    plan->prec = precision;
+    std::vector<VbProblem> probs((size_t)count);
+    struct Item {
+        int prob, tile;
+        double cost;
+    };
+    std::vector<Item> items;
+    std::vector<VbPackBlock> blocks;
+    int64_t pack_el = 0;
+    for (int64_t p = 0; p < count; ++p) {
+        VbProblem& q = probs[(size_t)p];
+        q.m = m[p];
+        q.n = n[p];
+        q.k = k[p];
+        q.lda = lda[p];
+        q.ldb = ldb[p];
+        q.ldc = ldc[p];
+        q.cls = class_table[bucket_of(q.m)][bucket_of(q.n)];
+        const int T = vb_tile(q.cls);
+        const int64_t tm = (q.m + T - 1) / T, tn = (q.n + T - 1) / T;
+        q.tiles_n = (int)tn;
+        q.ktiles = (int)((q.k + 31) / 32);
+        plan->flops += 2.0 * (double)q.m * (double)q.n * (double)q.k;
+        if (q.m == 0 || q.n == 0) continue;
+        plan->class_count[q.cls]++;
+        for (int64_t t = 0; t < tm * tn; ++t) items.push_back({(int)p, (int)t, (double)T * T * (double)q.k});
+        if (precision == MBH_PREC_BF16) {
+            q.a_off = pack_el;
+            pack_el += tm * T * q.ktiles * 32;
+            q.b_off = pack_el;
+            pack_el += tn * T * q.ktiles * 32;
+            for (int which = 0; which < 2; ++which) {
+                const int64_t rows = (which == 0 ? tm : tn) * T;
+                for (int64_t r0 = 0; r0 < rows; r0 += 64)
+                    for (int64_t k0 = 0; k0 < (int64_t)q.ktiles * 32; k0 += 32)
+                        blocks.push_back({(int)p, which, (int)r0, (int)k0});
+            }
+        }
+    }
+    // Longest-first: the dynamic queue then finishes with the short items.
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+    std::vector<VbItem> ditems(items.size());
+    for (size_t i = 0; i < items.size(); ++i) ditems[i] = {items[i].prob, items[i].tile};
+    if (items.size() > (size_t)INT32_MAX || blocks.size() > (size_t)INT32_MAX) {
+        delete plan;
+        set_error("mbh_vbatch_plan_create: too many tiles");
+        return MBH_E_ARG;
+    }
+    plan->nitems = (int)ditems.size();
+    plan->nblocks = (int)blocks.size();
+    int dev = 0, ncu = 256;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+    }
+    plan->grid = std::max(1, std::min(plan->nitems, ncu * wgs));
+    auto fail = [&](hipError_t err, const char* what) {
+        mbh_vbatch_plan_destroy((mbh_vbatch_plan)plan);
+        return hip_fail(err, what);
+    };
+    if ((e = hipMalloc(&plan->ctr, sizeof(int))) != hipSuccess) return fail(e, "vbatch plan: counter");
+    if (count > 0) {
+        if ((e = hipMalloc(&plan->probs, sizeof(VbProblem) * probs.size())) != hipSuccess)
+            return fail(e, "vbatch plan: problems");
+        if ((e = hipMemcpy(plan->probs, probs.data(), sizeof(VbProblem) * probs.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return fail(e, "vbatch plan: problems copy");
+    }
+    if (!ditems.empty()) {
+        if ((e = hipMalloc(&plan->items, sizeof(VbItem) * ditems.size())) != hipSuccess)
+            return fail(e, "vbatch plan: items");
+        if ((e = hipMemcpy(plan->items, ditems.data(), sizeof(VbItem) * ditems.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return fail(e, "vbatch plan: items copy");
+    }
+    if (!blocks.empty()) {
+        if ((e = hipMalloc(&plan->blocks, sizeof(VbPackBlock) * blocks.size())) != hipSuccess)
+            return fail(e, "vbatch plan: pack blocks");
+        if ((e = hipMemcpy(plan->blocks, blocks.data(), sizeof(VbPackBlock) * blocks.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "vbatch plan: pack blocks copy");
+        if ((e = hipMalloc(&plan->pack, sizeof(uint16_t) * (size_t)pack_el)) != hipSuccess)
+            return fail(e, "vbatch plan: bf16 panels");
+    }
+    *out = (mbh_vbatch_plan)plan;
+    return 0;
+}
+
+int mbh_vbatch_plan_info(mbh_vbatch_plan handle, double* info) {
+    const VbPlan* plan = (const VbPlan*)handle;
+    if (!plan || !info) {
+        set_error("mbh_vbatch_plan_info: null argument");
+        return MBH_E_ARG;
+    }
+    info[0] = (double)plan->count;
+    info[1] = (double)plan->nitems;
+    info[2] = (double)plan->grid;
+    info[3] = plan->flops;
+    info[4] = (double)plan->class_count[0];
+    info[5] = (double)plan->class_count[1];
+    info[6] = (double)plan->nblocks;
+    info[7] = (double)plan->prec;
+    return 0;
+}
+
+int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* const* a, const float* const* b,
+                            float beta, float* const* c, void* stream) {
+    const VbPlan* plan = (const VbPlan*)handle;
+    if (!plan) {
+        set_error("mbh_sgemm_vbatched_exec: null plan");
+        return MBH_E_ARG;
+    }
+    if (plan->count == 0 || plan->nitems == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (alpha == 0.0f) {
+        if (beta == 1.0f) return 0;
+        vb_scale_kernel<<<dim3((unsigned)std::min<int64_t>(plan->count, 65535)), 256, 0, s>>>(plan->probs,
+                                                                                               (int)plan->count,
+                                                                                               beta, c);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? 0 : hip_fail(e, "vbatched scale");
+    }
+    vb_reset_kernel<<<1, 1, 0, s>>>(plan->ctr);
+    if (plan->prec == MBH_PREC_BF16) {
+        vb_pack_kernel<<<dim3((unsigned)plan->nblocks), 256, 0, s>>>(plan->probs, plan->blocks, a, b, plan->pack);
+        vb_gemm_bf16_kernel<<<dim3((unsigned)plan->grid), 256, 0, s>>>(plan->probs, plan->items, plan->nitems,
+                                                                       plan->ctr, alpha, beta, plan->pack, c);
+    } else {
+        vb_gemm_f32_kernel<<<dim3((unsigned)plan->grid), 256, 0, s>>>(plan->probs, plan->items, plan->nitems,
+                                                                      plan->ctr, alpha, beta, a, b, c);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "vbatched exec");
+}
+
+int mbh_vbatch_plan_destroy(mbh_vbatch_plan handle) {
+    VbPlan* plan = (VbPlan*)handle;
+    if (!plan) return 0;
+    // The plan's buffers may still be read by queued work: wait for the device.
+    hipError_t e = hipDeviceSynchronize();
+    if (plan->probs) (void)hipFree(plan->probs);
+    if (plan->items) (void)hipFree(plan->items);
+    if (plan->blocks) (void)hipFree(plan->blocks);
+    if (plan->ctr) (void)hipFree(plan->ctr);
+    if (plan->pack) (void)hipFree(plan->pack);
+    delete plan;
+    return e == hipSuccess ? 0 : hip_fail(e, "mbh_vbatch_plan_destroy");
+}
+
+int mbh_sgemm_vbatched_dev(int64_t count, const int64_t* m, const int64_t* n, const int64_t* k, float alpha,
+                           const float* const* a, const int64_t* lda, const float* const* b, const int64_t* ldb,
+                           float beta, float* const* c, const int64_t* ldc, int precision, void* stream) {
+    mbh_vbatch_plan plan = nullptr;
+    int rc = mbh_vbatch_plan_create(count, m, n, k, lda, ldb, ldc, precision, &plan);
+    if (rc) return rc;
+    rc = mbh_sgemm_vbatched_exec(plan, alpha, a, b, beta, c, stream);
+    int rc2 = mbh_vbatch_plan_destroy(plan);
+    return rc ? rc : rc2;
+}
+
+}  // extern "C"

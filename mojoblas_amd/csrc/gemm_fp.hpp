@@ -142,22 +142,31 @@ constexpr int lds_stride() {
     return KCONTIG ? ROWS + (sizeof(T) == 4 ? 2 : 1) : ROWS + 16 / (int)sizeof(TIN);
 }
 
+// LDS elements (of T) one buffer pair of a tile needs.
+template <typename T, typename TA, typename TB, bool AT, bool BT, int BM, int BN, int BK>
+struct FpTileLds {
+    static constexpr int SA = lds_stride<T, TA, AT, BM>();
+    static constexpr int SB = lds_stride<T, TB, !BT, BN>();
+    static constexpr int A = BK * SA, B = BK * SB;
+    static constexpr int TOTAL = 2 * (A + B);  // double-buffered
+};
+
+// One BM x BN tile (tm, tn) of C := alpha*op(A)*op(B) + beta*C on 256
+// threads (2 x 2 waves); `lds` holds FpTileLds::TOTAL elements, 16-byte
+// aligned.  Shared by the grid kernel below and the persistent batched one.
 template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p) {
+__device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, int tn, T* lds) {
     typedef MfmaFp<T> M;
+    typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
     constexpr int NT = 256;
     constexpr int WM = BM / 2, WN = BN / 2;  // 2 x 2 waves
     constexpr int TM = WM / M::MT, TN = WN / M::MT;
     constexpr bool AK = AT;    // A 'T': op(A)(i,k) = A[k + i*lda] is k-contiguous
     constexpr bool BK_ = !BT;  // B 'N': op(B)(k,j) = B[k + j*ldb] is k-contiguous
-    constexpr int SA = lds_stride<T, TA, AK, BM>();
-    constexpr int SB = lds_stride<T, TB, BK_, BN>();
-    __shared__ __attribute__((aligned(16))) T As[2][BK * SA];
-    __shared__ __attribute__((aligned(16))) T Bs[2][BK * SB];
+    constexpr int SA = L::SA, SB = L::SB;
+    T* As[2] = {lds, lds + L::A};
+    T* Bs[2] = {lds + 2 * L::A, lds + 2 * L::A + L::B};
 
-    int tm, tn;
-    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
-    else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
@@ -232,6 +241,17 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p)
             }
         }
     }
+}
+
+
+template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p) {
+    typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
+    __shared__ __attribute__((aligned(16))) T lds[L::TOTAL];
+    int tm, tn;
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
+    else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+    gemm_fp_tile<T, TA, TB, TC, AT, BT, BM, BN, BK>(p, tm, tn, lds);
 }
 
 }  // namespace mbh

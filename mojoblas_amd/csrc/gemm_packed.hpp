@@ -105,8 +105,21 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* C, int
     }
 }
 
+// LDS (in 16-byte chunks) of one double-buffered tile: 64-byte rows, one or
+// two planes per operand.
+template <int NPROD, int BM, int BN>
+struct PackedTileLds {
+    static constexpr int NPL = (NPROD == 3) ? 2 : 1;
+    static constexpr int A = BM * 4, B = BN * 4;  // per plane per buffer
+    static constexpr int TOTAL = 2 * NPL * (A + B);
+};
+
+// One BM x BN tile (tm, tn) over packed panels: the panel of tile row tm
+// starts at chunk tm*ktiles*BM*4 of Ah/Al (likewise B), `lds` holds
+// PackedTileLds::TOTAL chunks.  Shared by the grid kernel and the persistent
+// batched kernel.
 template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
-__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(PackedGemmParams p) {
+__device__ inline void gemm_packed_tile(const PackedGemmParams& p, int tm, int tn, uint4* lds) {
     constexpr bool FP8 = (NPROD == 8);
     constexpr int BK = FP8 ? 64 : 32;  // k per 64-byte LDS row
     constexpr int NT = 64 * WAVES_M * WAVES_N;
@@ -117,13 +130,18 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     constexpr int B_CH = BN * 4;
     constexpr int A_LD = A_CH / NT, B_LD = B_CH / NT;  // chunks per thread per plane
     static_assert(A_LD * NT == A_CH && B_LD * NT == B_CH, "tile not divisible by the thread block");
-    // LDS: [buf][plane][rows * 64 bytes] as uint4 (16 bytes)
-    __shared__ uint4 As[2][NPL][BM * 4];
-    __shared__ uint4 Bs[2][NPL][BN * 4];
+    typedef PackedTileLds<NPROD, BM, BN> L;
+    // LDS views: As[buf][plane] / Bs[buf][plane], each rows * 4 chunks.
+    uint4* As[2][NPL];
+    uint4* Bs[2][NPL];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl) {
+            As[b][pl] = lds + (b * NPL + pl) * L::A;
+            Bs[b][pl] = lds + 2 * NPL * L::A + (b * NPL + pl) * L::B;
+        }
 
-    int tm, tn;
-    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
-    else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
 
@@ -138,43 +156,42 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
         gAl = reinterpret_cast<const uint4*>(p.Al) + a_off;
         gBl = reinterpret_cast<const uint4*>(p.Bl) + b_off;
     }
-    // LDS element offsets (in uint4) of this thread's two staging chunks per
-    // plane; the same for every k-tile.  Every configuration moves exactly two
-    // 16-byte chunks per plane per thread, held in named registers (arrays
-    // indexed inside the k-loop were demoted to memory by the compiler).
-    static_assert(A_LD == 2 && B_LD == 2, "staging assumes two chunks per plane per thread");
-    const int cA0 = tid, cA1 = tid + NT;
-    const int sA0 = (cA0 >> 2) * 4 + ((cA0 & 3) ^ lds_swz(cA0 >> 2));
-    const int sA1 = (cA1 >> 2) * 4 + ((cA1 & 3) ^ lds_swz(cA1 >> 2));
-    const int sB0 = sA0, sB1 = (BN == BM) ? sA1 : ((cA1 >> 2) * 4 + ((cA1 & 3) ^ lds_swz(cA1 >> 2)));
+    // This thread stages chunk tid (and tid + NT when a plane tile has 2*NT
+    // chunks) of every plane; s0/s1 are their swizzled LDS offsets (in uint4),
+    // the same for every k-tile.  The chunks live in named registers: arrays
+    // indexed inside the k-loop were demoted to memory by the compiler.
+    static_assert(A_LD >= 1 && A_LD <= 2 && B_LD >= 1 && B_LD <= 2, "staging holds one or two chunks per plane");
+    const int c0 = tid, c1 = tid + NT;
+    const int s0 = (c0 >> 2) * 4 + ((c0 & 3) ^ lds_swz(c0 >> 2));
+    const int s1 = (c1 >> 2) * 4 + ((c1 & 3) ^ lds_swz(c1 >> 2));
 
     uint4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
-#define MBH_GLOAD(kt)                                             \
-    do {                                                          \
+#define MBH_GLOAD(kt)                                                    \
+    do {                                                                 \
         const size_t oa = (size_t)(kt) * A_CH, ob = (size_t)(kt) * B_CH; \
-        ah0 = gAh[oa];                                            \
-        ah1 = gAh[oa + NT];                                       \
-        bh0 = gBh[ob];                                            \
-        bh1 = gBh[ob + NT];                                       \
-        if (NPL == 2) {                                           \
-            al0 = gAl[oa];                                        \
-            al1 = gAl[oa + NT];                                   \
-            bl0 = gBl[ob];                                        \
-            bl1 = gBl[ob + NT];                                   \
-        }                                                         \
+        ah0 = gAh[oa];                                                   \
+        if (A_LD == 2) ah1 = gAh[oa + NT];                               \
+        bh0 = gBh[ob];                                                   \
+        if (B_LD == 2) bh1 = gBh[ob + NT];                               \
+        if (NPL == 2) {                                                  \
+            al0 = gAl[oa];                                               \
+            if (A_LD == 2) al1 = gAl[oa + NT];                           \
+            bl0 = gBl[ob];                                               \
+            if (B_LD == 2) bl1 = gBl[ob + NT];                           \
+        }                                                                \
     } while (0)
-#define MBH_LSTORE(buf)                                           \
-    do {                                                          \
-        As[buf][0][sA0] = ah0;                                    \
-        As[buf][0][sA1] = ah1;                                    \
-        Bs[buf][0][sB0] = bh0;                                    \
-        Bs[buf][0][sB1] = bh1;                                    \
-        if (NPL == 2) {                                           \
-            As[buf][NPL - 1][sA0] = al0;                          \
-            As[buf][NPL - 1][sA1] = al1;                          \
-            Bs[buf][NPL - 1][sB0] = bl0;                          \
-            Bs[buf][NPL - 1][sB1] = bl1;                          \
-        }                                                         \
+#define MBH_LSTORE(buf)                                                  \
+    do {                                                                 \
+        As[buf][0][s0] = ah0;                                            \
+        if (A_LD == 2) As[buf][0][s1] = ah1;                             \
+        Bs[buf][0][s0] = bh0;                                            \
+        if (B_LD == 2) Bs[buf][0][s1] = bh1;                             \
+        if (NPL == 2) {                                                  \
+            As[buf][NPL - 1][s0] = al0;                                  \
+            if (A_LD == 2) As[buf][NPL - 1][s1] = al1;                   \
+            Bs[buf][NPL - 1][s0] = bl0;                                  \
+            if (B_LD == 2) Bs[buf][NPL - 1][s1] = bl1;                   \
+        }                                                                \
     } while (0)
 
     f32x16 acc[TM][TN];
@@ -260,6 +277,15 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
 
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     epilogue_32x32<TM, TN>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
+}
+
+template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
+__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(PackedGemmParams p) {
+    __shared__ uint4 lds[PackedTileLds<NPROD, BM, BN>::TOTAL];
+    int tm, tn;
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
+    else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+    gemm_packed_tile<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds);
 }
 
 }  // namespace mbh

@@ -39,14 +39,13 @@ struct PackParams {
     int vec;          // X 16-byte aligned and ld % 4 == 0 (vector loads along k)
 };
 
+// One 64-row x 32-k block (r0, k0) of the split pack; `s` is a 64 x 33
+// float LDS tile.  Shared by the grid kernel and the batched pack kernel.
 template <bool KCONTIG, int PR, int PK>
-__global__ __launch_bounds__(256) void pack_split_kernel(PackParams p) {
+__device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t k0, float (*s)[33]) {
     constexpr int BR = 64, BKK = 32;  // fp32 block moved per workgroup
     static_assert(PR % BR == 0 && PK % BKK == 0, "pack block must tile the gemm block");
-    __shared__ float s[BR][BKK + 1];
     const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * BR;
-    const int64_t k0 = (int64_t)blockIdx.y * BKK;
     if (KCONTIG) {
         // element (r, k) at X[k + r*ld]: 8 lanes cover one row's 32 k.
 #pragma unroll
@@ -98,6 +97,12 @@ __global__ __launch_bounds__(256) void pack_split_kernel(PackParams p) {
         lv.w = l[6] | (l[7] << 16);
         *reinterpret_cast<uint4*>(p.lo + off) = lv;
     }
+}
+
+template <bool KCONTIG, int PR, int PK>
+__global__ __launch_bounds__(256) void pack_split_kernel(PackParams p) {
+    __shared__ float s[64][33];
+    pack_split_block<KCONTIG, PR, PK>(p, (int64_t)blockIdx.x * 64, (int64_t)blockIdx.y * 32, s);
 }
 
 }  // namespace mbh

@@ -20,8 +20,10 @@ def rep_error(precision):
 
 
 def gemm_bound(precision, k, alpha, abs_ab, beta, c0):
-    """Componentwise bound  (e_rep + 2 k u)|alpha| |A||B| + 4u|beta C| + tiny."""
-    e = rep_error(precision) + 2.0 * max(k, 1) * U32
+    """Componentwise bound  (e_rep + (2k + 2) u)|alpha| |A||B| + 4u|beta C| + tiny:
+    k products and k additions in fp32 (gamma_2k), plus the epilogue's
+    rounding of alpha*acc and of the beta*C addition."""
+    e = rep_error(precision) + (2.0 * max(k, 1) + 2.0) * U32
     b = e * abs(alpha) * abs_ab + 4.0 * U32 * np.abs(beta * c0)
     return b + 1e-30
 

@@ -97,3 +97,16 @@ def test_invalid_precision_rejected():
                          ctypes.c_void_p(a.ctypes.data), 2, 0.0, ctypes.c_void_p(a.ctypes.data), 2, 99, None)
     assert rc == -3
     assert "precision" in _lib.last_error()
+
+
+def test_batched_plan_validates_every_problem_without_gpu():
+    from mojoblas_amd import batched
+
+    with pytest.raises(mb.HipBlasError, match="problem 1 fails gemm argument 8"):
+        batched.VBatchPlan([4, 4], [4, 4], [4, 4], lda=[4, 3])
+    with pytest.raises(mb.HipBlasError, match="problem 0 fails gemm argument 3"):
+        batched.VBatchPlan([-1], [4], [4])
+    with pytest.raises(mb.HipBlasError, match="precision"):
+        batched.VBatchPlan([4], [4], [4], precision="bf16x3")
+    with pytest.raises(mb.HipBlasError, match="classes are 0 and 1"):
+        batched.set_tuning([2] * 36)

@@ -1,0 +1,127 @@
+"""GPU parity of the variable-size batched gemm, problem by problem, against
+the fp64 oracle (the CPU gemm), in fp32 and bf16-input modes; plus hipGraph
+capture and replay of the exec.
+
+Bounds per problem (componentwise): (e_rep + (2k+2) u)|alpha||A||B| + 4u|beta C|,
+e_rep = 0 for fp32 and 2*2^-8 + 2^-16 for bf16 operands (|a - rn_bf16(a)| <=
+2^-8 |a| per operand).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers as H
+
+torch = pytest.importorskip("torch")
+import mojoblas_amd as mb
+from mojoblas_amd import batched
+
+pytestmark = pytest.mark.gpu
+E_REP = {"fp32": 0.0, "bf16": 2 * 2.0 ** -8 + 2.0 ** -16}
+
+
+def _check_problems(prec, m, n, k, bufs, offs, alpha, beta, c0, which, sample=None):
+    A, B, C = (t.cpu().numpy() for t in bufs)
+    aoff, boff, coff = offs
+    for p in which:
+        mp, np_, kp = int(m[p]), int(n[p]), int(k[p])
+        if mp == 0 or np_ == 0:
+            continue
+        a = A[aoff[p]:aoff[p] + mp * kp]
+        b = B[boff[p]:boff[p] + kp * np_]
+        got = C[coff[p]:coff[p] + mp * np_].astype(np.float64)
+        cin = c0[coff[p]:coff[p] + mp * np_]
+        if sample is None:
+            want = cin.astype(np.float64).copy()
+            oracle.dgemm("N", "N", mp, np_, kp, alpha, a.astype(np.float64), max(mp, 1), b.astype(np.float64),
+                         max(kp, 1), beta, want, max(mp, 1))
+            absab = (np.abs(a.reshape(kp, mp).T.astype(np.float64)) @ np.abs(b.reshape(np_, kp).T.astype(np.float64))
+                     ).T.ravel() if kp else np.zeros(mp * np_)
+            idx = slice(None)
+        else:
+            rng = np.random.default_rng(p)
+            rows = rng.integers(0, mp, sample)
+            cols = rng.integers(0, np_, sample)
+            want = oracle.sgemm_sample("N", "N", kp, alpha, a, max(mp, 1), b, max(kp, 1), beta,
+                                       cin[rows + cols * mp], rows, cols)
+            absab = oracle.sgemm_sample_abs("N", "N", kp, a, max(mp, 1), b, max(kp, 1), rows, cols)
+            idx = rows + cols * mp
+            cin = cin[idx]
+        e = E_REP[prec] + (2.0 * max(kp, 1) + 2.0) * H.U32
+        bound = e * abs(alpha) * absab + 4 * H.U32 * np.abs(beta * cin.astype(np.float64)) + 1e-30
+        err = np.abs(got[idx] - want)
+        assert (err <= bound).all(), (prec, p, mp, np_, kp, float((err / bound).max()))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_ragged_batch_vs_oracle(prec):
+    rng = np.random.default_rng(40)
+    count = 96
+    m = np.floor(np.exp(rng.uniform(0, np.log(300), count))).astype(np.int64)
+    n = np.floor(np.exp(rng.uniform(0, np.log(300), count))).astype(np.int64)
+    k = np.floor(np.exp(rng.uniform(0, np.log(300), count))).astype(np.int64)
+    m[3], n[5], k[7] = 0, 0, 0  # empty problems and a k == 0 problem
+    g = torch.Generator(device="cuda").manual_seed(41)
+    bufs, offs, ptrs = batched.pack_problems(torch, m, n, k, gen=g)
+    bufs[2].uniform_(-1, 1, generator=g)
+    c0 = bufs[2].cpu().numpy().copy()
+    plan = batched.VBatchPlan(m, n, k, precision=prec)
+    plan.exec(1.25, ptrs[0], ptrs[1], -0.5, ptrs[2])
+    torch.cuda.synchronize()
+    _check_problems(prec, m, n, k, bufs, offs, 1.25, -0.5, c0, range(count))
+    info = plan.info()
+    assert info["count"] == count and info["problems_64"] + info["problems_128"] == count - 2
+    plan.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_config4_4096_problems(prec):
+    """Config size: 4096 N,N problems, m, n, k log-uniform in [16, 512],
+    U[-1,1], alpha=1, beta=0, ld = rows.  Every problem is priced on 8 sampled
+    entries by the oracle, every 64th problem in full."""
+    m, n, k = batched.config4_shapes()
+    g = torch.Generator(device="cuda").manual_seed(42)
+    bufs, offs, ptrs = batched.pack_problems(torch, m, n, k, gen=g)
+    bufs[2].fill_(float("nan"))  # beta == 0: C is never read
+    c0 = np.zeros(bufs[2].numel(), np.float32)
+    plan = batched.VBatchPlan(m, n, k, precision=prec)
+    plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2])
+    torch.cuda.synchronize()
+    _check_problems(prec, m, n, k, bufs, offs, 1.0, 0.0, c0, range(len(m)), sample=8)
+    _check_problems(prec, m, n, k, bufs, offs, 1.0, 0.0, c0, range(0, len(m), 64))
+    plan.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_graph_capture_replays_match_eager(prec):
+    m, n, k = batched.config4_shapes(count=512, seed=7)
+    g = torch.Generator(device="cuda").manual_seed(43)
+    bufs, offs, ptrs = batched.pack_problems(torch, m, n, k, gen=g)
+    plan = batched.VBatchPlan(m, n, k, precision=prec)
+    plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2])
+    torch.cuda.synchronize()
+    eager = bufs[2].clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2])  # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
+        plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2])
+    for rep in range(3):
+        bufs[2].zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(bufs[2], eager), "replay %d differs from eager" % rep
+        # eager work between replays
+        bufs[2].mul_(2.0)
+        torch.cuda.synchronize()
+    plan.close()
+
+
+def test_invalid_problem_is_rejected_before_gpu_work():
+    with pytest.raises(mb.HipBlasError, match="problem 1 fails gemm argument 8"):
+        batched.VBatchPlan([4, 4], [4, 4], [4, 4], lda=[4, 3])

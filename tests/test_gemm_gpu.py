@@ -1,7 +1,7 @@
 """GPU parity of gemm (all four transposes, alpha/beta, lda/ldb/ldc, edge
 tiles) against the fp64 oracle, through the C-ABI (device and host entries).
 
-Bound, componentwise: |C - C_oracle| <= (e_rep + 2 k u)|alpha| |A||B| + 4u|beta C|
+Bound, componentwise: |C - C_oracle| <= (e_rep + (2k+2) u)|alpha| |A||B| + 4u|beta C|
 with u = 2^-24 and e_rep = 0 (fp32) or 3*2^-16 (bf16x3, see helpers.rep_error).
 FP8 (e4m3, per-128x128-tile scales): bit-exact on representable inputs and
 within the per-element rounding model |x - qs| <= 2^-4|x| + 2^-10 s otherwise.

@@ -3,7 +3,7 @@ through the C-ABI (device entries with torch tensors, host entries with numpy).
 
 Bounds:
   trmm/syrk/syr2k/symm (products): componentwise
-      |X - X_oracle| <= (e_rep + 2 k u)|alpha| P + 4u|beta C|
+      |X - X_oracle| <= (e_rep + (2k+2) u)|alpha| P + 4u|beta C|
     with P the matching |A||B| product (helpers.gemm_bound);
   trsm: normwise on the solution, max|X - X_oracle| <= tol * max|X_oracle|,
     tol = 2e-6 for fp32 data (one fp64-residual refinement step brings the
