@@ -164,9 +164,10 @@ int mbh_dsymm_dev(char side, char uplo, int64_t m, int64_t n, double alpha, cons
  *                accumulate).  Calls sharing one plan must not overlap.
  *  plan_info   : out[8] = {count, tiles, workgroups, flops, problems on
  *                64x64 tiles, on 128x128 tiles, pack blocks, precision}.
- *  set_tuning  : class table (36 ints, [bucket(m)][bucket(n)], 0 = 64x64,
- *                1 = 128x128; NULL = compiled default) and workgroups per CU
- *                (0 = default) for plans created afterwards. */
+ *  set_tuning  : for one precision, the class table (36 ints,
+ *                [bucket(m)][bucket(n)], buckets <=16,32,64,128,256,larger;
+ *                0 = 64x64 tiles, 1 = 128x128; NULL = compiled default) and
+ *                workgroups per CU (0 = default) for plans created afterwards. */
 typedef struct mbh_vbatch_plan_s* mbh_vbatch_plan;
 int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, const int64_t* k,
                            const int64_t* lda, const int64_t* ldb, const int64_t* ldc, int precision,
@@ -175,7 +176,7 @@ int mbh_sgemm_vbatched_exec(mbh_vbatch_plan plan, float alpha, const float* cons
                             const float* const* b, float beta, float* const* c, void* stream);
 int mbh_vbatch_plan_info(mbh_vbatch_plan plan, double* out);
 int mbh_vbatch_plan_destroy(mbh_vbatch_plan plan);
-int mbh_vbatch_set_tuning(const int* class_table, int wgs_per_cu);
+int mbh_vbatch_set_tuning(int precision, const int* class_table, int wgs_per_cu);
 /* One-shot convenience: plan_create + exec + plan_destroy (synchronous). */
 int mbh_sgemm_vbatched_dev(int64_t count, const int64_t* m, const int64_t* n, const int64_t* k,
                            float alpha, const float* const* a, const int64_t* lda,

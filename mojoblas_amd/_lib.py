@@ -56,7 +56,7 @@ SIGNATURES = {
     "mbh_sgemm_vbatched_exec": ([vp, f32, vp, vp, f32, vp, vp], i32),
     "mbh_vbatch_plan_info": ([vp, vp], i32),
     "mbh_vbatch_plan_destroy": ([vp], i32),
-    "mbh_vbatch_set_tuning": ([vp, i32], i32),
+    "mbh_vbatch_set_tuning": ([i32, vp, i32], i32),
     "mbh_sgemm_vbatched_dev": ([i64, vp, vp, vp, f32, vp, vp, vp, vp, f32, vp, vp, i32, vp], i32),
 }
 

@@ -73,14 +73,16 @@ class VBatchPlan:
             pass
 
 
-def set_tuning(class_table=None, wgs_per_cu=0):
-    """Override the tile-class table (6x6 of 0/1) and workgroups per CU for
-    plans created afterwards; None / 0 restore the compiled defaults."""
+def set_tuning(class_table=None, wgs_per_cu=0, precision="fp32"):
+    """Override, for one precision, the tile-class table (6x6 of 0/1, indexed
+    [bucket(m)][bucket(n)]) and workgroups per CU for plans created
+    afterwards; None / 0 restore the compiled defaults."""
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     if class_table is None:
-        rc = _lib.lib().mbh_vbatch_set_tuning(None, int(wgs_per_cu))
+        rc = _lib.lib().mbh_vbatch_set_tuning(prec, None, int(wgs_per_cu))
     else:
         t = np.ascontiguousarray(np.asarray(class_table, dtype=np.int32).reshape(36))
-        rc = _lib.lib().mbh_vbatch_set_tuning(ctypes.c_void_p(t.ctypes.data), int(wgs_per_cu))
+        rc = _lib.lib().mbh_vbatch_set_tuning(prec, ctypes.c_void_p(t.ctypes.data), int(wgs_per_cu))
     _check(rc, "mbh_vbatch_set_tuning")
 
 

@@ -61,24 +61,37 @@ constexpr int FP_LDS =
         : FpTileLds<float, float, float, false, false, 64, 64, 16>::TOTAL;
 constexpr int BF_LDS = PackedTileLds<1, 128, 128>::TOTAL;
 
-// Claims the next work item for the whole workgroup (-1 when drained).
-__device__ inline int vb_next(int* ctr, int nitems, int* slot) {
-    __syncthreads();  // everyone has consumed the previous value of *slot
+// Work-queue walk shared by the persistent kernels: thread 0 claims item
+// i+1 from the counter while the workgroup computes item i, so the atomic's
+// latency hides behind the tile.  Every workgroup leaves once the counter
+// passes nitems.
+template <typename F>
+__device__ inline void vb_walk(int* ctr, int nitems, int* slot, F&& body) {
     if (threadIdx.x == 0) *slot = atomicAdd(ctr, 1);
     __syncthreads();
-    const int it = *slot;
-    return it < nitems ? it : -1;
+    int it = *slot;
+    while (it < nitems) {
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(ctr, 1);
+        body(it);
+        __syncthreads();  // everyone is done with *slot and with the tile's LDS
+        if (threadIdx.x == 0) *slot = next;
+        __syncthreads();
+        it = *slot;
+    }
 }
 
+// MAXCLS: highest tile class compiled in.  The 64x64-only variant needs far
+// fewer registers and LDS, so more workgroups stay resident per CU.
+template <int MAXCLS>
 __global__ __launch_bounds__(256) void vb_gemm_f32_kernel(const VbProblem* __restrict__ probs,
                                                           const VbItem* __restrict__ items, int nitems, int* ctr,
                                                           float alpha, float beta, const float* const* A,
                                                           const float* const* B, float* const* C) {
-    __shared__ __attribute__((aligned(16))) float lds[FP_LDS];
+    constexpr int LDS = MAXCLS == 0 ? FpTileLds<float, float, float, false, false, 64, 64, 16>::TOTAL : FP_LDS;
+    __shared__ __attribute__((aligned(16))) float lds[LDS];
     __shared__ int slot;
-    for (;;) {
-        const int it = vb_next(ctr, nitems, &slot);
-        if (it < 0) break;
+    vb_walk(ctr, nitems, &slot, [&](int it) {
         const VbItem w = items[it];
         const VbProblem pr = probs[w.prob];
         GemmParams<float> p{};
@@ -96,11 +109,11 @@ __global__ __launch_bounds__(256) void vb_gemm_f32_kernel(const VbProblem* __res
         p.a_vec = ((uintptr_t)p.A % 16 == 0) && (pr.lda % 4 == 0);
         p.b_vec = ((uintptr_t)p.B % 16 == 0) && (pr.ldb % 4 == 0);
         const int tm = w.tile / pr.tiles_n, tn = w.tile % pr.tiles_n;
-        if (pr.cls == 0)
+        if (MAXCLS == 0 || pr.cls == 0)
             gemm_fp_tile<float, float, float, float, false, false, 64, 64, 16>(p, tm, tn, lds);
         else
             gemm_fp_tile<float, float, float, float, false, false, 128, 128, 16>(p, tm, tn, lds);
-    }
+    });
 }
 
 // bf16 pack of every problem: block b moves 64 rows x 32 k of op(A) (m x k,
@@ -138,15 +151,15 @@ __global__ __launch_bounds__(256) void vb_pack_kernel(const VbProblem* __restric
     }
 }
 
+template <int MAXCLS>
 __global__ __launch_bounds__(256) void vb_gemm_bf16_kernel(const VbProblem* __restrict__ probs,
                                                            const VbItem* __restrict__ items, int nitems, int* ctr,
                                                            float alpha, float beta, const uint16_t* pack,
                                                            float* const* C) {
-    __shared__ uint4 lds[BF_LDS];
+    constexpr int LDS = MAXCLS == 0 ? PackedTileLds<1, 64, 64>::TOTAL : BF_LDS;
+    __shared__ uint4 lds[LDS];
     __shared__ int slot;
-    for (;;) {
-        const int it = vb_next(ctr, nitems, &slot);
-        if (it < 0) break;
+    vb_walk(ctr, nitems, &slot, [&](int it) {
         const VbItem w = items[it];
         const VbProblem pr = probs[w.prob];
         PackedGemmParams p{};
@@ -160,9 +173,9 @@ __global__ __launch_bounds__(256) void vb_gemm_bf16_kernel(const VbProblem* __re
         p.alpha = alpha;
         p.beta = beta;
         const int tm = w.tile / pr.tiles_n, tn = w.tile % pr.tiles_n;
-        if (pr.cls == 0) gemm_packed_tile<1, 64, 64, 2, 2>(p, tm, tn, lds);
+        if (MAXCLS == 0 || pr.cls == 0) gemm_packed_tile<1, 64, 64, 2, 2>(p, tm, tn, lds);
         else gemm_packed_tile<1, 128, 128, 2, 2>(p, tm, tn, lds);
-    }
+    });
 }
 
 // C_p := beta * C_p for every problem (alpha == 0: A and B are not read).
@@ -180,15 +193,18 @@ __global__ void vb_scale_kernel(const VbProblem* __restrict__ probs, int count, 
 
 // ---- tuning state ------------------------------------------------------------
 std::mutex g_tune_mu;
-int g_class_table[VB_BUCKETS][VB_BUCKETS];
-int g_wgs_per_cu = VB_DEFAULT_WGS_PER_CU;
+int g_class_table[2][VB_BUCKETS][VB_BUCKETS];
+int g_wgs_per_cu[2];
 bool g_tune_init = false;
 
 void tune_init_locked() {
     if (g_tune_init) return;
     memcpy(g_class_table, VB_DEFAULT_CLASS, sizeof(g_class_table));
+    memcpy(g_wgs_per_cu, VB_DEFAULT_WGS_PER_CU, sizeof(g_wgs_per_cu));
     g_tune_init = true;
 }
+
+int prec_slot(int precision) { return precision == MBH_PREC_BF16 ? 1 : 0; }
 
 int bucket_of(int64_t x) {
     // 0: <=16, 1: <=32, 2: <=64, 3: <=128, 4: <=256, 5: larger
@@ -222,24 +238,27 @@ using namespace mbh;
 
 extern "C" {
 
-int mbh_vbatch_set_tuning(const int* class_table, int wgs_per_cu) {
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    tune_init_locked();
-    if (class_table) {
-        for (int i = 0; i < VB_BUCKETS * VB_BUCKETS; ++i)
-            if (class_table[i] < 0 || class_table[i] > 1) {
-                set_error("mbh_vbatch_set_tuning: class %d at %d (classes are 0 and 1)", class_table[i], i);
-                return MBH_E_ARG;
-            }
-        memcpy(g_class_table, class_table, sizeof(g_class_table));
-    } else {
-        memcpy(g_class_table, VB_DEFAULT_CLASS, sizeof(g_class_table));
+int mbh_vbatch_set_tuning(int precision, const int* class_table, int wgs_per_cu) {
+    if (precision != MBH_PREC_FP32 && precision != MBH_PREC_BF16) {
+        set_error("mbh_vbatch_set_tuning: precision %d (batched supports FP32 and BF16)", precision);
+        return MBH_E_PRECISION;
     }
     if (wgs_per_cu < 0 || wgs_per_cu > 8) {
         set_error("mbh_vbatch_set_tuning: wgs_per_cu %d outside [0, 8]", wgs_per_cu);
         return MBH_E_ARG;
     }
-    g_wgs_per_cu = wgs_per_cu ? wgs_per_cu : VB_DEFAULT_WGS_PER_CU;
+    if (class_table)
+        for (int i = 0; i < VB_BUCKETS * VB_BUCKETS; ++i)
+            if (class_table[i] < 0 || class_table[i] > 1) {
+                set_error("mbh_vbatch_set_tuning: class %d at %d (classes are 0 and 1)", class_table[i], i);
+                return MBH_E_ARG;
+            }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_init_locked();
+    const int ps = prec_slot(precision);
+    memcpy(g_class_table[ps], class_table ? (const void*)class_table : (const void*)VB_DEFAULT_CLASS[ps],
+           sizeof(g_class_table[ps]));
+    g_wgs_per_cu[ps] = wgs_per_cu ? wgs_per_cu : VB_DEFAULT_WGS_PER_CU[ps];
     return 0;
 }
 
@@ -275,8 +294,8 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         tune_init_locked();
-        memcpy(class_table, g_class_table, sizeof(class_table));
-        wgs = g_wgs_per_cu;
+        memcpy(class_table, g_class_table[prec_slot(precision)], sizeof(class_table));
+        wgs = g_wgs_per_cu[prec_slot(precision)];
     }
     VbPlan* plan = new VbPlan();
     plan->count = count;
@@ -404,13 +423,23 @@ int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* co
         return e == hipSuccess ? 0 : hip_fail(e, "vbatched scale");
     }
     vb_reset_kernel<<<1, 1, 0, s>>>(plan->ctr);
+    const dim3 grid((unsigned)plan->grid), block(256);
+    const bool mixed = plan->class_count[1] > 0;
     if (plan->prec == MBH_PREC_BF16) {
         vb_pack_kernel<<<dim3((unsigned)plan->nblocks), 256, 0, s>>>(plan->probs, plan->blocks, a, b, plan->pack);
-        vb_gemm_bf16_kernel<<<dim3((unsigned)plan->grid), 256, 0, s>>>(plan->probs, plan->items, plan->nitems,
-                                                                       plan->ctr, alpha, beta, plan->pack, c);
+        if (mixed)
+            vb_gemm_bf16_kernel<1><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
+                                                          beta, plan->pack, c);
+        else
+            vb_gemm_bf16_kernel<0><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
+                                                          beta, plan->pack, c);
     } else {
-        vb_gemm_f32_kernel<<<dim3((unsigned)plan->grid), 256, 0, s>>>(plan->probs, plan->items, plan->nitems,
-                                                                      plan->ctr, alpha, beta, a, b, c);
+        if (mixed)
+            vb_gemm_f32_kernel<1><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
+                                                         beta, a, b, c);
+        else
+            vb_gemm_f32_kernel<0><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
+                                                         beta, a, b, c);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "vbatched exec");
