@@ -183,6 +183,30 @@ int mbh_sgemm_vbatched_dev(int64_t count, const int64_t* m, const int64_t* n, co
                            const float* const* b, const int64_t* ldb, float beta, float* const* c,
                            const int64_t* ldc, int precision, void* stream);
 
+/* ---- adjoints (no counterpart in the reference) --------------------------
+ * Reverse-mode derivatives on the same kernels, for callers that
+ * differentiate through gemm / trsm (the Python layer's
+ * torch.autograd.Function binds these).
+ *  sgemm_backward: for C = alpha*op(A)*op(B) + beta*C0 and upstream dC,
+ *    dA = gradient w.r.t. A as stored (alpha*dC*op(B)^T, or its transpose for
+ *    transa T), dB likewise (alpha*op(A)^T*dC or its transpose); either output
+ *    may be NULL.  Shapes and flags are the forward call's (validated with the
+ *    gemm info codes); dC is m x n.
+ *  strsm_backward: for X = alpha*inv(op(A))*B (side L) or alpha*B*inv(op(A))
+ *    (side R) and upstream dX: dB = alpha*G with G = op(A)^-T dX (L) or
+ *    dX op(A)^-T (R); dA (optional) = -G X^T (L) or -X^T G (R) as a gradient
+ *    of op(A), returned for A as stored: only its `uplo` triangle is nonzero
+ *    and its diagonal is zero when diag is U.  The solve for G runs with one
+ *    fp64-residual refinement step. */
+int mbh_sgemm_backward(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                       const float* a, int64_t lda, const float* b, int64_t ldb, const float* dc,
+                       int64_t lddc, float* da, int64_t ldda, float* db, int64_t lddb, int precision,
+                       void* stream);
+int mbh_strsm_backward(char side, char uplo, char transa, char diag, int64_t m, int64_t n,
+                       float alpha, const float* a, int64_t lda, const float* x, int64_t ldx,
+                       const float* dx, int64_t lddx, float* db, int64_t lddb, float* da,
+                       int64_t ldda, int precision, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

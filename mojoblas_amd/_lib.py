@@ -58,6 +58,9 @@ SIGNATURES = {
     "mbh_vbatch_plan_destroy": ([vp], i32),
     "mbh_vbatch_set_tuning": ([i32, vp, i32], i32),
     "mbh_sgemm_vbatched_dev": ([i64, vp, vp, vp, f32, vp, vp, vp, vp, f32, vp, vp, i32, vp], i32),
+    "mbh_sgemm_backward": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, i32, vp], i32),
+    "mbh_strsm_backward": ([ch, ch, ch, ch, i64, i64, f32, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, i32, vp],
+                           i32),
 }
 
 
