@@ -30,9 +30,20 @@ def log(*a):
 
 
 def time_gemm(mb, torch, ta, tb, A, B, C, prec, steps, warmup):
+    """Whole-step time (events on torch's current stream, which is the launch
+    stream the library is handed) and, for the split-precision paths, the
+    library's own event timing of the main kernel and of the pack kernels."""
+    import ctypes
+
+    from mojoblas_amd import _lib
+
+    L = _lib.lib()
     for _ in range(warmup):
         mb.gemm(ta, tb, N, N, N, 1.0, A, N, B, N, 0.0, C, N, precision=prec)
     torch.cuda.synchronize()
+    L.mbh_timing_enable(1)
+    tm = np.zeros(4)
+    L.mbh_timing_read(ctypes.c_void_p(tm.ctypes.data), 1)
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     s.record()
@@ -42,7 +53,14 @@ def time_gemm(mb, torch, ta, tb, A, B, C, prec, steps, warmup):
     e.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return s.elapsed_time(e) / steps, wall / steps * 1e3
+    L.mbh_timing_read(ctypes.c_void_p(tm.ctypes.data), 1)
+    L.mbh_timing_enable(0)
+    step_ms = s.elapsed_time(e) / steps
+    if tm[1] > 0:
+        main_ms, pack_ms = tm[0] / tm[1], tm[2] / tm[3]
+    else:  # single-kernel path (fp32): the step is the kernel
+        main_ms, pack_ms = step_ms, 0.0
+    return step_ms, wall / steps * 1e3, main_ms, pack_ms
 
 
 def errors(torch, C, C64, ABabs):
@@ -90,6 +108,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 1 GiB device copy (known HBM bytes) to calibrate PMC byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the level-3 path runs on a single MI355X; "
@@ -104,13 +124,22 @@ def main():
     A = torch.randn(N * N, device="cuda", generator=g)
     B = torch.randn(N * N, device="cuda", generator=g)
     C = torch.empty(N * N, device="cuda")
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, device="cuda")  # 1 GiB read, 1 GiB written per copy
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     variants = [v.strip().upper() for v in args.variants.split(",") if v.strip()]
-    results = {}
+    results, kernel_ms = {}, {}
     for prec in ("bf16x3", "fp32", "fp8"):
         for v in variants:
-            ms, wall_ms = time_gemm(mb, torch, v[0], v[1], A, B, C, prec, args.steps, args.warmup)
+            ms, wall_ms, main_ms, pack_ms = time_gemm(mb, torch, v[0], v[1], A, B, C, prec, args.steps, args.warmup)
             results[(prec, v)] = ms
-            log("%-6s %s: %.3f ms  %.1f TFLOP/s (wall %.3f ms)" % (prec, v, ms, FLOPS / ms / 1e9, wall_ms))
+            kernel_ms[(prec, v)] = (main_ms, pack_ms)
+            log("%-6s %s: %.3f ms  %.1f TFLOP/s (main kernel %.3f ms = %.1f TFLOP/s, pack %.3f ms; wall %.3f ms)"
+                % (prec, v, ms, FLOPS / ms / 1e9, main_ms, FLOPS / main_ms / 1e9, pack_ms, wall_ms))
 
     err = {}
     if not args.no_errors:
@@ -137,7 +166,14 @@ def main():
     ms_bf = results[("bf16x3", "NN")]
     tf_bf = FLOPS / ms_bf / 1e9
     tf_fp = FLOPS / results[("fp32", "NN")] / 1e9
+    main_ms, pack_ms = kernel_ms[("bf16x3", "NN")]
+    tf_main = FLOPS / main_ms / 1e9
     peak = PEAK_BF16 / 3.0
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_packed_bf16x3_8192_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
     line = {
         "metric": "SGEMM TFLOP/s @8192^3: BF16x3-MFMA vs plain FP32 at stated FP64-oracle error",
         "value": round(tf_bf, 3),
@@ -161,13 +197,15 @@ def main():
             {k: round(err["bf16x3"][k] / err["fp32"][k], 3) for k in err["fp32"]} if err else None),
         "roofline": {
             "bound": "mfma",
-            "achieved": round(tf_bf, 3),
+            "achieved": round(tf_main, 3),
             "peak": round(peak, 1),
             "peak_basis": "MI355X dense bf16 MFMA 2516.6 TFLOP/s / 3 bf16 products per fp32 product",
             "unit": "TFLOP/s",
-            "frac": round(tf_bf / peak, 4),
-            "traffic": None,
-            "kernel": "gemm_bf16x3_kernel<false,false,...>",
+            "frac": round(tf_main / peak, 4),
+            "traffic": traffic,
+            "kernel": "gemm_packed_kernel<3,256,256,2,4>",
+            "kernel_ms": round(main_ms, 4),
+            "pack_ms_per_step": round(pack_ms, 4),
             "fp32_kernel_frac": round(tf_fp / PEAK_FP32, 4),
         },
     }

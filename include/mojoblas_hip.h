@@ -60,6 +60,14 @@ const char* mbh_last_error(void);                 /* thread-local message */
 int mbh_set_default_precision(int precision);     /* for _host entries; returns the old value */
 int mbh_get_default_precision(void);
 int mbh_device_sync(void);
+/* Opt-in timing for measurement harnesses: while enabled, every
+ * split-precision (BF16x3 / bf16 / FP8) gemm records HIP events on its launch
+ * stream before its pack kernels, before its main kernel and after it.
+ * timing_read waits for the recorded events and returns out[4] =
+ * {main-kernel ms summed, launches, pack ms summed, launches}; reset != 0
+ * starts a new tally.  enable returns the previous state. */
+int mbh_timing_enable(int on);
+int mbh_timing_read(double* out, int reset);
 
 /* ---- gemm: C := alpha*op(A)*op(B) + beta*C ------------------------------
  * Replaces gemm[dtype] (src/level3/gemm.mojo:56-124) for every transpose

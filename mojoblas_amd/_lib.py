@@ -28,6 +28,8 @@ SIGNATURES = {
     "mbh_set_default_precision": ([i32], i32),
     "mbh_get_default_precision": ([], i32),
     "mbh_device_sync": ([], i32),
+    "mbh_timing_enable": ([i32], i32),
+    "mbh_timing_read": ([vp, i32], i32),
     "mbh_sgemm_host": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64], i32),
     "mbh_dgemm_host": ([ch, ch, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64], i32),
     "mbh_sgemm_dev": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, i32, vp], i32),

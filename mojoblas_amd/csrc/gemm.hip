@@ -5,6 +5,7 @@
 #include "gemm_fp.hpp"
 #include "gemm_packed.hpp"
 #include "pack.hpp"
+#include "timing.hpp"
 
 namespace mbh {
 
@@ -60,6 +61,9 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     PackParams pa{g.A, g.lda, g.m, g.k, (int)ktiles, base, planes == 2 ? base + a_el : nullptr, g.a_vec};
     uint16_t* bbase = base + a_el * planes;
     PackParams pb{g.B, g.ldb, g.n, g.k, (int)ktiles, bbase, planes == 2 ? bbase + b_el : nullptr, g.b_vec};
+    TimingMark tmk;
+    tmk.active = timing_on();
+    timing_mark(tmk, 0, s);
     dim3 pblock(256);
     dim3 ga((unsigned)(tm * BT / 64), (unsigned)(ktiles * PK / 32));
     dim3 gb((unsigned)(tn * BT / 64), (unsigned)(ktiles * PK / 32));
@@ -86,7 +90,9 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     p.tri = g.tri;
     p.tri_tiles = tri_grid;
     p.krange = g.krange;
+    timing_mark(tmk, 1, s);
     gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)ntile), dim3(64 * WM_ * WN_), 0, s>>>(p);
+    timing_mark(tmk, 2, s);
     return hipGetLastError();
 }
 
@@ -109,6 +115,9 @@ hipError_t run_fp8(bool at, bool bt, const GemmParams<float>& g, int tri_grid, h
     int* sb = sa + na;
     PackFp8Params pa{g.A, g.lda, g.m, g.k, (int)ktiles, (int)kblocks, base, sa, g.a_vec};
     PackFp8Params pb{g.B, g.ldb, g.n, g.k, (int)ktiles, (int)kblocks, base + qa, sb, g.b_vec};
+    TimingMark tmk;
+    tmk.active = timing_on();
+    timing_mark(tmk, 0, s);
     dim3 ga((unsigned)(tm * BT / 128), (unsigned)kblocks), gb((unsigned)(tn * BT / 128), (unsigned)kblocks);
     if (at) pack_fp8_kernel<true, BT><<<ga, 256, 0, s>>>(pa);
     else pack_fp8_kernel<false, BT><<<ga, 256, 0, s>>>(pa);
@@ -133,7 +142,9 @@ hipError_t run_fp8(bool at, bool bt, const GemmParams<float>& g, int tri_grid, h
     p.sA = sa;
     p.sB = sb;
     p.kblocks = (int)kblocks;
+    timing_mark(tmk, 1, s);
     gemm_packed_kernel<8, BT, BT, WM_, WN_><<<dim3((unsigned)ntile), dim3(64 * WM_ * WN_), 0, s>>>(p);
+    timing_mark(tmk, 2, s);
     return hipGetLastError();
 }
 
