@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variants", default="NN,NT,TN,TT")
+    ap.add_argument("--precisions", default="bf16x3,fp32,fp8",
+                    help="subset to time (profiler passes); bf16x3 is always included")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
@@ -132,8 +134,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     variants = [v.strip().upper() for v in args.variants.split(",") if v.strip()]
+    precs = ["bf16x3"] + [p for p in ("fp32", "fp8") if p in args.precisions.split(",")]
+    if "NN" not in variants:
+        variants = ["NN"] + variants
     results, kernel_ms = {}, {}
-    for prec in ("bf16x3", "fp32", "fp8"):
+    for prec in precs:
         for v in variants:
             ms, wall_ms, main_ms, pack_ms = time_gemm(mb, torch, v[0], v[1], A, B, C, prec, args.steps, args.warmup)
             results[(prec, v)] = ms
@@ -156,7 +161,7 @@ def main():
         ref = oracle.sgemm_sample("N", "N", N, 1.0, A.cpu().numpy(), N, B.cpu().numpy(), N, 0.0, None, rows, cols)
         chk = C64[torch.from_numpy(rows).cuda(), torch.from_numpy(cols).cuda()].cpu().numpy()
         assert np.allclose(chk, ref, rtol=1e-12, atol=1e-9), "fp64 reference disagrees with the oracle"
-        for prec in ("bf16x3", "fp32", "fp8"):
+        for prec in precs:
             mb.gemm("N", "N", N, N, N, 1.0, A, N, B, N, 0.0, C, N, precision=prec)
             torch.cuda.synchronize()
             err[prec] = errors(torch, C.view(N, N).t(), C64, ABabs)
@@ -165,15 +170,19 @@ def main():
 
     ms_bf = results[("bf16x3", "NN")]
     tf_bf = FLOPS / ms_bf / 1e9
-    tf_fp = FLOPS / results[("fp32", "NN")] / 1e9
+    tf_fp = FLOPS / results[("fp32", "NN")] / 1e9 if ("fp32", "NN") in results else None
     main_ms, pack_ms = kernel_ms[("bf16x3", "NN")]
     tf_main = FLOPS / main_ms / 1e9
     peak = PEAK_BF16 / 3.0
     traffic = None
-    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_packed_bf16x3_8192_traffic.json")
+    # HBM bytes per launch of the main kernel, from the two rocprofv3 PMC
+    # passes (FETCH_SIZE, WRITE_SIZE) calibrated against a 1 GiB copy.
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_packed_bf16x3_8192_traffic.txt")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("traffic_bytes_per_launch")
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    traffic = int(ln.split(":")[1])
     line = {
         "metric": "SGEMM TFLOP/s @8192^3: BF16x3-MFMA vs plain FP32 at stated FP64-oracle error",
         "value": round(tf_bf, 3),
@@ -188,13 +197,13 @@ def main():
         "dtype": "f32 (bf16x3 split on bf16 MFMA, fp32 accumulate)",
         "data": "synthetic: A, B iid N(0,1) fp32 (torch cuda Generator seed 1234), column-major, resident in HBM",
         "config": {"workload": "SGEMM N,N m=n=k=8192 lda=ldb=ldc=8192 alpha=1 beta=0", "m": N, "n": N, "k": N},
-        "fp32_tflops": round(tf_fp, 3),
-        "fp8_tflops": round(FLOPS / results[("fp8", "NN")] / 1e9, 3),
-        "bf16x3_over_fp32": round(tf_bf / tf_fp, 3),
+        "fp32_tflops": round(tf_fp, 3) if tf_fp else None,
+        "fp8_tflops": round(FLOPS / results[("fp8", "NN")] / 1e9, 3) if ("fp8", "NN") in results else None,
+        "bf16x3_over_fp32": round(tf_bf / tf_fp, 3) if tf_fp else None,
         "variants_tflops": {"%s_%s" % (p, v): round(FLOPS / ms / 1e9, 3) for (p, v), ms in results.items()},
         "errors_vs_fp64": err,
         "error_ratio_bf16x3_over_fp32": (
-            {k: round(err["bf16x3"][k] / err["fp32"][k], 3) for k in err["fp32"]} if err else None),
+            {k: round(err["bf16x3"][k] / err["fp32"][k], 3) for k in err["fp32"]} if "fp32" in err else None),
         "roofline": {
             "bound": "mfma",
             "achieved": round(tf_main, 3),
@@ -206,7 +215,7 @@ def main():
             "kernel": "gemm_packed_kernel<3,256,256,2,4>",
             "kernel_ms": round(main_ms, 4),
             "pack_ms_per_step": round(pack_ms, 4),
-            "fp32_kernel_frac": round(tf_fp / PEAK_FP32, 4),
+            "fp32_kernel_frac": round(tf_fp / PEAK_FP32, 4) if tf_fp else None,
         },
     }
     if not args.no_cpu:
