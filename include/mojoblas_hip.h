@@ -3,9 +3,8 @@
  *
  * Drop-in boundary for mojoBLAS's src/level3 Mojo functions
  * (docs/reference.md:264-327).  A Mojo shim keeps the reference signature
- * and calls the matching entry here through `external_call` (bindings in
- * INTEGRATION.md); the Python layer (mojoblas_amd) binds the same symbols
- * with ctypes.
+ * and calls the matching entry here through `external_call`; the Python
+ * layer (mojoblas_amd) binds the same symbols with ctypes.
  *
  * Conventions (all entries):
  *  - Column-major; element (i,j) of X is X[i + j*ldx]; dimensions int64.
@@ -60,7 +59,8 @@ const char* mbh_last_error(void);                 /* thread-local message */
 int mbh_set_default_precision(int precision);     /* for _host entries; returns the old value */
 int mbh_get_default_precision(void);
 int mbh_device_sync(void);
-/* Opt-in timing for measurement harnesses: while enabled, every
+/* Opt-in timing for measurement harnesses (no counterpart in the reference,
+ * which has no GPU kernels to time): while enabled, every
  * split-precision (BF16x3 / bf16 / FP8) gemm records HIP events on its launch
  * stream before its pack kernels, before its main kernel and after it.
  * timing_read waits for the recorded events and returns out[4] =

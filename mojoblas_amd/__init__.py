@@ -139,14 +139,15 @@ def gemm(trans_a, trans_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, precisi
     return _finish("gemm", rc)
 
 
-def _call(name, prefix_args, ptr_args, tail_args, operands, precision, stream, host_extra=(), dev_extra_float=None,
-          dev_extra_double=()):
-    """Route one level-3 call to the s/d, host/dev entry of the C-ABI."""
+def _call(name, prefix_args, tail_args, operands, precision, stream, dev_extra_float=(), dev_extra_double=()):
+    """Route one level-3 call to the s/d, host/dev entry of the C-ABI.  The
+    device entries take the trailing `dev_extra_*` arguments and a stream;
+    the host entries take the precision from the library default."""
     L = _lib.lib()
     kind, dt = _common(*operands)
     t = "s" if dt == "float32" else "d"
     fn = getattr(L, "mbh_%s%s_%s" % (t, name, kind))
-    args = list(prefix_args) + list(ptr_args) + list(tail_args)
+    args = list(prefix_args) + list(tail_args)
     if kind == "dev":
         extra = (list(dev_extra_float) if t == "s" else list(dev_extra_double)) + [_stream(stream)]
         return _finish(name, fn(*(args + extra)))
@@ -170,7 +171,7 @@ def trsm(side, uplo, trans_a, diag, m, n, alpha, a, lda, b, ldb, precision=None,
     entry always refines once).
     """
     ptrs = [_p(a), lda, _p(b), ldb]
-    return _call("trsm", [_flag(side), _flag(uplo), _flag(trans_a), _flag(diag), m, n, alpha], [],
+    return _call("trsm", [_flag(side), _flag(uplo), _flag(trans_a), _flag(diag), m, n, alpha],
                  ptrs, (a, b), precision, stream, dev_extra_float=[_prec(precision), int(refine)])
 
 
@@ -178,21 +179,21 @@ def trmm(side, uplo, trans_a, diag, m, n, alpha, a, lda, b, ldb, precision=None,
     """B := alpha*op(A)*B (side L) or alpha*B*op(A) (side R), A triangular
     (src/level3/trmm.mojo:20-305)."""
     ptrs = [_p(a), lda, _p(b), ldb]
-    return _call("trmm", [_flag(side), _flag(uplo), _flag(trans_a), _flag(diag), m, n, alpha], [],
+    return _call("trmm", [_flag(side), _flag(uplo), _flag(trans_a), _flag(diag), m, n, alpha],
                  ptrs, (a, b), precision, stream, dev_extra_float=[_prec(precision)])
 
 
 def syrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, precision=None, stream=None):
     """C := alpha*A*A^T + beta*C (trans N) or alpha*A^T*A + beta*C on the
     `uplo` triangle (src/level3/syrk.mojo:20-200)."""
-    return _call("syrk", [_flag(uplo), _flag(trans), n, k, alpha], [], [_p(a), lda, beta, _p(c), ldc], (a, c),
+    return _call("syrk", [_flag(uplo), _flag(trans), n, k, alpha], [_p(a), lda, beta, _p(c), ldc], (a, c),
                  precision, stream, dev_extra_float=[_prec(precision)])
 
 
 def syr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc, precision=None, stream=None):
     """C := alpha*(A*B^T + B*A^T) + beta*C (trans N) or alpha*(A^T*B + B^T*A)
     + beta*C on the `uplo` triangle (src/level3/syr2k.mojo:20-190)."""
-    return _call("syr2k", [_flag(uplo), _flag(trans), n, k, alpha], [],
+    return _call("syr2k", [_flag(uplo), _flag(trans), n, k, alpha],
                  [_p(a), lda, _p(b), ldb, beta, _p(c), ldc], (a, b, c), precision, stream,
                  dev_extra_float=[_prec(precision)])
 
@@ -200,7 +201,7 @@ def syr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc, precision=None
 def symm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, precision=None, stream=None):
     """C := alpha*A*B + beta*C (side L) or alpha*B*A + beta*C (side R), A
     symmetric from its `uplo` triangle (src/level3/symm.mojo:20-241)."""
-    return _call("symm", [_flag(side), _flag(uplo), m, n, alpha], [],
+    return _call("symm", [_flag(side), _flag(uplo), m, n, alpha],
                  [_p(a), lda, _p(b), ldb, beta, _p(c), ldc], (a, b, c), precision, stream,
                  dev_extra_float=[_prec(precision)])
 

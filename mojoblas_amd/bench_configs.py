@@ -3,6 +3,8 @@
 
   cfg1   SGEMM N,N 512^3, lda=ldb=ldc=512, alpha=1.5, beta=0.5 (latency)
   cfg2b  SGEMM N,N m=n=4096, k=16384 (k-loop stress): fp32 / bf16x3 / fp8
+  cfg2h  SGEMM N,N 8192^3 through the host-pointer entry (mbh_sgemm_host):
+         wall time including the PCIe copies of A, B (and C back)
   cfg3a  strsm L,L,N,N m=n=4096 (diag U[1,2], off-diagonal U[-1,1]/n),
          fp32 and bf16x3 updates, with and without the refinement step
   cfg3b  ssyrk L,N n=8192, k=2048, alpha=1, beta=0: fp32 / bf16x3
@@ -84,6 +86,29 @@ def cfg2b(torch, mb, reps):
               "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK[prec], "error_vs_fp64": e})
 
 
+def cfg2h(torch, mb, reps):
+    import time
+
+    n = 8192
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal(n * n, dtype=np.float32)
+    b = rng.standard_normal(n * n, dtype=np.float32)
+    c = np.zeros(n * n, np.float32)
+    fl = 2.0 * n ** 3
+    for prec in ("fp32", "bf16x3"):
+        mb.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, precision=prec)
+        ts = []
+        for _ in range(max(3, reps // 3)):
+            t0 = time.perf_counter()
+            rc = mb.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, precision=prec)
+            ts.append((time.perf_counter() - t0) * 1e3)
+            assert rc == 0
+        ms = float(np.median(ts))
+        emit({"config": "cfg2h", "routine": "sgemm NN 8192^3, host pointers (copies in and out included)",
+              "precision": prec, "ms_wall": ms, "tflops": fl / ms / 1e9,
+              "pcie_bytes": 3 * 4 * n * n})
+
+
 def cfg3a(torch, mb, reps):
     m = n = 4096
     g = torch.Generator(device="cuda").manual_seed(3)
@@ -162,7 +187,7 @@ def cfg4(torch, mb, reps):
         plan.close()
 
 
-CONFIGS = {"cfg1": cfg1, "cfg2b": cfg2b, "cfg3a": cfg3a, "cfg3b": cfg3b, "cfg4": cfg4}
+CONFIGS = {"cfg1": cfg1, "cfg2b": cfg2b, "cfg2h": cfg2h, "cfg3a": cfg3a, "cfg3b": cfg3b, "cfg4": cfg4}
 
 
 def main():
