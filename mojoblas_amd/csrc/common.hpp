@@ -13,6 +13,19 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef short i16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));  // 16-byte chunk (builtin vector: usable in LDS)
+
+// Pointer qualifier for LDS.  Tile kernels select between double buffers at
+// run time; through plain (generic) pointers those accesses compile to flat
+// instructions, which share the vector-memory counter with the global
+// prefetch loads, so every fragment read would also wait for the next tile's
+// HBM loads.  LDS-qualified pointers keep them ds_read / ds_write.
+#define MBH_LDS __attribute__((address_space(3)))
+// Pointer qualifier for global memory, for operand pointers the compiler
+// cannot trace to a kernel argument (the batched kernels load them from a
+// plan in HBM): keeps their accesses global_load / global_store instead of
+// flat ones.
+#define MBH_GLOBAL __attribute__((address_space(1)))
 
 enum Prec { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP8 = 2, PREC_BF16 = 3 };
 

@@ -89,7 +89,7 @@ struct PanelLoader {
             }
             if (full) {
                 const TIN* p = KCONTIG ? src + (k0 + kk) + (row0 + row) * ld : src + (row0 + row) + (k0 + kk) * ld;
-                const vin_t x = *reinterpret_cast<const vin_t*>(p);
+                const vin_t x = *(const MBH_GLOBAL vin_t*)p;
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) r[v][e] = (T)x[e];
             } else {
@@ -98,7 +98,8 @@ struct PanelLoader {
                     const int64_t gr = row0 + row + (KCONTIG ? 0 : e);
                     const int64_t gk = k0 + kk + (KCONTIG ? e : 0);
                     T x = T(0);
-                    if (gr < nrows && gk < ktot) x = (T)(KCONTIG ? src[gk + gr * ld] : src[gr + gk * ld]);
+                    const MBH_GLOBAL TIN* g = (const MBH_GLOBAL TIN*)src;
+                    if (gr < nrows && gk < ktot) x = (T)(KCONTIG ? g[gk + gr * ld] : g[gr + gk * ld]);
                     r[v][e] = x;
                 }
             }
@@ -107,7 +108,7 @@ struct PanelLoader {
 
     // Register -> LDS ([BK][S] layout).
     template <int S>
-    __device__ inline void store(T* lds, int tid) const {
+    __device__ inline void store(MBH_LDS T* lds, int tid) const {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const int idx = tid + NT * v;
@@ -124,7 +125,7 @@ struct PanelLoader {
                     vt x;
 #pragma unroll
                     for (int e = 0; e < VEC; ++e) x[e] = r[v][e];
-                    *reinterpret_cast<vt*>(lds + kk * S + row) = x;
+                    *reinterpret_cast<MBH_LDS vt*>(lds + kk * S + row) = x;
                 } else {
 #pragma unroll
                     for (int e = 0; e < VEC; ++e) lds[kk * S + row + e] = r[v][e];
@@ -164,8 +165,9 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
     constexpr bool AK = AT;    // A 'T': op(A)(i,k) = A[k + i*lda] is k-contiguous
     constexpr bool BK_ = !BT;  // B 'N': op(B)(k,j) = B[k + j*ldb] is k-contiguous
     constexpr int SA = L::SA, SB = L::SB;
-    T* As[2] = {lds, lds + L::A};
-    T* Bs[2] = {lds + 2 * L::A, lds + 2 * L::A + L::B};
+    MBH_LDS T* const l3 = (MBH_LDS T*)lds;
+    MBH_LDS T* As[2] = {l3, l3 + L::A};
+    MBH_LDS T* Bs[2] = {l3 + 2 * L::A, l3 + 2 * L::A + L::B};
 
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -199,8 +201,8 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
             la.load(p.A, p.lda, m0, p.m, (int64_t)(kt + 1) * BK, p.k, p.a_vec, tid);
             lb.load(p.B, p.ldb, n0, p.n, (int64_t)(kt + 1) * BK, p.k, p.b_vec, tid);
         }
-        const T* Ab = As[cur] + lgrp * SA + wm0 + lrow;
-        const T* Bb = Bs[cur] + lgrp * SB + wn0 + lrow;
+        const MBH_LDS T* Ab = As[cur] + lgrp * SA + wm0 + lrow;
+        const MBH_LDS T* Bb = Bs[cur] + lgrp * SB + wn0 + lrow;
 #pragma unroll
         for (int kk = 0; kk < BK; kk += G) {
             T a[TM], b[TN];
@@ -233,7 +235,7 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
             for (int r = 0; r < M::NACC; ++r) {
                 const int64_t col = n0 + wn0 + j * M::MT + acc_col<M::MT>(r, lane);
                 if (col < p.n && tri_keep(p.tri, row, col)) {
-                    TC* cp = p.C + row + col * p.ldc;
+                    MBH_GLOBAL TC* cp = (MBH_GLOBAL TC*)p.C + row + col * p.ldc;
                     T v = alpha * acc[i][j][r];
                     if (beta != T(0)) v += beta * (T)*cp;
                     *cp = (TC)v;

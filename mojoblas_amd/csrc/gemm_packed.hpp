@@ -22,9 +22,10 @@
 // operand: the accumulator holds C^T, so each register's 32 lanes are 32
 // consecutive rows of one C column (coalesced column-major stores).
 //
-// Pipeline: the next k-tile's global loads are issued into registers before
-// the current tile's MFMAs and written to the other LDS buffer after them;
-// one barrier per k-tile.
+// Pipeline: two LDS buffers, one barrier per k-tile.  The grid kernel loads
+// the next k-tile straight into the other buffer (global_load_lds) before the
+// current tile's MFMAs; the register-staged form (gemm_packed_tile) serves
+// the persistent batched kernel.
 #pragma once
 #include "common.hpp"
 #include "gemm_fp.hpp"
@@ -58,12 +59,13 @@ __device__ inline int lds_swz(int row) { return ((row >> 1) ^ (row >> 2)) & 3; }
 // (register r of lane l is C(row0 + 32i + l%32, col0 + 32j + acc_col(r, l))).
 // Interior blocks without a triangle mask take a branch-free path.
 template <int TM, int TN>
-__device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* C, int64_t ldc, int64_t m, int64_t n,
+__device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* Cgen, int64_t ldc, int64_t m, int64_t n,
                                       int64_t row0, int64_t col0, int lane, float alpha, float beta, int tri) {
+    MBH_GLOBAL float* const C = (MBH_GLOBAL float*)Cgen;
     const int lrow = lane & 31;
     const bool interior = tri == TRI_NONE && row0 + 32 * TM <= m && col0 + 32 * TN <= n;
     if (interior) {
-        float* cb = C + (row0 + lrow) + col0 * ldc;
+        MBH_GLOBAL float* cb = C + (row0 + lrow) + col0 * ldc;
         if (beta == 0.0f) {
 #pragma unroll
             for (int i = 0; i < TM; ++i)
@@ -79,7 +81,7 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* C, int
                 for (int j = 0; j < TN; ++j)
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        float* cp = cb + i * 32 + (int64_t)(j * 32 + acc_col<32>(r, lane)) * ldc;
+                        MBH_GLOBAL float* cp = cb + i * 32 + (int64_t)(j * 32 + acc_col<32>(r, lane)) * ldc;
                         *cp = alpha * acc[i][j][r] + beta * *cp;
                     }
         }
@@ -95,7 +97,7 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* C, int
             for (int r = 0; r < 16; ++r) {
                 const int64_t col = col0 + j * 32 + acc_col<32>(r, lane);
                 if (col < n && tri_keep(tri, row, col)) {
-                    float* cp = C + row + col * ldc;
+                    MBH_GLOBAL float* cp = C + row + col * ldc;
                     float v = alpha * acc[i][j][r];
                     if (beta != 0.0f) v += beta * *cp;
                     *cp = v;
@@ -132,14 +134,15 @@ __device__ inline void gemm_packed_tile(const PackedGemmParams& p, int tm, int t
     static_assert(A_LD * NT == A_CH && B_LD * NT == B_CH, "tile not divisible by the thread block");
     typedef PackedTileLds<NPROD, BM, BN> L;
     // LDS views: As[buf][plane] / Bs[buf][plane], each rows * 4 chunks.
-    uint4* As[2][NPL];
-    uint4* Bs[2][NPL];
+    MBH_LDS u32x4* const l3 = (MBH_LDS u32x4*)lds;
+    MBH_LDS u32x4* As[2][NPL];
+    MBH_LDS u32x4* Bs[2][NPL];
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int pl = 0; pl < NPL; ++pl) {
-            As[b][pl] = lds + (b * NPL + pl) * L::A;
-            Bs[b][pl] = lds + 2 * NPL * L::A + (b * NPL + pl) * L::B;
+            As[b][pl] = l3 + (b * NPL + pl) * L::A;
+            Bs[b][pl] = l3 + 2 * NPL * L::A + (b * NPL + pl) * L::B;
         }
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -148,13 +151,14 @@ __device__ inline void gemm_packed_tile(const PackedGemmParams& p, int tm, int t
     // Panel offsets in 16-byte chunks: a k-tile block is BM (BN) rows of 64
     // bytes in every mode.
     const size_t a_off = (size_t)tm * p.ktiles * A_CH + tid, b_off = (size_t)tn * p.ktiles * B_CH + tid;
-    const uint4* __restrict__ gAh = reinterpret_cast<const uint4*>(p.Ah) + a_off;
-    const uint4* __restrict__ gBh = reinterpret_cast<const uint4*>(p.Bh) + b_off;
-    const uint4* __restrict__ gAl = gAh;
-    const uint4* __restrict__ gBl = gBh;
+    typedef const MBH_GLOBAL u32x4* gptr;
+    gptr __restrict__ gAh = ((gptr)p.Ah) + a_off;
+    gptr __restrict__ gBh = ((gptr)p.Bh) + b_off;
+    gptr __restrict__ gAl = gAh;
+    gptr __restrict__ gBl = gBh;
     if (NPL == 2) {
-        gAl = reinterpret_cast<const uint4*>(p.Al) + a_off;
-        gBl = reinterpret_cast<const uint4*>(p.Bl) + b_off;
+        gAl = ((gptr)p.Al) + a_off;
+        gBl = ((gptr)p.Bl) + b_off;
     }
     // This thread stages chunk tid (and tid + NT when a plane tile has 2*NT
     // chunks) of every plane; s0/s1 are their swizzled LDS offsets (in uint4),
@@ -165,7 +169,7 @@ __device__ inline void gemm_packed_tile(const PackedGemmParams& p, int tm, int t
     const int s0 = (c0 >> 2) * 4 + ((c0 & 3) ^ lds_swz(c0 >> 2));
     const int s1 = (c1 >> 2) * 4 + ((c1 & 3) ^ lds_swz(c1 >> 2));
 
-    uint4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
+    u32x4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
 #define MBH_GLOAD(kt)                                                    \
     do {                                                                 \
         const size_t oa = (size_t)(kt) * A_CH, ob = (size_t)(kt) * B_CH; \
@@ -219,19 +223,19 @@ __device__ inline void gemm_packed_tile(const PackedGemmParams& p, int tm, int t
         if constexpr (FP8) {
             // Lane's 32 k values: chunks 2*lgrp and 2*lgrp+1 of its row.
             const int c0 = (2 * lgrp) ^ lsw, c1 = (2 * lgrp + 1) ^ lsw;
-            const int sa = p.sA[((tm * BM + wm0) >> 7) * p.kblocks + (kt >> 1)];
-            const int sb = p.sB[((tn * BN + wn0) >> 7) * p.kblocks + (kt >> 1)];
+            const int sa = ((const MBH_GLOBAL int*)p.sA)[((tm * BM + wm0) >> 7) * p.kblocks + (kt >> 1)];
+            const int sb = ((const MBH_GLOBAL int*)p.sB)[((tn * BN + wn0) >> 7) * p.kblocks + (kt >> 1)];
             i32x8 af[TM], bfr[TN];
 #pragma unroll
             for (int i = 0; i < TM; ++i) {
                 const int base = (wm0 + i * 32 + lrow) * 4;
-                const uint4 x0 = As[cur][0][base + c0], x1 = As[cur][0][base + c1];
+                const u32x4 x0 = As[cur][0][base + c0], x1 = As[cur][0][base + c1];
                 af[i] = i32x8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
             }
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
                 const int base = (wn0 + j * 32 + lrow) * 4;
-                const uint4 x0 = Bs[cur][0][base + c0], x1 = Bs[cur][0][base + c1];
+                const u32x4 x0 = Bs[cur][0][base + c0], x1 = Bs[cur][0][base + c1];
                 bfr[j] = i32x8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
             }
 #pragma unroll
@@ -248,14 +252,14 @@ __device__ inline void gemm_packed_tile(const PackedGemmParams& p, int tm, int t
 #pragma unroll
             for (int i = 0; i < TM; ++i) {
                 const int idx = (wm0 + i * 32 + lrow) * 4 + ch;
-                ah[i] = __builtin_bit_cast(bf16x8, As[cur][0][idx]);
-                if (NPROD == 3) al[i] = __builtin_bit_cast(bf16x8, As[cur][NPL - 1][idx]);
+                ah[i] = __builtin_bit_cast(bf16x8, (u32x4)As[cur][0][idx]);
+                if (NPROD == 3) al[i] = __builtin_bit_cast(bf16x8, (u32x4)As[cur][NPL - 1][idx]);
             }
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
                 const int idx = (wn0 + j * 32 + lrow) * 4 + ch;
-                bh[j] = __builtin_bit_cast(bf16x8, Bs[cur][0][idx]);
-                if (NPROD == 3) bl[j] = __builtin_bit_cast(bf16x8, Bs[cur][NPL - 1][idx]);
+                bh[j] = __builtin_bit_cast(bf16x8, (u32x4)Bs[cur][0][idx]);
+                if (NPROD == 3) bl[j] = __builtin_bit_cast(bf16x8, (u32x4)Bs[cur][NPL - 1][idx]);
             }
 #pragma unroll
             for (int i = 0; i < TM; ++i)
@@ -279,13 +283,184 @@ __device__ inline void gemm_packed_tile(const PackedGemmParams& p, int tm, int t
     epilogue_32x32<TM, TN>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
 }
 
+// MFMAs of one k-tile held in LDS: a[pl] / b[pl] are the plane bases of the
+// buffer (rows * 4 swizzled chunks each), the wave's block starts at
+// (wm0, wn0).  sa / sb: E8M0 scales of the k-tile (FP8 only).
+template <int NPROD, int TM, int TN>
+__device__ inline void packed_ktile_mfma(const MBH_LDS u32x4* a0, const MBH_LDS u32x4* a1, const MBH_LDS u32x4* b0,
+                                         const MBH_LDS u32x4* b1, int wm0, int wn0, int lane, int sa, int sb,
+                                         f32x16 (&acc)[TM][TN]) {
+    const int lrow = lane & 31, lgrp = lane >> 5, lsw = lds_swz(lrow);
+    if constexpr (NPROD == 8) {
+        // Lane's 32 k values: chunks 2*lgrp and 2*lgrp+1 of its row.
+        const int c0 = (2 * lgrp) ^ lsw, c1 = (2 * lgrp + 1) ^ lsw;
+        i32x8 af[TM], bfr[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            const int base = (wm0 + i * 32 + lrow) * 4;
+            const u32x4 x0 = a0[base + c0], x1 = a0[base + c1];
+            af[i] = i32x8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int base = (wn0 + j * 32 + lrow) * 4;
+            const u32x4 x0 = b0[base + c0], x1 = b0[base + c1];
+            bfr[j] = i32x8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(bfr[j], af[i], acc[i][j], 0, 0, 0, sb, 0, sa);
+    } else {
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int ch = (2 * ks + lgrp) ^ lsw;
+            bf16x8 ah[TM], bh[TN], al[TM], bl[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                const int idx = (wm0 + i * 32 + lrow) * 4 + ch;
+                ah[i] = __builtin_bit_cast(bf16x8, (u32x4)a0[idx]);
+                if (NPROD == 3) al[i] = __builtin_bit_cast(bf16x8, (u32x4)a1[idx]);
+            }
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const int idx = (wn0 + j * 32 + lrow) * 4 + ch;
+                bh[j] = __builtin_bit_cast(bf16x8, (u32x4)b0[idx]);
+                if (NPROD == 3) bl[j] = __builtin_bit_cast(bf16x8, (u32x4)b1[idx]);
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    if (NPROD == 3) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+                    }
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+                }
+        }
+    }
+}
+
+// The same tile with the global -> LDS copy done by the load unit itself
+// (global_load_lds_dwordx4: lane i of a wave writes 16 bytes at LDS base +
+// 16*i), so the staged k-tile never passes through registers and no
+// ds_write is issued.  Each lane loads the global chunk that belongs in its
+// swizzled LDS slot: a wave-load covers 16 rows x 4 chunks, and slot
+// (row, c) holds global chunk (row, c ^ swz(row)).  The two LDS buffers are
+// separate arrays (lds0 / lds1) and the k-loop is unrolled by two, so every
+// fragment read names a buffer that no load in flight writes; the wait for
+// the loads (vmcnt) comes once per k-tile, before the barrier.
+template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
+__device__ inline void gemm_packed_tile_dma(const PackedGemmParams& p, int tm, int tn, u32x4* lds0, u32x4* lds1) {
+    constexpr bool FP8 = (NPROD == 8);
+    constexpr int NW = WAVES_M * WAVES_N;
+    constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+    constexpr int TM = WM / 32, TN = WN / 32;
+    constexpr int NPL = (NPROD == 3) ? 2 : 1;
+    constexpr int A_CH = BM * 4, B_CH = BN * 4;           // chunks per plane per k-tile
+    constexpr int A_WL = A_CH / (64 * NW), B_WL = B_CH / (64 * NW);  // wave-loads per plane
+    static_assert(A_WL * 64 * NW == A_CH && B_WL * 64 * NW == B_CH && A_WL >= 1 && B_WL >= 1,
+                  "tile not divisible into wave-loads");
+    typedef const MBH_GLOBAL u32x4* gptr;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+    // Chunk this lane fetches inside its wave-load's 16 x 4 block (row bits
+    // above 3 do not change the swizzle).
+    const int lrow16 = lane >> 2;
+    const int lchunk = lrow16 * 4 + ((lane & 3) ^ lds_swz(lrow16));
+    const size_t a_off = (size_t)tm * p.ktiles * A_CH + lchunk, b_off = (size_t)tn * p.ktiles * B_CH + lchunk;
+    gptr gA0 = ((gptr)p.Ah) + a_off;
+    gptr gB0 = ((gptr)p.Bh) + b_off;
+    gptr gA1 = gA0;
+    gptr gB1 = gB0;
+    if (NPL == 2) {
+        gA1 = ((gptr)p.Al) + a_off;
+        gB1 = ((gptr)p.Bl) + b_off;
+    }
+    MBH_LDS u32x4* const L0 = (MBH_LDS u32x4*)lds0;
+    MBH_LDS u32x4* const L1 = (MBH_LDS u32x4*)lds1;
+    // Buffer layout: A plane 0, [A plane 1], B plane 0, [B plane 1].
+    constexpr int OA1 = A_CH, OB0 = NPL * A_CH, OB1 = NPL * A_CH + B_CH;
+
+#define MBH_DMA(kt, LB)                                                                                 \
+    do {                                                                                                \
+        const size_t ga = (size_t)(kt) * A_CH, gb = (size_t)(kt) * B_CH;                                \
+        _Pragma("unroll") for (int v = 0; v < A_WL; ++v) {                                              \
+            const int blk = (wave + v * NW) * 64;                                                       \
+            __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gA0 + ga + blk),                  \
+                                             (MBH_LDS void*)((LB) + blk), 16, 0, 0);                    \
+            if (NPL == 2)                                                                               \
+                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gA1 + ga + blk),              \
+                                                 (MBH_LDS void*)((LB) + OA1 + blk), 16, 0, 0);          \
+        }                                                                                               \
+        _Pragma("unroll") for (int v = 0; v < B_WL; ++v) {                                              \
+            const int blk = (wave + v * NW) * 64;                                                       \
+            __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gB0 + gb + blk),                  \
+                                             (MBH_LDS void*)((LB) + OB0 + blk), 16, 0, 0);              \
+            if (NPL == 2)                                                                               \
+                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gB1 + gb + blk),              \
+                                                 (MBH_LDS void*)((LB) + OB1 + blk), 16, 0, 0);          \
+        }                                                                                               \
+        __builtin_amdgcn_sched_barrier(0); /* keep the loads ahead of the k-tile's MFMAs */             \
+    } while (0)
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    constexpr int BK = FP8 ? 64 : 32;
+    int kt0, kt1;
+    kt_range(p.krange, tm, tn, BM, BN, BK, p.ktiles, kt0, kt1);
+#define MBH_SCALES(kt)                                                                                          \
+    if (FP8) {                                                                                                  \
+        sa = ((const MBH_GLOBAL int*)p.sA)[((tm * BM + wm0) >> 7) * p.kblocks + ((kt) >> 1)];                    \
+        sb = ((const MBH_GLOBAL int*)p.sB)[((tn * BN + wn0) >> 7) * p.kblocks + ((kt) >> 1)];                    \
+    }
+    int sa = 0, sb = 0;
+    if (kt0 < kt1) MBH_DMA(kt0, L0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int kt = kt0;
+    for (; kt + 1 < kt1; kt += 2) {
+        MBH_DMA(kt + 1, L1);
+        MBH_SCALES(kt);
+        packed_ktile_mfma<NPROD, TM, TN>(L0, L0 + OA1, L0 + OB0, L0 + OB1, wm0, wn0, lane, sa, sb, acc);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (kt + 2 < kt1) MBH_DMA(kt + 2, L0);
+        MBH_SCALES(kt + 1);
+        packed_ktile_mfma<NPROD, TM, TN>(L1, L1 + OA1, L1 + OB0, L1 + OB1, wm0, wn0, lane, sa, sb, acc);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+    if (kt < kt1) {  // odd count: the last k-tile is in buffer 0
+        MBH_SCALES(kt);
+        packed_ktile_mfma<NPROD, TM, TN>(L0, L0 + OA1, L0 + OB0, L0 + OB1, wm0, wn0, lane, sa, sb, acc);
+    }
+#undef MBH_SCALES
+#undef MBH_DMA
+
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    epilogue_32x32<TM, TN>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
+}
+
 template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(PackedGemmParams p) {
-    __shared__ uint4 lds[PackedTileLds<NPROD, BM, BN>::TOTAL];
+    typedef PackedTileLds<NPROD, BM, BN> L;
+    __shared__ u32x4 lds0[L::TOTAL / 2];
+    __shared__ u32x4 lds1[L::TOTAL / 2];
     int tm, tn;
     if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
     else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
-    gemm_packed_tile<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds);
+    gemm_packed_tile_dma<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
 }
 
 }  // namespace mbh

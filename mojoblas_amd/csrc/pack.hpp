@@ -46,6 +46,7 @@ __device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t
     constexpr int BR = 64, BKK = 32;  // fp32 block moved per workgroup
     static_assert(PR % BR == 0 && PK % BKK == 0, "pack block must tile the gemm block");
     const int tid = threadIdx.x;
+    const MBH_GLOBAL float* const X = (const MBH_GLOBAL float*)p.X;
     if (KCONTIG) {
         // element (r, k) at X[k + r*ld]: 8 lanes cover one row's 32 k.
 #pragma unroll
@@ -53,14 +54,14 @@ __device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t
             const int r = pass * 32 + tid / 8, kq = (tid % 8) * 4;
             const int64_t gr = r0 + r;
             if (p.vec && gr < p.R && k0 + kq + 4 <= p.K) {
-                const f32x4 v = *reinterpret_cast<const f32x4*>(p.X + (k0 + kq) + gr * p.ld);
+                const f32x4 v = *(const MBH_GLOBAL f32x4*)(X + (k0 + kq) + gr * p.ld);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) s[r][kq + e] = v[e];
             } else {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int64_t gk = k0 + kq + e;
-                    s[r][kq + e] = (gr < p.R && gk < p.K) ? p.X[gk + gr * p.ld] : 0.0f;
+                    s[r][kq + e] = (gr < p.R && gk < p.K) ? X[gk + gr * p.ld] : 0.0f;
                 }
             }
         }
@@ -70,7 +71,7 @@ __device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t
         for (int pass = 0; pass < 8; ++pass) {
             const int kk = pass * 4 + tid / 64, r = tid % 64;
             const int64_t gr = r0 + r, gk = k0 + kk;
-            s[r][kk] = (gr < p.R && gk < p.K) ? p.X[gr + gk * p.ld] : 0.0f;
+            s[r][kk] = (gr < p.R && gk < p.K) ? X[gr + gk * p.ld] : 0.0f;
         }
     }
     __syncthreads();
@@ -89,13 +90,13 @@ __device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t
     hv.y = h[2] | (h[3] << 16);
     hv.z = h[4] | (h[5] << 16);
     hv.w = h[6] | (h[7] << 16);
-    *reinterpret_cast<uint4*>(p.hi + off) = hv;
+    *(MBH_GLOBAL u32x4*)(p.hi + off) = u32x4{hv.x, hv.y, hv.z, hv.w};
     if (p.lo) {
         lv.x = l[0] | (l[1] << 16);
         lv.y = l[2] | (l[3] << 16);
         lv.z = l[4] | (l[5] << 16);
         lv.w = l[6] | (l[7] << 16);
-        *reinterpret_cast<uint4*>(p.lo + off) = lv;
+        *(MBH_GLOBAL u32x4*)(p.lo + off) = u32x4{lv.x, lv.y, lv.z, lv.w};
     }
 }
 
