@@ -55,10 +55,14 @@ __global__ void vb_reset_kernel(int* ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = 0;
 }
 
-constexpr int FP_LDS =
-    FpTileLds<float, float, float, false, false, 128, 128, 16>::TOTAL > FpTileLds<float, float, float, false, false, 64, 64, 16>::TOTAL
-        ? FpTileLds<float, float, float, false, false, 128, 128, 16>::TOTAL
-        : FpTileLds<float, float, float, false, false, 64, 64, 16>::TOTAL;
+// fp32 batched tiles: the 64x64 class runs 32-deep k-tiles with prefetch
+// distance two (its short k-loops are latency-bound); the 128x128 class keeps
+// 16-deep k-tiles at distance one, whose register footprint leaves room for
+// two resident workgroups per CU.
+constexpr int VB_BK64 = 32, VB_BK128 = 16;
+constexpr int FP_LDS64 = FpTileLds<float, float, float, false, false, 64, 64, VB_BK64>::TOTAL;
+constexpr int FP_LDS128 = FpTileLds<float, float, float, false, false, 128, 128, VB_BK128>::TOTAL;
+constexpr int FP_LDS = FP_LDS128 > FP_LDS64 ? FP_LDS128 : FP_LDS64;
 constexpr int BF_LDS = PackedTileLds<1, 128, 128>::TOTAL;
 
 // Work-queue walk shared by the persistent kernels: thread 0 claims item
@@ -84,11 +88,11 @@ __device__ inline void vb_walk(int* ctr, int nitems, int* slot, F&& body) {
 // MAXCLS: highest tile class compiled in.  The 64x64-only variant needs far
 // fewer registers and LDS, so more workgroups stay resident per CU.
 template <int MAXCLS>
-__global__ __launch_bounds__(256) void vb_gemm_f32_kernel(const VbProblem* __restrict__ probs,
+__global__ __launch_bounds__(256, 2) void vb_gemm_f32_kernel(const VbProblem* __restrict__ probs,
                                                           const VbItem* __restrict__ items, int nitems, int* ctr,
                                                           float alpha, float beta, const float* const* A,
                                                           const float* const* B, float* const* C) {
-    constexpr int LDS = MAXCLS == 0 ? FpTileLds<float, float, float, false, false, 64, 64, 16>::TOTAL : FP_LDS;
+    constexpr int LDS = MAXCLS == 0 ? FP_LDS64 : FP_LDS;
     __shared__ __attribute__((aligned(16))) float lds[LDS];
     __shared__ int slot;
     vb_walk(ctr, nitems, &slot, [&](int it) {
@@ -110,9 +114,9 @@ __global__ __launch_bounds__(256) void vb_gemm_f32_kernel(const VbProblem* __res
         p.b_vec = ((uintptr_t)p.B % 16 == 0) && (pr.ldb % 4 == 0);
         const int tm = w.tile / pr.tiles_n, tn = w.tile % pr.tiles_n;
         if (MAXCLS == 0 || pr.cls == 0)
-            gemm_fp_tile<float, float, float, float, false, false, 64, 64, 16>(p, tm, tn, lds);
+            gemm_fp_tile_d2<float, float, float, float, false, false, 64, 64, VB_BK64>(p, tm, tn, lds);
         else
-            gemm_fp_tile<float, float, float, float, false, false, 128, 128, 16>(p, tm, tn, lds);
+            gemm_fp_tile<float, float, float, float, false, false, 128, 128, VB_BK128>(p, tm, tn, lds);
     });
 }
 
@@ -151,13 +155,15 @@ __global__ __launch_bounds__(256) void vb_pack_kernel(const VbProblem* __restric
     }
 }
 
+// Two waves per SIMD: the register cap keeps two workgroups resident per CU.
 template <int MAXCLS>
-__global__ __launch_bounds__(256) void vb_gemm_bf16_kernel(const VbProblem* __restrict__ probs,
+__global__ __launch_bounds__(256, 2) void vb_gemm_bf16_kernel(const VbProblem* __restrict__ probs,
                                                            const VbItem* __restrict__ items, int nitems, int* ctr,
                                                            float alpha, float beta, const uint16_t* pack,
                                                            float* const* C) {
     constexpr int LDS = MAXCLS == 0 ? PackedTileLds<1, 64, 64>::TOTAL : BF_LDS;
-    __shared__ uint4 lds[LDS];
+    __shared__ u32x4 lds0[LDS / 2];
+    __shared__ u32x4 lds1[LDS / 2];
     __shared__ int slot;
     vb_walk(ctr, nitems, &slot, [&](int it) {
         const VbItem w = items[it];
@@ -173,8 +179,8 @@ __global__ __launch_bounds__(256) void vb_gemm_bf16_kernel(const VbProblem* __re
         p.alpha = alpha;
         p.beta = beta;
         const int tm = w.tile / pr.tiles_n, tn = w.tile % pr.tiles_n;
-        if (MAXCLS == 0 || pr.cls == 0) gemm_packed_tile<1, 64, 64, 2, 2>(p, tm, tn, lds);
-        else gemm_packed_tile<1, 128, 128, 2, 2>(p, tm, tn, lds);
+        if (MAXCLS == 0 || pr.cls == 0) gemm_packed_tile_dma<1, 64, 64, 2, 2>(p, tm, tn, lds0, lds1);
+        else gemm_packed_tile_dma<1, 128, 128, 2, 2>(p, tm, tn, lds0, lds1);
     });
 }
 

@@ -246,6 +246,114 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
 }
 
 
+// The same tile with register prefetch distance two: the k-loop is unrolled
+// by two with two named staging sets, so the global loads of k-tile kt+2 are
+// issued while tile kt is computed and have two k-tiles of compute to land
+// (tile kt+1 moves from registers to LDS after tile kt's MFMAs).  Used where
+// k-loops are short and latency dominates (the persistent batched kernel).
+template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
+__device__ inline void gemm_fp_tile_d2(const GemmArgs<TA, TB, TC, T>& p, int tm, int tn, T* lds) {
+    typedef MfmaFp<T> M;
+    typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
+    constexpr int NT = 256;
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int TM = WM / M::MT, TN = WN / M::MT;
+    constexpr bool AK = AT, BK_ = !BT;
+    constexpr int SA = L::SA, SB = L::SB;
+    MBH_LDS T* const l3 = (MBH_LDS T*)lds;
+    MBH_LDS T* const A0 = l3;
+    MBH_LDS T* const A1 = l3 + L::A;
+    MBH_LDS T* const B0 = l3 + 2 * L::A;
+    MBH_LDS T* const B1 = l3 + 2 * L::A + L::B;
+
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+    constexpr int G = 64 / M::MT;
+    const int lrow = lane % M::MT, lgrp = lane / M::MT;
+
+    PanelLoader<T, TA, AK, BM, BK, NT> la0, la1;
+    PanelLoader<T, TB, BK_, BN, BK, NT> lb0, lb1;
+    typename M::acc_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < M::NACC; ++r) acc[i][j][r] = T(0);
+
+    auto mfma_tile = [&](const MBH_LDS T* As, const MBH_LDS T* Bs) {
+        const MBH_LDS T* Ab = As + lgrp * SA + wm0 + lrow;
+        const MBH_LDS T* Bb = Bs + lgrp * SB + wn0 + lrow;
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += G) {
+            T a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = Ab[kk * SA + i * M::MT];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b[j] = Bb[kk * SB + j * M::MT];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) acc[i][j] = M::mma(b[j], a[i], acc[i][j]);
+        }
+    };
+#define MBH_LD(LA, LB_, kt)                                                            \
+    do {                                                                              \
+        LA.load(p.A, p.lda, m0, p.m, (int64_t)(kt) * BK, p.k, p.a_vec, tid);         \
+        LB_.load(p.B, p.ldb, n0, p.n, (int64_t)(kt) * BK, p.k, p.b_vec, tid);        \
+    } while (0)
+
+    int kt0, kt1;
+    kt_range(p.krange, tm, tn, BM, BN, BK, (int)((p.k + BK - 1) / BK), kt0, kt1);
+    if (kt0 < kt1) {
+        MBH_LD(la0, lb0, kt0);
+        if (kt0 + 1 < kt1) MBH_LD(la1, lb1, kt0 + 1);
+        la0.template store<SA>(A0, tid);
+        lb0.template store<SB>(B0, tid);
+    }
+    __syncthreads();
+    int kt = kt0;
+    for (; kt + 1 < kt1; kt += 2) {
+        // LDS 0: tile kt; registers 1: tile kt+1 (in flight)
+        if (kt + 2 < kt1) MBH_LD(la0, lb0, kt + 2);
+        mfma_tile(A0, B0);
+        la1.template store<SA>(A1, tid);
+        lb1.template store<SB>(B1, tid);
+        __syncthreads();
+        // LDS 1: tile kt+1; registers 0: tile kt+2 (in flight)
+        if (kt + 3 < kt1) MBH_LD(la1, lb1, kt + 3);
+        mfma_tile(A1, B1);
+        if (kt + 2 < kt1) {
+            la0.template store<SA>(A0, tid);
+            lb0.template store<SB>(B0, tid);
+        }
+        __syncthreads();
+    }
+    if (kt < kt1) mfma_tile(A0, B0);
+#undef MBH_LD
+
+    const T alpha = p.alpha, beta = p.beta;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int64_t row = m0 + wm0 + i * M::MT + lrow;
+        if (row >= p.m) continue;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+#pragma unroll
+            for (int r = 0; r < M::NACC; ++r) {
+                const int64_t col = n0 + wn0 + j * M::MT + acc_col<M::MT>(r, lane);
+                if (col < p.n && tri_keep(p.tri, row, col)) {
+                    MBH_GLOBAL TC* cp = (MBH_GLOBAL TC*)p.C + row + col * p.ldc;
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * (T)*cp;
+                    *cp = (TC)v;
+                }
+            }
+        }
+    }
+}
+
 template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p) {
     typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
