@@ -60,8 +60,8 @@ __global__ void vb_reset_kernel(int* ctr) {
 // 16-deep k-tiles at distance one, whose register footprint leaves room for
 // two resident workgroups per CU.
 constexpr int VB_BK64 = 32, VB_BK128 = 16;
-constexpr int FP_LDS64 = FpTileLds<float, float, float, false, false, 64, 64, VB_BK64>::TOTAL;
-constexpr int FP_LDS128 = FpTileLds<float, float, float, false, false, 128, 128, VB_BK128>::TOTAL;
+constexpr int FP_LDS64 = FpTileLds<float, float, float, false, false, 64, 64, VB_BK64>::BYTES;
+constexpr int FP_LDS128 = FpTileLds<float, float, float, false, false, 128, 128, VB_BK128>::BYTES;
 constexpr int FP_LDS = FP_LDS128 > FP_LDS64 ? FP_LDS128 : FP_LDS64;
 constexpr int BF_LDS = PackedTileLds<1, 128, 128>::TOTAL;
 
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256, 2) void vb_gemm_f32_kernel(const VbProblem* __
                                                           float alpha, float beta, const float* const* A,
                                                           const float* const* B, float* const* C) {
     constexpr int LDS = MAXCLS == 0 ? FP_LDS64 : FP_LDS;
-    __shared__ __attribute__((aligned(16))) float lds[LDS];
+    __shared__ __attribute__((aligned(16))) unsigned char lds[LDS];
     __shared__ int slot;
     vb_walk(ctr, nitems, &slot, [&](int it) {
         const VbItem w = items[it];

@@ -148,15 +148,25 @@ hipError_t run_fp8(bool at, bool bt, const GemmParams<float>& g, int tri_grid, h
     return hipGetLastError();
 }
 
+// Workgroups a launch with BM x BN tiles would have (block triangle when
+// tri_grid is set).
+int64_t tile_count(int64_t m, int64_t n, int BM, int BN, int tri_grid) {
+    const int64_t tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+    return tri_grid ? tm * (tm + 1) / 2 : tm * tn;
+}
+
+// Tile size by grid fill: 256x256 (8 waves) while that gives a tile per CU,
+// then 128x128, then 64x64 for the small products of the level-3 recursions.
 template <int NPROD>
 hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
-    const int64_t t256 = ((g.m + 255) / 256) * ((g.n + 255) / 256);
+    const int64_t t256 = tile_count(g.m, g.n, 256, 256, tri_grid);
     if (NPROD == 8) {
         if (t256 >= 256) return run_fp8<256, 2, 4>(at, bt, g, tri_grid, s);
         return run_fp8<128, 2, 2>(at, bt, g, tri_grid, s);
     }
     if (t256 >= 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s);
-    return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s);
+    if (tile_count(g.m, g.n, 128, 128, tri_grid) >= 128) return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s);
+    return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s);
 }
 
 template <typename T>
@@ -214,7 +224,9 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
     GemmParams<float> p = make_params<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, krange);
     switch (prec) {
         case PREC_FP32:
-            return launch_fp<float, float, float, float, 128, 128, 16>(at, bt, p, tri_grid, s);
+            if (tile_count(m, n, 128, 128, tri_grid) >= 256)
+                return launch_fp<float, float, float, float, 128, 128, 16>(at, bt, p, tri_grid, s);
+            return launch_fp<float, float, float, float, 64, 64, 16>(at, bt, p, tri_grid, s);
         case PREC_BF16X3:
             return launch_packed<3>(at, bt, p, tri_grid, s);
         case PREC_BF16:
@@ -256,8 +268,12 @@ hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double
     p.krange = krange;
     p.a_vec = vec_ok(A, lda);
     p.b_vec = vec_ok(B, ldb);
-    if (shape == 1) return launch_fp<double, TA, TB, TC, 128, 64, 16>(at, bt, p, 0, s);
-    if (shape == 2) return launch_fp<double, TA, TB, TC, 64, 128, 16>(at, bt, p, 0, s);
+    // 1 / 2: one 128-row tile row / 128-column tile column (in-place leaf
+    // products of the triangular solves), 32-deep k-tiles.  0: 64 x 64, which
+    // measured faster than 128 x 64 and 128 x 128 for the 4096^2
+    // refinement residual.
+    if (shape == 1) return launch_fp<double, TA, TB, TC, 128, 32, 32>(at, bt, p, 0, s);
+    if (shape == 2) return launch_fp<double, TA, TB, TC, 32, 128, 32>(at, bt, p, 0, s);
     return launch_fp<double, TA, TB, TC, 64, 64, 16>(at, bt, p, 0, s);
 }
 

@@ -62,8 +62,10 @@ __device__ inline int acc_col(int r, int lane) {
     return 4 * r + (lane >> 4);
 }
 
-// Global -> register staging of one operand panel (ROWS x BK of op(X)),
-// converting the storage type TIN to the MFMA type T on load.
+// Global -> register staging of one operand panel (ROWS x BK of op(X)) in
+// its storage type TIN; LDS holds TIN as well and fragment reads convert to
+// the MFMA type T (an fp32 operand of an fp64 product then costs half the
+// LDS bytes of a converted one).
 // KCONTIG: element (row, kk) lives at src[kk + row*ld]; else src[row + kk*ld].
 template <typename T, typename TIN, bool KCONTIG, int ROWS, int BK, int NT>
 struct PanelLoader {
@@ -71,7 +73,7 @@ struct PanelLoader {
     static constexpr int NV = ROWS * BK / VEC / NT;
     static_assert(NV * VEC * NT == ROWS * BK, "panel not divisible by the thread block");
     typedef TIN vin_t __attribute__((ext_vector_type(VEC)));
-    T r[NV][VEC];
+    TIN r[NV][VEC];
 
     __device__ inline void load(const TIN* __restrict__ src, int64_t ld, int64_t row0, int64_t nrows, int64_t k0,
                                 int64_t ktot, bool vec_ok, int tid) {
@@ -91,15 +93,15 @@ struct PanelLoader {
                 const TIN* p = KCONTIG ? src + (k0 + kk) + (row0 + row) * ld : src + (row0 + row) + (k0 + kk) * ld;
                 const vin_t x = *(const MBH_GLOBAL vin_t*)p;
 #pragma unroll
-                for (int e = 0; e < VEC; ++e) r[v][e] = (T)x[e];
+                for (int e = 0; e < VEC; ++e) r[v][e] = x[e];
             } else {
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) {
                     const int64_t gr = row0 + row + (KCONTIG ? 0 : e);
                     const int64_t gk = k0 + kk + (KCONTIG ? e : 0);
-                    T x = T(0);
+                    TIN x = TIN(0);
                     const MBH_GLOBAL TIN* g = (const MBH_GLOBAL TIN*)src;
-                    if (gr < nrows && gk < ktot) x = (T)(KCONTIG ? g[gk + gr * ld] : g[gr + gk * ld]);
+                    if (gr < nrows && gk < ktot) x = KCONTIG ? g[gk + gr * ld] : g[gr + gk * ld];
                     r[v][e] = x;
                 }
             }
@@ -108,7 +110,7 @@ struct PanelLoader {
 
     // Register -> LDS ([BK][S] layout).
     template <int S>
-    __device__ inline void store(MBH_LDS T* lds, int tid) const {
+    __device__ inline void store(MBH_LDS TIN* lds, int tid) const {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const int idx = tid + NT * v;
@@ -120,43 +122,40 @@ struct PanelLoader {
             } else {
                 const int kk = idx / (ROWS / VEC);
                 const int row = (idx % (ROWS / VEC)) * VEC;
-                if (sizeof(T) * VEC == 16) {
-                    typedef T vt __attribute__((ext_vector_type(VEC)));
-                    vt x;
+                vin_t x;
 #pragma unroll
-                    for (int e = 0; e < VEC; ++e) x[e] = r[v][e];
-                    *reinterpret_cast<MBH_LDS vt*>(lds + kk * S + row) = x;
-                } else {
-#pragma unroll
-                    for (int e = 0; e < VEC; ++e) lds[kk * S + row + e] = r[v][e];
-                }
+                for (int e = 0; e < VEC; ++e) x[e] = r[v][e];
+                *reinterpret_cast<MBH_LDS vin_t*>(lds + kk * S + row) = x;
             }
         }
     }
 };
 
-// LDS row stride for a panel: rows + the vector width when rows are written as
-// vectors; rows + 2 (fp32) / rows + 1 (fp64) when written as scattered scalars
-// (4 consecutive kk from 8 rows land in distinct banks).
+// LDS row stride (elements of the storage type TIN) for a panel: rows + the
+// vector width when rows are written as vectors; rows + 2 (4-byte) / rows + 1
+// (8-byte elements) when written as scattered scalars (4 consecutive kk from
+// 8 rows land in distinct banks).
 template <typename T, typename TIN, bool KCONTIG, int ROWS>
 constexpr int lds_stride() {
-    return KCONTIG ? ROWS + (sizeof(T) == 4 ? 2 : 1) : ROWS + 16 / (int)sizeof(TIN);
+    return KCONTIG ? ROWS + (sizeof(TIN) == 4 ? 2 : 1) : ROWS + 16 / (int)sizeof(TIN);
 }
 
-// LDS elements (of T) one buffer pair of a tile needs.
+// LDS bytes one tile's double-buffered panels need: [BK][SA] of TA and
+// [BK][SB] of TB per buffer (every region a multiple of 16 bytes).
 template <typename T, typename TA, typename TB, bool AT, bool BT, int BM, int BN, int BK>
 struct FpTileLds {
     static constexpr int SA = lds_stride<T, TA, AT, BM>();
     static constexpr int SB = lds_stride<T, TB, !BT, BN>();
-    static constexpr int A = BK * SA, B = BK * SB;
-    static constexpr int TOTAL = 2 * (A + B);  // double-buffered
+    static constexpr int A_BYTES = BK * SA * (int)sizeof(TA), B_BYTES = BK * SB * (int)sizeof(TB);
+    static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "LDS regions must stay 16-byte aligned");
+    static constexpr int BYTES = 2 * (A_BYTES + B_BYTES);
 };
 
 // One BM x BN tile (tm, tn) of C := alpha*op(A)*op(B) + beta*C on 256
-// threads (2 x 2 waves); `lds` holds FpTileLds::TOTAL elements, 16-byte
-// aligned.  Shared by the grid kernel below and the persistent batched one.
+// threads (2 x 2 waves); `lds` holds FpTileLds::BYTES, 16-byte aligned.
+// Shared by the grid kernel below and the persistent batched one.
 template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
-__device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, int tn, T* lds) {
+__device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, int tn, void* lds) {
     typedef MfmaFp<T> M;
     typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
     constexpr int NT = 256;
@@ -165,9 +164,10 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
     constexpr bool AK = AT;    // A 'T': op(A)(i,k) = A[k + i*lda] is k-contiguous
     constexpr bool BK_ = !BT;  // B 'N': op(B)(k,j) = B[k + j*ldb] is k-contiguous
     constexpr int SA = L::SA, SB = L::SB;
-    MBH_LDS T* const l3 = (MBH_LDS T*)lds;
-    MBH_LDS T* As[2] = {l3, l3 + L::A};
-    MBH_LDS T* Bs[2] = {l3 + 2 * L::A, l3 + 2 * L::A + L::B};
+    MBH_LDS TA* const la3 = (MBH_LDS TA*)lds;
+    MBH_LDS TB* const lb3 = (MBH_LDS TB*)((MBH_LDS char*)lds + 2 * L::A_BYTES);
+    MBH_LDS TA* As[2] = {la3, la3 + BK * SA};
+    MBH_LDS TB* Bs[2] = {lb3, lb3 + BK * SB};
 
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -201,15 +201,15 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
             la.load(p.A, p.lda, m0, p.m, (int64_t)(kt + 1) * BK, p.k, p.a_vec, tid);
             lb.load(p.B, p.ldb, n0, p.n, (int64_t)(kt + 1) * BK, p.k, p.b_vec, tid);
         }
-        const MBH_LDS T* Ab = As[cur] + lgrp * SA + wm0 + lrow;
-        const MBH_LDS T* Bb = Bs[cur] + lgrp * SB + wn0 + lrow;
+        const MBH_LDS TA* Ab = As[cur] + lgrp * SA + wm0 + lrow;
+        const MBH_LDS TB* Bb = Bs[cur] + lgrp * SB + wn0 + lrow;
 #pragma unroll
         for (int kk = 0; kk < BK; kk += G) {
             T a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = Ab[kk * SA + i * M::MT];
+            for (int i = 0; i < TM; ++i) a[i] = (T)Ab[kk * SA + i * M::MT];
 #pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = Bb[kk * SB + j * M::MT];
+            for (int j = 0; j < TN; ++j) b[j] = (T)Bb[kk * SB + j * M::MT];
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -252,7 +252,7 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
 // (tile kt+1 moves from registers to LDS after tile kt's MFMAs).  Used where
 // k-loops are short and latency dominates (the persistent batched kernel).
 template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
-__device__ inline void gemm_fp_tile_d2(const GemmArgs<TA, TB, TC, T>& p, int tm, int tn, T* lds) {
+__device__ inline void gemm_fp_tile_d2(const GemmArgs<TA, TB, TC, T>& p, int tm, int tn, void* lds) {
     typedef MfmaFp<T> M;
     typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
     constexpr int NT = 256;
@@ -260,11 +260,10 @@ __device__ inline void gemm_fp_tile_d2(const GemmArgs<TA, TB, TC, T>& p, int tm,
     constexpr int TM = WM / M::MT, TN = WN / M::MT;
     constexpr bool AK = AT, BK_ = !BT;
     constexpr int SA = L::SA, SB = L::SB;
-    MBH_LDS T* const l3 = (MBH_LDS T*)lds;
-    MBH_LDS T* const A0 = l3;
-    MBH_LDS T* const A1 = l3 + L::A;
-    MBH_LDS T* const B0 = l3 + 2 * L::A;
-    MBH_LDS T* const B1 = l3 + 2 * L::A + L::B;
+    MBH_LDS TA* const A0 = (MBH_LDS TA*)lds;
+    MBH_LDS TA* const A1 = A0 + BK * SA;
+    MBH_LDS TB* const B0 = (MBH_LDS TB*)((MBH_LDS char*)lds + 2 * L::A_BYTES);
+    MBH_LDS TB* const B1 = B0 + BK * SB;
 
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -282,16 +281,16 @@ __device__ inline void gemm_fp_tile_d2(const GemmArgs<TA, TB, TC, T>& p, int tm,
 #pragma unroll
             for (int r = 0; r < M::NACC; ++r) acc[i][j][r] = T(0);
 
-    auto mfma_tile = [&](const MBH_LDS T* As, const MBH_LDS T* Bs) {
-        const MBH_LDS T* Ab = As + lgrp * SA + wm0 + lrow;
-        const MBH_LDS T* Bb = Bs + lgrp * SB + wn0 + lrow;
+    auto mfma_tile = [&](const MBH_LDS TA* As, const MBH_LDS TB* Bs) {
+        const MBH_LDS TA* Ab = As + lgrp * SA + wm0 + lrow;
+        const MBH_LDS TB* Bb = Bs + lgrp * SB + wn0 + lrow;
 #pragma unroll
         for (int kk = 0; kk < BK; kk += G) {
             T a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = Ab[kk * SA + i * M::MT];
+            for (int i = 0; i < TM; ++i) a[i] = (T)Ab[kk * SA + i * M::MT];
 #pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = Bb[kk * SB + j * M::MT];
+            for (int j = 0; j < TN; ++j) b[j] = (T)Bb[kk * SB + j * M::MT];
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -357,7 +356,7 @@ __device__ inline void gemm_fp_tile_d2(const GemmArgs<TA, TB, TC, T>& p, int tm,
 template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p) {
     typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
-    __shared__ __attribute__((aligned(16))) T lds[L::TOTAL];
+    __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
     int tm, tn;
     if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
     else tile_coords(p.tiles_m, p.tiles_n, tm, tn);

@@ -264,13 +264,14 @@ hipError_t trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t
 
     // R = alpha*B0 - op(T) X in fp64 (b0 becomes R); op(T) visits only its
     // nonzero k-tiles.
+    const int resid_shape = 0;
     if constexpr (sizeof(T) == 4) {
         if (left)
-            MBH_TRY((gemm_f64acc<float, float, double>(trans, false, m, n, m, -1.0, tri, na, B, ldb, 1.0, b0, m, s, 0,
-                                                       eff_lower ? KR_LE_ROW : KR_GE_ROW)));
+            MBH_TRY((gemm_f64acc<float, float, double>(trans, false, m, n, m, -1.0, tri, na, B, ldb, 1.0, b0, m, s,
+                                                       resid_shape, eff_lower ? KR_LE_ROW : KR_GE_ROW)));
         else
-            MBH_TRY((gemm_f64acc<float, float, double>(false, trans, m, n, n, -1.0, B, ldb, tri, na, 1.0, b0, m, s, 0,
-                                                       eff_lower ? KR_GE_COL : KR_LE_COL)));
+            MBH_TRY((gemm_f64acc<float, float, double>(false, trans, m, n, n, -1.0, B, ldb, tri, na, 1.0, b0, m, s,
+                                                       resid_shape, eff_lower ? KR_GE_COL : KR_LE_COL)));
     }
     copy_kernel<double, T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, b0, m, d, m, 1.0);
     MBH_TRY(hipGetLastError());
