@@ -13,14 +13,14 @@ static const int VB_DEFAULT_CLASS[2][VB_BUCKETS][VB_BUCKETS] = {{  // FP32
     {0, 0, 0, 0, 0, 0},
     {0, 0, 0, 0, 0, 0},
     {0, 0, 0, 0, 0, 0},
-    {0, 0, 0, 0, 0, 0},
-    {0, 0, 0, 0, 0, 0},
-    {0, 0, 0, 0, 0, 0},
-}, {  // BF16
-    {0, 0, 0, 0, 1, 1},
-    {0, 0, 0, 0, 1, 1},
-    {0, 0, 0, 0, 1, 1},
     {0, 0, 0, 1, 1, 1},
-    {1, 1, 1, 1, 1, 1},
-    {1, 1, 1, 1, 1, 1},
+    {0, 0, 0, 1, 1, 1},
+    {0, 0, 0, 1, 1, 1},
+}, {  // BF16
+    {0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 1, 1, 1},
+    {0, 0, 0, 1, 1, 1},
+    {0, 0, 0, 1, 1, 1},
 }};
