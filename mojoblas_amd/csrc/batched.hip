@@ -6,13 +6,12 @@
 //
 // Plan (host, once per shape list): validate, give every problem a tile class
 // from the tuning table, enumerate its C tiles as work items, order the items
-// longest-first (by BM*BN*k) and copy everything to HBM; in bf16 mode also
-// lay out the packed bf16 panels and the pack work list.
+// longest-first (by BM*BN*k) and copy everything to HBM.
 // Exec (device only, no allocation, no host synchronisation, capturable in a
-// hipGraph): a one-thread kernel resets the work counter, [bf16: one pack
-// kernel converts every problem's A and B^T to k-contiguous bf16 panels],
-// then the persistent kernel's workgroups pull items from the counter until
-// the list is exhausted.
+// hipGraph): a one-thread kernel resets the work counter, then the persistent
+// kernel's workgroups pull items from the counter until the list is
+// exhausted.  In bf16-input mode the fp32 operands are rounded to bf16 as
+// they are staged into LDS, so no separate conversion pass runs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,17 +32,11 @@ namespace mbh {
 
 struct VbProblem {
     int64_t m, n, k, lda, ldb, ldc;
-    int cls, tiles_n, ktiles, pad;
-    int64_t a_off, b_off;  // bf16 mode: panel offsets (bf16 elements) in the pack buffer
+    int cls, tiles_n;
 };
 
 struct VbItem {
     int prob, tile;
-};
-
-struct VbPackBlock {
-    int prob, which;  // which: 0 = op(A) rows (m), 1 = op(B)^T rows (n)
-    int r0, k0;
 };
 
 // Tile classes: 0 = 64x64 (2x2 waves of 32x32), 1 = 128x128 (2x2 waves of 64x64).
@@ -63,7 +56,6 @@ constexpr int VB_BK64 = 32, VB_BK128 = 16;
 constexpr int FP_LDS64 = FpTileLds<float, float, float, false, false, 64, 64, VB_BK64>::BYTES;
 constexpr int FP_LDS128 = FpTileLds<float, float, float, false, false, 128, 128, VB_BK128>::BYTES;
 constexpr int FP_LDS = FP_LDS128 > FP_LDS64 ? FP_LDS128 : FP_LDS64;
-constexpr int BF_LDS = PackedTileLds<1, 128, 128>::TOTAL;
 
 // Work-queue walk shared by the persistent kernels: thread 0 claims item
 // i+1 from the counter while the workgroup computes item i, so the atomic's
@@ -120,67 +112,106 @@ __global__ __launch_bounds__(256, 2) void vb_gemm_f32_kernel(const VbProblem* __
     });
 }
 
-// bf16 pack of every problem: block b moves 64 rows x 32 k of op(A) (m x k,
-// m-contiguous for N) or op(B)^T (n x k, k-contiguous for N) into the
-// problem's panel (tile rows of the problem's class).
-__global__ __launch_bounds__(256) void vb_pack_kernel(const VbProblem* __restrict__ probs,
-                                                      const VbPackBlock* __restrict__ blocks,
-                                                      const float* const* A, const float* const* B, uint16_t* pack) {
-    __shared__ float s[64][33];
-    const VbPackBlock blk = blocks[blockIdx.x];
-    const VbProblem pr = probs[blk.prob];
-    PackParams pp{};
-    if (blk.which == 0) {
-        pp.X = A[blk.prob];
-        pp.ld = pr.lda;
-        pp.R = pr.m;
-        pp.hi = pack + pr.a_off;
-        pp.vec = ((uintptr_t)pp.X % 16 == 0) && (pr.lda % 4 == 0);
-    } else {
-        pp.X = B[blk.prob];
-        pp.ld = pr.ldb;
-        pp.R = pr.n;
-        pp.hi = pack + pr.b_off;
-        pp.vec = ((uintptr_t)pp.X % 16 == 0) && (pr.ldb % 4 == 0);
+// bf16-input tile: C tile (tm, tn) of problem p, BT x BT, 2 x 2 waves.  The
+// fp32 panels of A (m-contiguous) and B (k-contiguous, staged as op(B)^T
+// rows) are loaded with bounds checks, rounded to bf16 and written in the
+// packed LDS layout of the split-precision kernels (64-byte k-rows, chunk
+// index XOR-swizzled by row), whose fragment reads and MFMAs are reused.
+template <int BT>
+__device__ inline void vb_bf16in_tile(const GemmParams<float>& p, int tm, int tn, u32x4* lds0, u32x4* lds1) {
+    constexpr int NT = 256, BK = 32, CH = BT * 4;  // CH: 16-byte chunks per operand per buffer
+    constexpr int TM = BT / 64, TN = BT / 64;     // 32x32 blocks per wave
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    PanelLoader<float, float, false, BT, BK, NT> la;  // A(i, k) at A[i + k*lda]
+    PanelLoader<float, float, true, BT, BK, NT> lb;   // op(B)^T(j, k) at B[k + j*ldb]
+    MBH_LDS u32x4* L[2] = {(MBH_LDS u32x4*)lds0, (MBH_LDS u32x4*)lds1};
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave & 1) * (BT / 2), wn0 = (wave >> 1) * (BT / 2);
+    const int64_t m0 = (int64_t)tm * BT, n0 = (int64_t)tn * BT;
+
+    // Registers -> bf16 in LDS.  A: a thread holds 4 consecutive rows at one
+    // k (four 2-byte writes); B: 4 consecutive k of one row (one 8-byte write).
+    auto stage = [&](MBH_LDS u32x4* buf) {
+        MBH_LDS uint16_t* a16 = (MBH_LDS uint16_t*)buf;
+        MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + CH);
+#pragma unroll
+        for (int v = 0; v < la.NV; ++v) {
+            const int idx = tid + NT * v;
+            const int kk = idx / (BT / 4), row = (idx % (BT / 4)) * 4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = row + e;
+                a16[r * 32 + (((kk >> 3) ^ lds_swz(r)) << 3) + (kk & 7)] = (uint16_t)bf16_rn_bits(la.r[v][e]);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < lb.NV; ++v) {
+            const int idx = tid + NT * v;
+            const int row = idx / (BK / 4), kk = (idx % (BK / 4)) * 4;
+            const u32x2 w = {bf16_rn_bits(lb.r[v][0]) | (bf16_rn_bits(lb.r[v][1]) << 16),
+                             bf16_rn_bits(lb.r[v][2]) | (bf16_rn_bits(lb.r[v][3]) << 16)};
+            *(MBH_LDS u32x2*)(b16 + row * 32 + (((kk >> 3) ^ lds_swz(row)) << 3) + (kk & 7)) = w;
+        }
+    };
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int nkt = (int)((p.k + BK - 1) / BK);
+    if (nkt > 0) {
+        la.load(p.A, p.lda, m0, p.m, 0, p.k, p.a_vec, tid);
+        lb.load(p.B, p.ldb, n0, p.n, 0, p.k, p.b_vec, tid);
+        stage(L[0]);
     }
-    pp.K = pr.k;
-    pp.ktiles = pr.ktiles;
-    pp.lo = nullptr;
-    if (blk.which == 0) {  // A 'N': element (i, k) at A[i + k*lda]
-        if (pr.cls == 0) pack_split_block<false, 64, 32>(pp, blk.r0, blk.k0, s);
-        else pack_split_block<false, 128, 32>(pp, blk.r0, blk.k0, s);
-    } else {  // B 'N': element (j, k) of B^T at B[k + j*ldb]
-        if (pr.cls == 0) pack_split_block<true, 64, 32>(pp, blk.r0, blk.k0, s);
-        else pack_split_block<true, 128, 32>(pp, blk.r0, blk.k0, s);
+    __syncthreads();
+    for (int kt = 0; kt < nkt; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nkt) {
+            la.load(p.A, p.lda, m0, p.m, (int64_t)(kt + 1) * BK, p.k, p.a_vec, tid);
+            lb.load(p.B, p.ldb, n0, p.n, (int64_t)(kt + 1) * BK, p.k, p.b_vec, tid);
+        }
+        packed_ktile_mfma<1, TM, TN>(L[cur], L[cur], L[cur] + CH, L[cur] + CH, wm0, wn0, lane, 0, 0, acc);
+        if (kt + 1 < nkt) stage(L[cur ^ 1]);
+        __syncthreads();
     }
+    epilogue_32x32<TM, TN>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, TRI_NONE);
 }
 
 // Two waves per SIMD: the register cap keeps two workgroups resident per CU.
 template <int MAXCLS>
 __global__ __launch_bounds__(256, 2) void vb_gemm_bf16_kernel(const VbProblem* __restrict__ probs,
                                                            const VbItem* __restrict__ items, int nitems, int* ctr,
-                                                           float alpha, float beta, const uint16_t* pack,
-                                                           float* const* C) {
-    constexpr int LDS = MAXCLS == 0 ? PackedTileLds<1, 64, 64>::TOTAL : BF_LDS;
-    __shared__ u32x4 lds0[LDS / 2];
-    __shared__ u32x4 lds1[LDS / 2];
+                                                           float alpha, float beta, const float* const* A,
+                                                           const float* const* B, float* const* C) {
+    constexpr int CH = (MAXCLS == 0 ? 64 : 128) * 4;
+    __shared__ u32x4 lds0[2 * CH];
+    __shared__ u32x4 lds1[2 * CH];
     __shared__ int slot;
     vb_walk(ctr, nitems, &slot, [&](int it) {
         const VbItem w = items[it];
         const VbProblem pr = probs[w.prob];
-        PackedGemmParams p{};
+        GemmParams<float> p{};
         p.m = pr.m;
         p.n = pr.n;
-        p.ktiles = pr.ktiles;
-        p.Ah = pack + pr.a_off;
-        p.Bh = pack + pr.b_off;
+        p.k = pr.k;
+        p.A = A[w.prob];
+        p.lda = pr.lda;
+        p.B = B[w.prob];
+        p.ldb = pr.ldb;
         p.C = C[w.prob];
         p.ldc = pr.ldc;
         p.alpha = alpha;
         p.beta = beta;
+        p.a_vec = ((uintptr_t)p.A % 16 == 0) && (pr.lda % 4 == 0);
+        p.b_vec = ((uintptr_t)p.B % 16 == 0) && (pr.ldb % 4 == 0);
         const int tm = w.tile / pr.tiles_n, tn = w.tile % pr.tiles_n;
-        if (MAXCLS == 0 || pr.cls == 0) gemm_packed_tile_dma<1, 64, 64, 2, 2>(p, tm, tn, lds0, lds1);
-        else gemm_packed_tile_dma<1, 128, 128, 2, 2>(p, tm, tn, lds0, lds1);
+        if (MAXCLS == 0 || pr.cls == 0) vb_bf16in_tile<64>(p, tm, tn, lds0, lds1);
+        else vb_bf16in_tile<128>(p, tm, tn, lds0, lds1);
     });
 }
 
@@ -228,13 +259,11 @@ int bucket_of(int64_t x) {
 struct VbPlan {
     int64_t count = 0;
     int prec = 0;
-    int nitems = 0, nblocks = 0, grid = 0;
+    int nitems = 0, grid = 0;
     double flops = 0.0;
     VbProblem* probs = nullptr;
     VbItem* items = nullptr;
-    VbPackBlock* blocks = nullptr;
     int* ctr = nullptr;
-    uint16_t* pack = nullptr;
     int64_t class_count[2] = {0, 0};
 };
 
@@ -312,8 +341,6 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         double cost;
     };
     std::vector<Item> items;
-    std::vector<VbPackBlock> blocks;
-    int64_t pack_el = 0;
     for (int64_t p = 0; p < count; ++p) {
         VbProblem& q = probs[(size_t)p];
         q.m = m[p];
@@ -326,35 +353,21 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         const int T = vb_tile(q.cls);
         const int64_t tm = (q.m + T - 1) / T, tn = (q.n + T - 1) / T;
         q.tiles_n = (int)tn;
-        q.ktiles = (int)((q.k + 31) / 32);
         plan->flops += 2.0 * (double)q.m * (double)q.n * (double)q.k;
         if (q.m == 0 || q.n == 0) continue;
         plan->class_count[q.cls]++;
         for (int64_t t = 0; t < tm * tn; ++t) items.push_back({(int)p, (int)t, (double)T * T * (double)q.k});
-        if (precision == MBH_PREC_BF16) {
-            q.a_off = pack_el;
-            pack_el += tm * T * q.ktiles * 32;
-            q.b_off = pack_el;
-            pack_el += tn * T * q.ktiles * 32;
-            for (int which = 0; which < 2; ++which) {
-                const int64_t rows = (which == 0 ? tm : tn) * T;
-                for (int64_t r0 = 0; r0 < rows; r0 += 64)
-                    for (int64_t k0 = 0; k0 < (int64_t)q.ktiles * 32; k0 += 32)
-                        blocks.push_back({(int)p, which, (int)r0, (int)k0});
-            }
-        }
     }
     // Longest-first: the dynamic queue then finishes with the short items.
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
     std::vector<VbItem> ditems(items.size());
     for (size_t i = 0; i < items.size(); ++i) ditems[i] = {items[i].prob, items[i].tile};
-    if (items.size() > (size_t)INT32_MAX || blocks.size() > (size_t)INT32_MAX) {
+    if (items.size() > (size_t)INT32_MAX) {
         delete plan;
         set_error("mbh_vbatch_plan_create: too many tiles");
         return MBH_E_ARG;
     }
     plan->nitems = (int)ditems.size();
-    plan->nblocks = (int)blocks.size();
     int dev = 0, ncu = 256;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) {
@@ -381,15 +394,6 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
             hipSuccess)
             return fail(e, "vbatch plan: items copy");
     }
-    if (!blocks.empty()) {
-        if ((e = hipMalloc(&plan->blocks, sizeof(VbPackBlock) * blocks.size())) != hipSuccess)
-            return fail(e, "vbatch plan: pack blocks");
-        if ((e = hipMemcpy(plan->blocks, blocks.data(), sizeof(VbPackBlock) * blocks.size(),
-                           hipMemcpyHostToDevice)) != hipSuccess)
-            return fail(e, "vbatch plan: pack blocks copy");
-        if ((e = hipMalloc(&plan->pack, sizeof(uint16_t) * (size_t)pack_el)) != hipSuccess)
-            return fail(e, "vbatch plan: bf16 panels");
-    }
     *out = (mbh_vbatch_plan)plan;
     return 0;
 }
@@ -406,7 +410,7 @@ int mbh_vbatch_plan_info(mbh_vbatch_plan handle, double* info) {
     info[3] = plan->flops;
     info[4] = (double)plan->class_count[0];
     info[5] = (double)plan->class_count[1];
-    info[6] = (double)plan->nblocks;
+    info[6] = 0.0;  // conversion pass work units (none: bf16 staging is fused)
     info[7] = (double)plan->prec;
     return 0;
 }
@@ -432,13 +436,12 @@ int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* co
     const dim3 grid((unsigned)plan->grid), block(256);
     const bool mixed = plan->class_count[1] > 0;
     if (plan->prec == MBH_PREC_BF16) {
-        vb_pack_kernel<<<dim3((unsigned)plan->nblocks), 256, 0, s>>>(plan->probs, plan->blocks, a, b, plan->pack);
         if (mixed)
             vb_gemm_bf16_kernel<1><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
-                                                          beta, plan->pack, c);
+                                                          beta, a, b, c);
         else
             vb_gemm_bf16_kernel<0><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
-                                                          beta, plan->pack, c);
+                                                          beta, a, b, c);
     } else {
         if (mixed)
             vb_gemm_f32_kernel<1><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
@@ -458,9 +461,7 @@ int mbh_vbatch_plan_destroy(mbh_vbatch_plan handle) {
     hipError_t e = hipDeviceSynchronize();
     if (plan->probs) (void)hipFree(plan->probs);
     if (plan->items) (void)hipFree(plan->items);
-    if (plan->blocks) (void)hipFree(plan->blocks);
     if (plan->ctr) (void)hipFree(plan->ctr);
-    if (plan->pack) (void)hipFree(plan->pack);
     delete plan;
     return e == hipSuccess ? 0 : hip_fail(e, "mbh_vbatch_plan_destroy");
 }
