@@ -10,6 +10,12 @@
   cfg3b  ssyrk L,N n=8192, k=2048, alpha=1, beta=0: fp32 / bf16x3
   cfg4   4096 variable-size N,N problems (m, n, k log-uniform in [16, 512]),
          fp32 and bf16-input, eager exec and hipGraph replay
+  cfg5   adjoints at 2048: C = A B forward + backward (dA, dB) and
+         X = A^-1 B (strsm L,L,N,N) forward + backward (dB, dA), with the
+         error of each gradient against fp64 analytic gradients
+  l3     the level-3 routines without a sized workload of their own: ssymm
+         L,L m=n=8192, strmm L,L,N,N m=n=4096, ssyr2k L,N n=8192 k=2048
+  dgemm  DGEMM N,N 8192^3 on the fp64 MFMA (alpha=1, beta=0)
 
     python -m mojoblas_amd.bench_configs [--only cfg4,cfg3a] [--reps 10]
 
@@ -187,7 +193,137 @@ def cfg4(torch, mb, reps):
         plan.close()
 
 
-CONFIGS = {"cfg1": cfg1, "cfg2b": cfg2b, "cfg2h": cfg2h, "cfg3a": cfg3a, "cfg3b": cfg3b, "cfg4": cfg4}
+def cfg5(torch, mb, reps):
+    from mojoblas_amd import autograd as ag
+
+    n = 2048
+    g = torch.Generator(device="cuda").manual_seed(6)
+    A = torch.randn(n, n, device="cuda", generator=g)
+    B = torch.randn(n, n, device="cuda", generator=g)
+    dC = torch.randn(n, n, device="cuda", generator=g)
+    fl = 2.0 * n ** 3
+    A64, B64, dC64 = A.double(), B.double(), dC.double()
+    dA_ref, dB_ref = dC64 @ B64.t(), A64.t() @ dC64
+    for prec in ("fp32", "bf16x3"):
+        Ar = A.clone().requires_grad_(True)
+        Br = B.clone().requires_grad_(True)
+
+        def fwd():
+            return ag.gemm(Ar, Br, 1.0, prec)
+
+        def fwd_bwd():
+            Ar.grad = None
+            Br.grad = None
+            fwd().backward(dC)
+
+        ms_f = timed(torch, fwd, reps)
+        ms_fb = timed(torch, fwd_bwd, reps)
+        fwd_bwd()
+        torch.cuda.synchronize()
+        emit({"config": "cfg5", "routine": "gemm C = A B, 2048^3, forward + backward (dA = dC B^T, dB = A^T dC)",
+              "precision": prec, "ms_forward": ms_f, "ms_backward": ms_fb - ms_f,
+              "tflops_forward": fl / ms_f / 1e9, "tflops_backward": 2 * fl / (ms_fb - ms_f) / 1e9,
+              "error_dA_vs_fp64": err_stats(torch, Ar.grad.double(), dA_ref),
+              "error_dB_vs_fp64": err_stats(torch, Br.grad.double(), dB_ref)})
+    # X = L^-1 B, L lower triangular: off-diagonal U[-1,1]/n, diagonal U[1,2]
+    L = (torch.rand(n, n, device="cuda", generator=g, dtype=torch.float64) * 2 - 1) / n
+    L = torch.tril(L, -1) + torch.diag(1 + torch.rand(n, device="cuda", generator=g, dtype=torch.float64))
+    Lf = L.float()
+    dX = torch.randn(n, n, device="cuda", generator=g)
+    X64 = torch.linalg.solve_triangular(L, B64, upper=False)
+    dB_ref = torch.linalg.solve_triangular(L.t(), dX.double(), upper=True)
+    dL_ref = -torch.tril(dB_ref @ X64.t())
+    fl_t = float(n) ** 3
+    for prec in ("fp32", "bf16x3"):
+        Lr = Lf.clone().requires_grad_(True)
+        Br = B.clone().requires_grad_(True)
+
+        def fwd():
+            return ag.trsm(Lr, Br, "L", "L", "N", "N", 1.0, prec)
+
+        def fwd_bwd():
+            Lr.grad = None
+            Br.grad = None
+            fwd().backward(dX)
+
+        ms_f = timed(torch, fwd, reps)
+        ms_fb = timed(torch, fwd_bwd, reps)
+        fwd_bwd()
+        torch.cuda.synchronize()
+        # backward work: one solve (m^2 n) plus the masked product tril(dB X^T) (about m^2 n)
+        emit({"config": "cfg5", "routine": "strsm X = A^-1 B, L,L,N,N 2048^2, forward + backward "
+                                           "(dB = A^-T dX, dA = -tril(dB X^T))",
+              "precision": prec, "ms_forward": ms_f, "ms_backward": ms_fb - ms_f,
+              "tflops_forward": fl_t / ms_f / 1e9, "tflops_backward": 2 * fl_t / (ms_fb - ms_f) / 1e9,
+              "error_dA_vs_fp64": err_stats(torch, torch.tril(Lr.grad.double()), dL_ref),
+              "error_dB_vs_fp64": err_stats(torch, Br.grad.double(), dB_ref)})
+
+
+def l3(torch, mb, reps):
+    g = torch.Generator(device="cuda").manual_seed(8)
+    # ssymm L,L: C = A B, A symmetric 8192^2 (lower triangle read), B 8192^2
+    m = n = 8192
+    A = torch.randn(m * m, device="cuda", generator=g)
+    B = torch.randn(m * n, device="cuda", generator=g)
+    C = torch.zeros(m * n, device="cuda")
+    a = torch.tril(A.view(m, m).t().double())
+    a = a + torch.tril(a, -1).t()
+    ref = a @ B.view(n, m).t().double()
+    fl = 2.0 * m * m * n
+    for prec in ("fp32", "bf16x3"):
+        ms = timed(torch, lambda: mb.symm("L", "L", m, n, 1.0, A, m, B, m, 0.0, C, m, precision=prec), reps)
+        e = err_stats(torch, C.view(n, m).t().double(), ref)
+        emit({"config": "l3", "routine": "ssymm L,L m=n=8192", "precision": prec, "ms": ms,
+              "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK[prec], "error_vs_fp64": e})
+    del A, B, C, a, ref
+    # strmm L,L,N,N: B := A B, A lower 4096^2, B 4096^2
+    m = n = 4096
+    A = torch.randn(m * m, device="cuda", generator=g)
+    B0 = torch.randn(m * n, device="cuda", generator=g)
+    B = torch.empty_like(B0)
+    ref = torch.tril(A.view(m, m).t().double()) @ B0.view(n, m).t().double()
+    fl = float(m) * m * n
+    for prec in ("fp32", "bf16x3"):
+        def run():
+            B.copy_(B0)
+            mb.trmm("L", "L", "N", "N", m, n, 1.0, A, m, B, m, precision=prec)
+        ms = timed(torch, run, reps) - timed(torch, lambda: B.copy_(B0), reps)
+        run()
+        e = err_stats(torch, B.view(n, m).t().double(), ref)
+        emit({"config": "l3", "routine": "strmm L,L,N,N m=n=4096", "precision": prec, "ms": ms,
+              "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK[prec], "error_vs_fp64": e})
+    del A, B0, B, ref
+    # ssyr2k L,N: C = A B^T + B A^T on the lower triangle, n=8192, k=2048
+    n, k = 8192, 2048
+    A = torch.randn(n * k, device="cuda", generator=g)
+    B = torch.randn(n * k, device="cuda", generator=g)
+    C = torch.zeros(n * n, device="cuda")
+    a64, b64 = A.view(k, n).t().double(), B.view(k, n).t().double()
+    ref = torch.tril(a64 @ b64.t() + b64 @ a64.t())
+    fl = 2.0 * n * (n + 1) * k
+    for prec in ("fp32", "bf16x3"):
+        ms = timed(torch, lambda: mb.syr2k("L", "N", n, k, 1.0, A, n, B, n, 0.0, C, n, precision=prec), reps)
+        e = err_stats(torch, torch.tril(C.view(n, n).t().double()), ref)
+        emit({"config": "l3", "routine": "ssyr2k L,N n=8192 k=2048", "precision": prec, "ms": ms,
+              "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK[prec], "error_vs_fp64": e})
+
+
+def dgemm(torch, mb, reps):
+    n = 8192
+    g = torch.Generator(device="cuda").manual_seed(9)
+    A = torch.randn(n * n, device="cuda", generator=g, dtype=torch.float64)
+    B = torch.randn(n * n, device="cuda", generator=g, dtype=torch.float64)
+    C = torch.empty(n * n, device="cuda", dtype=torch.float64)
+    fl = 2.0 * n ** 3
+    ms = timed(torch, lambda: mb.gemm("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n), max(3, reps // 2))
+    ref = A.view(n, n).t() @ B.view(n, n).t()
+    e = err_stats(torch, C.view(n, n).t(), ref)
+    emit({"config": "dgemm", "routine": "dgemm NN 8192^3", "precision": "fp64", "ms": ms,
+          "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK["fp64"], "diff_vs_torch_fp64": e})
+
+
+CONFIGS = {"cfg1": cfg1, "cfg2b": cfg2b, "cfg2h": cfg2h, "cfg3a": cfg3a, "cfg3b": cfg3b, "cfg4": cfg4, "cfg5": cfg5,
+           "l3": l3, "dgemm": dgemm}
 
 
 def main():
