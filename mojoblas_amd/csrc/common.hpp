@@ -105,6 +105,24 @@ enum KRange {
     KR_LE_COL = 4   // op(B) upper: B(k,j) = 0 for k > j      -> k < (tn+1)*BN
 };
 
+// Tile order of a k-range launch: a tile's work grows with its distance
+// along the triangle (KR_LE_ROW: with tm, KR_GE_ROW: with tiles_m - tm, and
+// likewise along tn), so the grid is walked heaviest tile first.  The
+// hardware deals consecutive workgroups round-robin to the XCDs, which then
+// share the heavy tiles evenly (the XCD-contiguous order of tile_coords
+// would hand one XCD all the heavy tile rows).  Consecutive workgroups keep
+// one tile row (column) for A (B) panel reuse.
+__device__ inline void tile_coords_kr(int krange, int tiles_m, int tiles_n, int& tm, int& tn) {
+    const int w = (int)blockIdx.x;
+    switch (krange) {
+        case KR_LE_ROW: tm = tiles_m - 1 - w / tiles_n; tn = w % tiles_n; return;
+        case KR_GE_ROW: tm = w / tiles_n; tn = w % tiles_n; return;
+        case KR_GE_COL: tn = w / tiles_m; tm = w % tiles_m; return;
+        case KR_LE_COL: tn = tiles_n - 1 - w / tiles_m; tm = w % tiles_m; return;
+        default: tile_coords(tiles_m, tiles_n, tm, tn); return;
+    }
+}
+
 __device__ inline void kt_range(int mode, int tm, int tn, int BM, int BN, int BK, int nk, int& kt0, int& kt1) {
     kt0 = 0;
     kt1 = nk;

@@ -157,14 +157,18 @@ int64_t tile_count(int64_t m, int64_t n, int BM, int BN, int tri_grid) {
 
 // Tile size by grid fill: 256x256 (8 waves) while that gives a tile per CU,
 // then 128x128, then 64x64 for the small products of the level-3 recursions.
+// A k-range launch (triangular operand) asks for four tiles per CU before it
+// takes the larger tile: its tiles differ in length, and with one tile per CU
+// the longest one would set the time.
 template <int NPROD>
 hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
     const int64_t t256 = tile_count(g.m, g.n, 256, 256, tri_grid);
+    const int64_t need256 = g.krange ? 1024 : 256;
     if (NPROD == 8) {
-        if (t256 >= 256) return run_fp8<256, 2, 4>(at, bt, g, tri_grid, s);
+        if (t256 >= need256) return run_fp8<256, 2, 4>(at, bt, g, tri_grid, s);
         return run_fp8<128, 2, 2>(at, bt, g, tri_grid, s);
     }
-    if (t256 >= 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s);
+    if (t256 >= need256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s);
     if (tile_count(g.m, g.n, 128, 128, tri_grid) >= 128) return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s);
     return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s);
 }
@@ -224,7 +228,7 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
     GemmParams<float> p = make_params<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, krange);
     switch (prec) {
         case PREC_FP32:
-            if (tile_count(m, n, 128, 128, tri_grid) >= 256)
+            if (tile_count(m, n, 128, 128, tri_grid) >= (krange ? 1024 : 256))
                 return launch_fp<float, float, float, float, 128, 128, 16>(at, bt, p, tri_grid, s);
             return launch_fp<float, float, float, float, 64, 64, 16>(at, bt, p, tri_grid, s);
         case PREC_BF16X3:

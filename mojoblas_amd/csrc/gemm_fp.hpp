@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p)
     __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
     int tm, tn;
     if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
-    else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+    else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
     gemm_fp_tile<T, TA, TB, TC, AT, BT, BM, BN, BK>(p, tm, tn, lds);
 }
 

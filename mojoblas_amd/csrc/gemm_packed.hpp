@@ -459,7 +459,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     __shared__ u32x4 lds1[L::TOTAL / 2];
     int tm, tn;
     if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
-    else tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+    else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
     gemm_packed_tile_dma<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
 }
 

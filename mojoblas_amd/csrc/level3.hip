@@ -87,46 +87,101 @@ __global__ void add_kernel(int64_t m, int64_t n, const T* d, int64_t ldd, T* out
 }
 
 // Inverses of the NB x NB diagonal blocks of op(A) (op(A) is eff_lower /
-// upper triangular, trans = op is A^T), computed in fp64: one workgroup of NB
-// threads per block, thread j solving op(A)_bb x = e_j by substitution.  The
-// running columns live in LDS ([row][col], col contiguous, so the NB threads
-// read one row conflict-free); row i of the block is staged in LDS once per
-// step and read as a broadcast.  Output: inv + b*NB*NB, column-major, ld NB,
-// zero-padded for a short last block.
+// upper triangular, trans = op is A^T), computed in fp64.  Output: inv +
+// b*NB*NB, column-major, ld NB, zero-padded for a short last block.
+//
+// The columns of a triangular inverse are independent, so a block is split
+// over TI_GROUPS workgroups of TI_COLS inverse columns each (grid nblk x
+// TI_GROUPS).  Work happens in "solve space": indices are reversed for an
+// upper block (s(i) = nb-1-i), which turns it into a lower one, L'(i,k) =
+// op(A)_bb(s(i), s(k)) and inv(op(A)_bb)(s(i), s(j)) = inv(L')(i, j).  A
+// workgroup stages the lower triangle of L' (rows and columns from its first
+// column on) in LDS with all its global loads in flight at once, then walks
+// the rows in chunks of TI_CH: one parallel product of the chunk rows with
+// the already solved rows, then a TI_CH x TI_CH substitution per column in
+// registers, scaled by precomputed reciprocals of the diagonal.
+constexpr int TI_COLS = 32, TI_GROUPS = NB / TI_COLS, TI_CH = 16, TI_NT = 256;
+constexpr int TI_TRI = NB * (NB + 1) / 2;  // packed lower triangle of L'
+
+__device__ inline int ti_row(int i) { return i * (i + 1) / 2; }
+
 template <typename Tin>
-__global__ __launch_bounds__(NB) void tri_inv_kernel(int64_t m, const Tin* A, int64_t lda, int eff_lower, int trans,
-                                                     int unit, double* inv) {
-    __shared__ double x[NB][NB + 1];
-    __shared__ double lrow[NB];
-    const int b = blockIdx.x, j = threadIdx.x;
+__global__ __launch_bounds__(TI_NT) void tri_inv_kernel(int64_t m, const Tin* A, int64_t lda, int eff_lower,
+                                                        int trans, int unit, double* inv) {
+    __shared__ double tri[TI_TRI];             // tri[ti_row(i) + k] = L'(i, k), k <= i
+    __shared__ double xs[NB][TI_COLS + 1];     // xs[i][c] = inv(L')(i, c0 + c)
+    __shared__ double rdiag[NB];               // 1 / L'(i, i)
+    const int b = blockIdx.x, g = blockIdx.y, tid = threadIdx.x;
     const int64_t r0 = (int64_t)b * NB;
     const int nb = (int)((m - r0 < NB) ? (m - r0) : NB);
-    auto opA = [&](int i, int k) -> double {
-        const int64_t gi = r0 + i, gk = r0 + k;
-        return trans ? (double)A[gk + gi * lda] : (double)A[gi + gk * lda];
-    };
-    for (int i = 0; i < NB; ++i) x[i][j] = 0.0;
-    for (int step = 0; step < nb; ++step) {
-        const int i = eff_lower ? step : nb - 1 - step;
-        __syncthreads();
-        if (j < nb) {
-            const bool in_tri = eff_lower ? (j <= i) : (j >= i);
-            lrow[j] = (j == i) ? (unit ? 1.0 : opA(i, i)) : (in_tri ? opA(i, j) : 0.0);
+    const int c0 = g * TI_COLS;
+    auto sp = [&](int i) { return eff_lower ? i : nb - 1 - i; };
+
+    // Stage L' rows/columns >= c0 (identity outside the nb x nb block).  The
+    // flat index runs along whichever of (i, k) is contiguous in memory.
+    constexpr int PER = NB * NB / TI_NT;
+    double v[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int idx = tid + e * TI_NT, fast = idx % NB, slow = idx / NB;
+        const int i = trans ? slow : fast, k = trans ? fast : slow;
+        double x = (i == k) ? 1.0 : 0.0;
+        if (k <= i && k >= c0 && i < nb && !(unit && i == k)) {
+            const int64_t gi = r0 + sp(i), gk = r0 + sp(k);
+            x = trans ? (double)A[gk + gi * lda] : (double)A[gi + gk * lda];
         }
-        __syncthreads();
-        if (j < nb) {
-            double s = (i == j) ? 1.0 : 0.0;
-            if (eff_lower) {
-                for (int k = 0; k < i; ++k) s -= lrow[k] * x[k][j];
-            } else {
-                for (int k = i + 1; k < nb; ++k) s -= lrow[k] * x[k][j];
-            }
-            x[i][j] = s / lrow[i];
-        }
+        v[e] = x;
     }
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int idx = tid + e * TI_NT, fast = idx % NB, slow = idx / NB;
+        const int i = trans ? slow : fast, k = trans ? fast : slow;
+        if (k <= i && k >= c0) tri[ti_row(i) + k] = v[e];
+    }
+    for (int e = tid; e < NB * (TI_COLS + 1); e += TI_NT) (&xs[0][0])[e] = 0.0;
     __syncthreads();
+    if (tid < NB && tid >= c0) rdiag[tid] = 1.0 / tri[ti_row(tid) + tid];
+
+    const int col = tid % TI_COLS, r = tid / TI_COLS;  // product: rows r and r + 8 of the chunk
+    constexpr int RPT = TI_CH / (TI_NT / TI_COLS);       // rows per thread in the product
+    for (int ch = c0 / TI_CH; ch < NB / TI_CH; ++ch) {
+        const int i0 = ch * TI_CH;
+        // rhs(i, col) = [i == c0 + col] - sum_{k in [c0, i0)} L'(i, k) xs[k][col]
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int i = i0 + r + q * (TI_NT / TI_COLS);
+            const double* li = tri + ti_row(i);
+            double acc = 0.0;
+            for (int k = c0; k < i0; ++k) acc += li[k] * xs[k][col];
+            xs[i][col] = ((i == c0 + col) ? 1.0 : 0.0) - acc;
+        }
+        __syncthreads();
+        if (tid < TI_COLS) {
+            double x[TI_CH];
+#pragma unroll
+            for (int a = 0; a < TI_CH; ++a) x[a] = xs[i0 + a][tid];
+#pragma unroll
+            for (int a = 0; a < TI_CH; ++a) {
+                const double* la = tri + ti_row(i0 + a) + i0;
+                double t = x[a];
+#pragma unroll
+                for (int q = 0; q < a; ++q) t -= la[q] * x[q];
+                x[a] = t * rdiag[i0 + a];
+            }
+#pragma unroll
+            for (int a = 0; a < TI_CH; ++a) xs[i0 + a][tid] = x[a];
+        }
+        __syncthreads();
+    }
+
+    // Solve-space column c0 + c goes to output column s(c0 + c), rows s(i);
+    // output columns of this group's range that lie past nb are zero.
     double* out = inv + (size_t)b * NB * NB;
-    for (int i = 0; i < NB; ++i) out[i + (size_t)j * NB] = (i < nb && j < nb) ? x[i][j] : 0.0;
+    for (int e = tid; e < NB * TI_COLS; e += TI_NT) {
+        const int pi = e % NB, c = e / NB, j = c0 + c;
+        if (j < nb) out[pi + (size_t)sp(j) * NB] = (pi < nb) ? xs[sp(pi)][c] : 0.0;
+        if (j >= nb) out[pi + (size_t)j * NB] = 0.0;
+    }
 }
 
 template <typename T>
@@ -248,8 +303,8 @@ hipError_t trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t
     T* tri = (T*)(base + al(inv_b) + al(b0_b));
     T* d = (T*)(base + al(inv_b) + al(b0_b) + al(tri_b));
 
-    tri_inv_kernel<T><<<dim3((unsigned)nblk), dim3(NB), 0, s>>>(na, A, lda, eff_lower ? 1 : 0, trans ? 1 : 0,
-                                                                unit ? 1 : 0, inv);
+    tri_inv_kernel<T><<<dim3((unsigned)nblk, TI_GROUPS), dim3(TI_NT), 0, s>>>(na, A, lda, eff_lower ? 1 : 0,
+                                                                             trans ? 1 : 0, unit ? 1 : 0, inv);
     MBH_TRY(hipGetLastError());
     if (do_ir) {
         copy_kernel<T, double><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, B, ldb, b0, m, (double)alpha);
