@@ -64,15 +64,19 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     TimingMark tmk;
     tmk.active = timing_on();
     timing_mark(tmk, 0, s);
-    dim3 pblock(256);
-    dim3 ga((unsigned)(tm * BT / 64), (unsigned)(ktiles * PK / 32));
-    dim3 gb((unsigned)(tn * BT / 64), (unsigned)(ktiles * PK / 32));
-    // op(A) rows = m: k-contiguous iff A is transposed.
-    if (at) pack_split_kernel<true, BT, PK><<<ga, pblock, 0, s>>>(pa);
-    else pack_split_kernel<false, BT, PK><<<ga, pblock, 0, s>>>(pa);
-    // op(B)^T rows = n: k-contiguous iff B is NOT transposed.
-    if (!bt) pack_split_kernel<true, BT, PK><<<gb, pblock, 0, s>>>(pb);
-    else pack_split_kernel<false, BT, PK><<<gb, pblock, 0, s>>>(pb);
+    // One launch packs both operands: 64-row x 128-k blocks, op(A) first.
+    // op(A) rows = m: k-contiguous iff A is transposed; op(B)^T rows = n:
+    // k-contiguous iff B is NOT transposed.
+    const int64_t kblk = (ktiles + PACK_NKB - 1) / PACK_NKB;
+    const int64_t gx_a = tm * BT / 64, gx_b = tn * BT / 64;
+    const int64_t nblk_a = gx_a * kblk, nblk = nblk_a + gx_b * kblk;
+    if (nblk > INT_MAX) return hipErrorInvalidValue;
+    dim3 pgrid((unsigned)nblk), pblock(256);
+    const int ga = (int)gx_a, na = (int)nblk_a, gb = (int)gx_b;
+    if (at && !bt) pack_split_kernel<true, true, BT, PK><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+    else if (at && bt) pack_split_kernel<true, false, BT, PK><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+    else if (!at && !bt) pack_split_kernel<false, true, BT, PK><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+    else pack_split_kernel<false, false, BT, PK><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     p.m = g.m;
     p.n = g.n;

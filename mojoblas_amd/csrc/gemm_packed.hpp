@@ -23,12 +23,16 @@
 // consecutive rows of one C column (coalesced column-major stores).
 //
 // Pipeline: two LDS buffers, one barrier per k-tile.  The grid kernel loads
-// the next k-tile straight into the other buffer (global_load_lds) before the
-// current tile's MFMAs; the register-staged form (gemm_packed_tile) serves
-// the persistent batched kernel.
+// the next k-tile straight into the other buffer (global_load_lds); in the
+// bf16 modes it also software-pipelines the fragment reads (the reads of the
+// next 16-deep k-step are issued between the MFMAs of the current one,
+// gemm_packed_tile_pipe).  FP8 keeps the unpipelined gemm_packed_tile_dma.
+// The register-staged form (gemm_packed_tile) serves the persistent batched
+// kernel.
 #pragma once
 #include "common.hpp"
 #include "gemm_fp.hpp"
+
 
 namespace mbh {
 
@@ -452,6 +456,189 @@ __device__ inline void gemm_packed_tile_dma(const PackedGemmParams& p, int tm, i
     epilogue_32x32<TM, TN>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
 }
 
+// Fragments of one 16-deep k-step of a wave's TM x TN block of 32x32
+// outputs: hi (and, for BF16x3, lo) planes of A and B.
+template <int NPL, int TM, int TN>
+struct PackedFrag {
+    bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+};
+
+// LDS -> registers: chunk `ch` (swizzled) of the wave's rows in each plane.
+template <int NPL, int TM, int TN>
+__device__ inline void packed_frag_read(PackedFrag<NPL, TM, TN>& f, const MBH_LDS u32x4* buf, int oa1, int ob0,
+                                        int ob1, int wm0, int wn0, int lrow, int ch) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int idx = (wm0 + i * 32 + lrow) * 4 + ch;
+        f.ah[i] = __builtin_bit_cast(bf16x8, (u32x4)buf[idx]);
+        if (NPL == 2) f.al[i] = __builtin_bit_cast(bf16x8, (u32x4)buf[oa1 + idx]);
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int idx = (wn0 + j * 32 + lrow) * 4 + ch;
+        f.bh[j] = __builtin_bit_cast(bf16x8, (u32x4)buf[ob0 + idx]);
+        if (NPL == 2) f.bl[j] = __builtin_bit_cast(bf16x8, (u32x4)buf[ob1 + idx]);
+    }
+}
+
+// MFMAs of one k-step.  Per accumulator the products come in the order
+// lo*hi, hi*lo, hi*hi (as in packed_ktile_mfma); between two products on the
+// same accumulator the other TM*TN - 1 accumulators are issued.
+template <int NPROD, int NPL, int TM, int TN>
+__device__ inline void packed_frag_mfma(const PackedFrag<NPL, TM, TN>& f, f32x16 (&acc)[TM][TN]) {
+    if (NPROD == 3) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.bl[j], f.ah[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.bh[j], f.al[i], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.bh[j], f.ah[i], acc[i][j], 0, 0, 0);
+}
+
+// Scheduling of one "read the next k-step while multiplying the current one"
+// region: NREAD ds_reads spread evenly over NMFMA MFMAs.
+template <int NREAD, int NMFMA>
+__device__ inline void packed_interleave() {
+    constexpr int PER = NMFMA / NREAD;
+#pragma unroll
+    for (int q = 0; q < NREAD; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);    // DS read
+        __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);  // MFMA
+    }
+    if (NMFMA > PER * NREAD) __builtin_amdgcn_sched_group_barrier(0x008, NMFMA - PER * NREAD, 0);
+}
+
+// BF16x3 / bf16 tile with the fragment reads software-pipelined: two register
+// sets hold consecutive k-steps, and the reads of step s+1 are issued between
+// the MFMAs of step s, so a wave never waits for LDS with an empty MFMA queue
+// (without this the reads of a step were issued after the previous step's
+// MFMAs and their full latency fell between the two).  The k-tile loads are
+// the direct-to-LDS ones of gemm_packed_tile_dma; one barrier per k-tile,
+// after which the next k-tile's first step is read while the current tile's
+// second step (already in registers) is multiplied.
+template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
+__device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, int tn, u32x4* lds0, u32x4* lds1) {
+    static_assert(NPROD == 1 || NPROD == 3, "bf16 modes only");
+    constexpr int NW = WAVES_M * WAVES_N;
+    constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+    constexpr int TM = WM / 32, TN = WN / 32;
+    constexpr int NPL = (NPROD == 3) ? 2 : 1;
+    constexpr int A_CH = BM * 4, B_CH = BN * 4;
+    constexpr int A_WL = A_CH / (64 * NW), B_WL = B_CH / (64 * NW);
+    static_assert(A_WL * 64 * NW == A_CH && B_WL * 64 * NW == B_CH && A_WL >= 1 && B_WL >= 1,
+                  "tile not divisible into wave-loads");
+    constexpr int NREAD = NPL * (TM + TN), NMFMA = NPROD * TM * TN;
+    typedef const MBH_GLOBAL u32x4* gptr;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+    const int lrow16 = lane >> 2;
+    const int lchunk = lrow16 * 4 + ((lane & 3) ^ lds_swz(lrow16));
+    const size_t a_off = (size_t)tm * p.ktiles * A_CH + lchunk, b_off = (size_t)tn * p.ktiles * B_CH + lchunk;
+    gptr gA0 = ((gptr)p.Ah) + a_off;
+    gptr gB0 = ((gptr)p.Bh) + b_off;
+    gptr gA1 = gA0;
+    gptr gB1 = gB0;
+    if (NPL == 2) {
+        gA1 = ((gptr)p.Al) + a_off;
+        gB1 = ((gptr)p.Bl) + b_off;
+    }
+    MBH_LDS u32x4* const L0 = (MBH_LDS u32x4*)lds0;
+    MBH_LDS u32x4* const L1 = (MBH_LDS u32x4*)lds1;
+    constexpr int OA1 = A_CH, OB0 = NPL * A_CH, OB1 = NPL * A_CH + B_CH;
+
+#define MBH_DMA(kt, LB)                                                                                 \
+    do {                                                                                                \
+        const size_t ga = (size_t)(kt) * A_CH, gb = (size_t)(kt) * B_CH;                                \
+        _Pragma("unroll") for (int v = 0; v < A_WL; ++v) {                                              \
+            const int blk = (wave + v * NW) * 64;                                                       \
+            __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gA0 + ga + blk),                  \
+                                             (MBH_LDS void*)((LB) + blk), 16, 0, 0);                    \
+            if (NPL == 2)                                                                               \
+                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gA1 + ga + blk),              \
+                                                 (MBH_LDS void*)((LB) + OA1 + blk), 16, 0, 0);          \
+        }                                                                                               \
+        _Pragma("unroll") for (int v = 0; v < B_WL; ++v) {                                              \
+            const int blk = (wave + v * NW) * 64;                                                       \
+            __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gB0 + gb + blk),                  \
+                                             (MBH_LDS void*)((LB) + OB0 + blk), 16, 0, 0);              \
+            if (NPL == 2)                                                                               \
+                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gB1 + gb + blk),              \
+                                                 (MBH_LDS void*)((LB) + OB1 + blk), 16, 0, 0);          \
+        }                                                                                               \
+        __builtin_amdgcn_sched_barrier(0);                                                              \
+    } while (0)
+#define MBH_STEP(FN, BUF, CH, FC)                                                                       \
+    do {                                                                                                \
+        packed_frag_read<NPL, TM, TN>(FN, BUF, OA1, OB0, OB1, wm0, wn0, lrow, CH);                      \
+        packed_frag_mfma<NPROD, NPL, TM, TN>(FC, acc);                                                  \
+        packed_interleave<NREAD, NMFMA>();                                                              \
+        __builtin_amdgcn_sched_barrier(0);                                                              \
+    } while (0)
+#define MBH_SYNC()                                                                                      \
+    do {                                                                                                \
+        __builtin_amdgcn_s_waitcnt(0);                                                                  \
+        __syncthreads();                                                                                \
+    } while (0)
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    int kt0, kt1;
+    kt_range(p.krange, tm, tn, BM, BN, 32, p.ktiles, kt0, kt1);
+    const int lrow = lane & 31, lgrp = lane >> 5, lsw = lds_swz(lrow);
+    const int ch0 = lgrp ^ lsw, ch1 = (2 + lgrp) ^ lsw;  // chunks of k-steps 0 and 1
+    PackedFrag<NPL, TM, TN> f0, f1;
+    if (kt0 < kt1) {
+        MBH_DMA(kt0, L0);
+        MBH_SYNC();
+        if (kt0 + 1 < kt1) MBH_DMA(kt0 + 1, L1);
+        packed_frag_read<NPL, TM, TN>(f0, L0, OA1, OB0, OB1, wm0, wn0, lrow, ch0);
+    }
+    // Straight-line body, one exit: the accumulators stay in place.  The last
+    // step's read of the next k-tile may read a stale buffer (never used).
+    int kt = kt0;
+    for (; kt + 1 < kt1; kt += 2) {
+        // L0: k-tile kt (f0 = its step 0); L1: k-tile kt+1 in flight
+        MBH_STEP(f1, L0, ch1, f0);
+        MBH_SYNC();
+        if (kt + 2 < kt1) MBH_DMA(kt + 2, L0);
+        MBH_STEP(f0, L1, ch0, f1);
+        // L1: k-tile kt+1 (f0 = its step 0); L0: k-tile kt+2 in flight
+        MBH_STEP(f1, L1, ch1, f0);
+        MBH_SYNC();
+        if (kt + 3 < kt1) MBH_DMA(kt + 3, L1);
+        MBH_STEP(f0, L0, ch0, f1);
+    }
+    if (kt < kt1) {  // odd count: k-tile kt in L0, f0 = its step 0
+        MBH_STEP(f1, L0, ch1, f0);
+        packed_frag_mfma<NPROD, NPL, TM, TN>(f1, acc);
+    }
+#undef MBH_SYNC
+#undef MBH_STEP
+#undef MBH_DMA
+
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    epilogue_32x32<TM, TN>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
+}
+
+
 template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(PackedGemmParams p) {
     typedef PackedTileLds<NPROD, BM, BN> L;
@@ -460,7 +647,10 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     int tm, tn;
     if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
     else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
-    gemm_packed_tile_dma<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
+    if constexpr (NPROD != 8)
+        gemm_packed_tile_pipe<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
+    else
+        gemm_packed_tile_dma<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
 }
 
 }  // namespace mbh

@@ -7,8 +7,8 @@
 // k-tile of a macro-tile panel as one contiguous, bounds-check-free run.
 //
 // Split: hi = rn_bf16(x), lo = rn_bf16(x - hi); x - hi is exact in fp32.
-// One workgroup moves a 64-row x 32-k fp32 block through LDS so that both the
-// HBM read (along whichever index is contiguous) and the packed write (k
+// One workgroup moves a 64-row x 128-k fp32 block through LDS so that both
+// the HBM read (along whichever index is contiguous) and the packed write (k
 // fastest) are coalesced.
 #pragma once
 #include "common.hpp"
@@ -39,71 +39,94 @@ struct PackParams {
     int vec;          // X 16-byte aligned and ld % 4 == 0 (vector loads along k)
 };
 
-// One 64-row x 32-k block (r0, k0) of the split pack; `s` is a 64 x 33
-// float LDS tile.  Shared by the grid kernel and the batched pack kernel.
-template <bool KCONTIG, int PR, int PK>
-__device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t k0, float (*s)[33]) {
-    constexpr int BR = 64, BKK = 32;  // fp32 block moved per workgroup
-    static_assert(PR % BR == 0 && PK % BKK == 0, "pack block must tile the gemm block");
+// One 64-row x (32*NKB)-k block (r0, k0) of the split pack, staged in the
+// LDS tile `s` (64 rows of 32*NKB + 1 floats).  Every thread issues all of
+// its 16-byte loads (along whichever index is contiguous in HBM) before the
+// first LDS store, so a workgroup keeps 32*NKB KB of reads in flight; the
+// write side stores 16 bytes per plane per thread, 64 contiguous bytes per
+// packed row.
+template <bool KCONTIG, int PR, int PK, int NKB>
+__device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t k0, float* s) {
+    constexpr int BR = 64, BKK = 32 * NKB, S = BKK + 1, NT = 256;
+    constexpr int NLD = BR * BKK / 4 / NT;  // 16-byte loads per thread
+    static_assert(PR % BR == 0 && PK == 32, "pack block must tile the gemm block");
+    static_assert(NLD * 4 * NT == BR * BKK, "block not divisible by the thread block");
     const int tid = threadIdx.x;
     const MBH_GLOBAL float* const X = (const MBH_GLOBAL float*)p.X;
-    if (KCONTIG) {
-        // element (r, k) at X[k + r*ld]: 8 lanes cover one row's 32 k.
+    // Load q of this thread: KCONTIG -> 4 consecutive k of one row, else 4
+    // consecutive rows at one k.
+    auto coords = [&](int q, int& r, int& kk) {
+        const int idx = tid + q * NT;
+        if (KCONTIG) {
+            r = idx / (BKK / 4);
+            kk = (idx % (BKK / 4)) * 4;
+        } else {
+            kk = idx / (BR / 4);
+            r = (idx % (BR / 4)) * 4;
+        }
+    };
+    f32x4 v[NLD];
 #pragma unroll
-        for (int pass = 0; pass < 2; ++pass) {
-            const int r = pass * 32 + tid / 8, kq = (tid % 8) * 4;
-            const int64_t gr = r0 + r;
-            if (p.vec && gr < p.R && k0 + kq + 4 <= p.K) {
-                const f32x4 v = *(const MBH_GLOBAL f32x4*)(X + (k0 + kq) + gr * p.ld);
+    for (int q = 0; q < NLD; ++q) {
+        int r, kk;
+        coords(q, r, kk);
+        const int64_t gr = r0 + r, gk = k0 + kk;
+        const bool full = p.vec && (KCONTIG ? (gr < p.R && gk + 4 <= p.K) : (gr + 4 <= p.R && gk < p.K));
+        if (full) {
+            v[q] = *(const MBH_GLOBAL f32x4*)(KCONTIG ? X + gk + gr * p.ld : X + gr + gk * p.ld);
+        } else {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) s[r][kq + e] = v[e];
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int64_t gk = k0 + kq + e;
-                    s[r][kq + e] = (gr < p.R && gk < p.K) ? X[gk + gr * p.ld] : 0.0f;
-                }
+            for (int e = 0; e < 4; ++e) {
+                const int64_t er = KCONTIG ? gr : gr + e, ek = KCONTIG ? gk + e : gk;
+                v[q][e] = (er < p.R && ek < p.K) ? (KCONTIG ? X[ek + er * p.ld] : X[er + ek * p.ld]) : 0.0f;
             }
         }
-    } else {
-        // element (r, k) at X[r + k*ld]: 64 lanes cover one k column's 64 rows.
+    }
 #pragma unroll
-        for (int pass = 0; pass < 8; ++pass) {
-            const int kk = pass * 4 + tid / 64, r = tid % 64;
-            const int64_t gr = r0 + r, gk = k0 + kk;
-            s[r][kk] = (gr < p.R && gk < p.K) ? X[gr + gk * p.ld] : 0.0f;
+    for (int q = 0; q < NLD; ++q) {
+        int r, kk;
+        coords(q, r, kk);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (KCONTIG) s[r * S + kk + e] = v[q][e];
+            else s[(r + e) * S + kk] = v[q][e];
         }
     }
     __syncthreads();
-    // Each thread: 8 consecutive k of one row -> 16 bytes per plane.
+    // Each thread: 8 consecutive k of one row per 32-k sub-block.
     const int r = tid / 4, kq = (tid % 4) * 8;
     const int64_t gr = r0 + r;
     const int64_t t = gr / PR, rr = gr % PR;
-    const int64_t gk = k0 + kq;
-    const int64_t kt = gk / PK, kk = gk % PK;
-    const int64_t off = ((t * p.ktiles + kt) * PR + rr) * PK + kk;
-    uint32_t h[8], l[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) split_bf16(s[r][kq + e], h[e], l[e]);
-    uint4 hv, lv;
-    hv.x = h[0] | (h[1] << 16);
-    hv.y = h[2] | (h[3] << 16);
-    hv.z = h[4] | (h[5] << 16);
-    hv.w = h[6] | (h[7] << 16);
-    *(MBH_GLOBAL u32x4*)(p.hi + off) = u32x4{hv.x, hv.y, hv.z, hv.w};
-    if (p.lo) {
-        lv.x = l[0] | (l[1] << 16);
-        lv.y = l[2] | (l[3] << 16);
-        lv.z = l[4] | (l[5] << 16);
-        lv.w = l[6] | (l[7] << 16);
-        *(MBH_GLOBAL u32x4*)(p.lo + off) = u32x4{lv.x, lv.y, lv.z, lv.w};
+    for (int sb = 0; sb < NKB; ++sb) {
+        const int64_t kt = (k0 + sb * 32) / PK;  // the same for every thread of the sub-block
+        if (kt >= p.ktiles) break;
+        const int64_t off = ((t * p.ktiles + kt) * PR + rr) * PK + kq;
+        uint32_t h[8], l[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) split_bf16(s[r * S + sb * 32 + kq + e], h[e], l[e]);
+        *(MBH_GLOBAL u32x4*)(p.hi + off) =
+            u32x4{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+        if (p.lo)
+            *(MBH_GLOBAL u32x4*)(p.lo + off) =
+                u32x4{l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16), l[6] | (l[7] << 16)};
     }
 }
 
-template <bool KCONTIG, int PR, int PK>
-__global__ __launch_bounds__(256) void pack_split_kernel(PackParams p) {
-    __shared__ float s[64][33];
-    pack_split_block<KCONTIG, PR, PK>(p, (int64_t)blockIdx.x * 64, (int64_t)blockIdx.y * 32, s);
+// Both operands of one gemm in one launch: workgroups [0, nblk_a) pack op(A)
+// (grid gx_a row blocks x k blocks), the rest op(B)^T.  KA / KB: k is the
+// contiguous index of op(A) / op(B)^T.
+constexpr int PACK_NKB = 4;  // 32-k sub-blocks per workgroup
+template <bool KA, bool KB, int PR, int PK>
+__global__ __launch_bounds__(256) void pack_split_kernel(PackParams a, PackParams b, int gx_a, int nblk_a, int gx_b) {
+    __shared__ float s[64 * (32 * PACK_NKB + 1)];
+    int w = (int)blockIdx.x;
+    if (w < nblk_a) {
+        pack_split_block<KA, PR, PK, PACK_NKB>(a, (int64_t)(w % gx_a) * 64, (int64_t)(w / gx_a) * 32 * PACK_NKB, s);
+    } else {
+        w -= nblk_a;
+        pack_split_block<KB, PR, PK, PACK_NKB>(b, (int64_t)(w % gx_b) * 64, (int64_t)(w / gx_b) * 32 * PACK_NKB, s);
+    }
 }
 
 }  // namespace mbh
