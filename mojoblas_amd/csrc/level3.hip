@@ -184,6 +184,82 @@ __global__ __launch_bounds__(TI_NT) void tri_inv_kernel(int64_t m, const Tin* A,
     }
 }
 
+// Leaf of the triangular solve: X_b := inv_b * B_b (left, B_b = len rows x
+// n) or X_b := B_b * inv_b (right, B_b = m rows x len columns), in place, in
+// fp64 (inv_b: NB x NB column-major, ld NB, zero-padded past len).
+//
+// A workgroup (4 waves) owns LEAF_W columns (left) or rows (right) of B_b
+// and reads all of them into LDS (as fp64) before it writes any, so in place
+// is safe and n / 16 (m / 16) workgroups run at once.  Wave w produces the
+// 32 x 16 (left) or 16 x 32 (right) block at offset 32w along the block,
+// with v_mfma_f64_16x16x4 (operand fragments as in the tile gemm: the "row"
+// operand indexes C's rows by lane % 16, the "column" operand C's columns;
+// register r of lane l holds C(l % 16, 4r + l / 16) of a 16 x 16 block).
+// Its 2 x 32 fragments of inv_b come straight from global memory (one
+// NB x NB block, shared by all workgroups through L2), all issued up front.
+constexpr int LEAF_W = 16;
+
+template <typename T, bool LEFT>
+__global__ __launch_bounds__(256) void trsm_leaf_kernel(int64_t len, int64_t other, const double* __restrict__ inv,
+                                                        T* B, int64_t ldb) {
+    __shared__ double bs[NB][LEAF_W + 1];  // bs[k][c]: left B_b(k, c0 + c); right B_b(r0 + c, k)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lg = lane >> 4;
+    const int64_t c0 = (int64_t)blockIdx.x * LEAF_W;
+    // inv fragments: left, "row" operand inv(32w + 16i + l16, 4kk + lg);
+    // right, "column" operand inv(4kk + lg, 32w + 16i + l16).
+    double af[2][NB / 4];
+#pragma unroll
+    for (int kk = 0; kk < NB / 4; ++kk)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int q = 32 * wave + 16 * i + l16, k = 4 * kk + lg;
+            af[i][kk] = LEFT ? inv[q + (size_t)k * NB] : inv[k + (size_t)q * NB];
+        }
+    for (int e = tid; e < NB * LEAF_W; e += 256) {
+        int k, c;
+        if (LEFT) {  // consecutive threads: consecutive rows k of one column
+            k = e % NB;
+            c = e / NB;
+        } else {     // consecutive threads: consecutive rows c of one column k
+            c = e % LEAF_W;
+            k = e / LEAF_W;
+        }
+        double v = 0.0;
+        if (k < len && c0 + c < other)
+            v = LEFT ? (double)B[k + (c0 + c) * ldb] : (double)B[(c0 + c) + k * ldb];
+        bs[k][c] = v;
+    }
+    __syncthreads();
+    f64x4 acc[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) acc[i] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < NB / 4; ++kk) {
+        const double bv = bs[4 * kk + lg][l16];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (LEFT)  // C^T block: column operand B_b(k, c0 + l16), row operand inv
+                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, af[i][kk], acc[i], 0, 0, 0);
+            else       // column operand inv(k, col), row operand B_b(r0 + l16, k)
+                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i][kk], bv, acc[i], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            // left: C(row = 32w + 16i + l16, col = c0 + 4r + lg); right: C(row = c0 + l16, col = 32w + 16i + 4r + lg)
+            if (LEFT) {
+                const int64_t row = 32 * wave + 16 * i + l16, col = c0 + 4 * r + lg;
+                if (row < len && col < other) B[row + col * ldb] = (T)acc[i][r];
+            } else {
+                const int64_t row = c0 + l16, col = 32 * wave + 16 * i + 4 * r + lg;
+                if (row < other && col < len) B[row + col * ldb] = (T)acc[i][r];
+            }
+        }
+}
+
 template <typename T>
 hipError_t xgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
                  int64_t ldb, T beta, T* C, int64_t ldc, int prec, hipStream_t s, int tri = 0, int tri_grid = 0,
@@ -229,12 +305,15 @@ struct TriSolver {
 
     hipError_t leaf(T* B, int64_t ldb, int64_t r0, int64_t len) const {
         const double* iv = inv + (size_t)(r0 / NB) * NB * NB;
-        if (left)  // X_b = inv_b * B_b  (len x n), one 128-row tile row: in place is safe
-            return gemm_f64acc<double, T, T>(false, false, len, n, len, 1.0, iv, NB, B + r0, ldb, 0.0, B + r0, ldb, s,
-                                             1);
-        // X_b = B_b * inv_b  (m x len), one 128-column tile column: in place is safe
-        return gemm_f64acc<T, double, T>(false, false, m, len, len, 1.0, B + r0 * ldb, ldb, iv, NB, 0.0,
-                                         B + r0 * ldb, ldb, s, 2);
+        if ((left ? n : m) / LEAF_W >= INT_MAX) return hipErrorInvalidValue;
+        if (left) {  // X_b = inv_b * B_b  (len x n)
+            trsm_leaf_kernel<T, true><<<dim3((unsigned)((n + LEAF_W - 1) / LEAF_W)), 256, 0, s>>>(len, n, iv, B + r0,
+                                                                                              ldb);
+        } else {     // X_b = B_b * inv_b  (m x len)
+            trsm_leaf_kernel<T, false><<<dim3((unsigned)((m + LEAF_W - 1) / LEAF_W)), 256, 0, s>>>(
+                len, m, iv, B + r0 * ldb, ldb);
+        }
+        return hipGetLastError();
     }
 
     hipError_t solve(T* B, int64_t ldb, int64_t r0, int64_t len) const {
