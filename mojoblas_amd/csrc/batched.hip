@@ -77,6 +77,154 @@ __device__ inline void vb_walk(int* ctr, int nitems, int* slot, F&& body) {
     }
 }
 
+// ---- lean 64x64 tile ---------------------------------------------------------
+// One 64x64 C tile of a batched problem on 256 threads (2 x 2 waves of 32x32),
+// 32-deep k-tiles double-buffered in LDS through registers.  The batched
+// operands are column-major with ld = rows, so most columns start off 16-byte
+// alignment; the loads are therefore single dwords at 32-bit element offsets
+// from the operand's (uniform) base, with the lanes of an instruction on
+// consecutive addresses, which coalesces for any ld and alignment.  Masks are
+// formed once per tile (rows, columns) and once per k-tile (depth) instead of
+// per element with 64-bit addresses, which keeps the register count low enough
+// for several workgroups per CU.  The plan takes this path only when every
+// problem's offsets fit in int32 (vb_lean_ok).
+//
+//   A staging: thread t holds A(m0 + t%64, k0 + 8*(t/64) + q), q < 8
+//              (instruction q: 64 lanes read 64 consecutive rows of one column);
+//   B staging: thread t holds B(k0 + t%32, n0 + t/32 + 8q), q < 8
+//              (instruction q: two runs of 32 consecutive k).
+// fp32 mode: LDS [k][64 + 1] per operand, v_mfma_f32_32x32x2_f32.
+// bf16 mode: the packed swizzled 64-byte-row layout of the split-precision
+//            kernels (A: one 16-byte write of 8 k per thread; B: 2-byte
+//            writes), fragments and MFMAs of packed_ktile_mfma.
+template <bool BF16>
+struct VbLean {
+    static constexpr int S = 65;                      // fp32 row stride (floats)
+    static constexpr int BUF = BF16 ? 2 * 64 * 64 : 2 * 32 * S * 4;  // bytes per buffer, A + B
+    static constexpr int BYTES = 2 * BUF;
+};
+
+template <bool BF16>
+__device__ inline void vb_lean_tile(const VbProblem& pr, const float* Ag, const float* Bg, float* Cg, float alpha,
+                                    float beta, int tm, int tn, unsigned char* lds) {
+    typedef VbLean<BF16> L;
+    constexpr int S = L::S;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = (int)pr.m, n = (int)pr.n, K = (int)pr.k, lda = (int)pr.lda, ldb = (int)pr.ldb;
+    const int m0 = tm * 64, n0 = tn * 64;
+    const MBH_GLOBAL float* const A = (const MBH_GLOBAL float*)Ag;
+    const MBH_GLOBAL float* const B = (const MBH_GLOBAL float*)Bg;
+    const int arow = m0 + lane;
+    const bool arow_ok = arow < m;
+    const int ak = 8 * wave;
+    const int bk = tid & 31, bc = tid >> 5;
+    // Branch-free masked loads: a masked lane reads element 0 (every problem
+    // on this path has k >= 1 when it loads) and selects zero, so the 16
+    // loads of a k-tile issue back to back (guarded loads each became a
+    // branch, and the register reuse across them serialized the loads).
+    float ra[8], rb[8];
+    auto load = [&](int k0) {
+        bool va[8], vb[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 + ak + q;
+            va[q] = arow_ok && k < K;
+            ra[q] = A[va[q] ? arow + k * lda : 0];
+        }
+        const int k = k0 + bk;
+        const bool kok = k < K;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int col = n0 + bc + 8 * q;
+            vb[q] = kok && col < n;
+            rb[q] = B[vb[q] ? k + col * ldb : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            ra[q] = va[q] ? ra[q] : 0.0f;
+            rb[q] = vb[q] ? rb[q] : 0.0f;
+        }
+    };
+    auto store = [&](MBH_LDS unsigned char* buf) {
+        if constexpr (BF16) {
+            MBH_LDS u32x4* a4 = (MBH_LDS u32x4*)buf;
+            MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + 64 * 64);
+            a4[lane * 4 + (wave ^ lds_swz(lane))] =
+                u32x4{bf16_rn_bits(ra[0]) | (bf16_rn_bits(ra[1]) << 16), bf16_rn_bits(ra[2]) | (bf16_rn_bits(ra[3]) << 16),
+                      bf16_rn_bits(ra[4]) | (bf16_rn_bits(ra[5]) << 16), bf16_rn_bits(ra[6]) | (bf16_rn_bits(ra[7]) << 16)};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int j = bc + 8 * q;
+                b16[j * 32 + (((bk >> 3) ^ lds_swz(j)) << 3) + (bk & 7)] = (uint16_t)bf16_rn_bits(rb[q]);
+            }
+        } else {
+            MBH_LDS float* as = (MBH_LDS float*)buf;
+            MBH_LDS float* bs = as + 32 * S;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) as[(ak + q) * S + lane] = ra[q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) bs[bk * S + bc + 8 * q] = rb[q];
+        }
+    };
+    const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
+    f32x16 acc[1][1];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[0][0][r] = 0.0f;
+    auto compute = [&](const MBH_LDS unsigned char* buf) {
+        if constexpr (BF16) {
+            const MBH_LDS u32x4* a4 = (const MBH_LDS u32x4*)buf;
+            const MBH_LDS u32x4* b4 = (const MBH_LDS u32x4*)(buf + 64 * 64);
+            packed_ktile_mfma<1, 1, 1>(a4, a4, b4, b4, wm0, wn0, lane, 0, 0, acc);
+        } else {
+            const MBH_LDS float* as = (const MBH_LDS float*)buf + (lane >> 5) * S + wm0 + (lane & 31);
+            const MBH_LDS float* bs = (const MBH_LDS float*)buf + 32 * S + (lane >> 5) * S + wn0 + (lane & 31);
+#pragma unroll
+            for (int kk = 0; kk < 32; kk += 2)
+                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(bs[kk * S], as[kk * S], acc[0][0], 0, 0, 0);
+        }
+    };
+    MBH_LDS unsigned char* const L0 = (MBH_LDS unsigned char*)lds;
+    MBH_LDS unsigned char* const L1 = L0 + L::BUF;
+    const int nkt = (K + 31) / 32;
+    if (nkt > 0) {
+        load(0);
+        store(L0);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nkt; ++kt) {
+        MBH_LDS unsigned char* cur = (kt & 1) ? L1 : L0;
+        MBH_LDS unsigned char* nxt = (kt & 1) ? L0 : L1;
+        if (kt + 1 < nkt) load((kt + 1) * 32);
+        compute(cur);
+        if (kt + 1 < nkt) store(nxt);
+        __syncthreads();
+    }
+    epilogue_32x32<1, 1>(acc, Cg, pr.ldc, pr.m, pr.n, m0 + wm0, n0 + wn0, lane, alpha, beta, TRI_NONE);
+}
+
+template <bool BF16>
+__global__ __launch_bounds__(256, BF16 ? 4 : 2) void vb_gemm_lean_kernel(const VbProblem* __restrict__ probs,
+                                                           const VbItem* __restrict__ items, int nitems, int* ctr,
+                                                           float alpha, float beta, const float* const* A,
+                                                           const float* const* B, float* const* C) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[VbLean<BF16>::BYTES];
+    __shared__ int slot;
+    vb_walk(ctr, nitems, &slot, [&](int it) {
+        const VbItem w = items[it];
+        const VbProblem pr = probs[w.prob];
+        vb_lean_tile<BF16>(pr, A[w.prob], B[w.prob], C[w.prob], alpha, beta, w.tile / pr.tiles_n,
+                           w.tile % pr.tiles_n, lds);
+    });
+}
+
+// A problem can take the lean tile when its element offsets (with the
+// k-tile and tile overhang the tile computes before masking) fit in int32.
+bool vb_lean_ok(const VbProblem& q) {
+    const int64_t lim = (int64_t)1 << 31;
+    return q.m < lim && q.n < lim && q.k < lim && q.lda * (q.k + 64) < lim && q.ldb * (q.n + 64) + q.k + 64 < lim &&
+           q.lda + q.m + 64 < lim;
+}
+
 // MAXCLS: highest tile class compiled in.  The 64x64-only variant needs far
 // fewer registers and LDS, so more workgroups stay resident per CU.
 template <int MAXCLS>
@@ -265,6 +413,7 @@ struct VbPlan {
     VbItem* items = nullptr;
     int* ctr = nullptr;
     int64_t class_count[2] = {0, 0};
+    bool lean = true;  // every problem fits the lean 64x64 tile's int32 offsets
 };
 
 }  // namespace mbh
@@ -356,6 +505,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         plan->flops += 2.0 * (double)q.m * (double)q.n * (double)q.k;
         if (q.m == 0 || q.n == 0) continue;
         plan->class_count[q.cls]++;
+        if (!vb_lean_ok(q)) plan->lean = false;
         for (int64_t t = 0; t < tm * tn; ++t) items.push_back({(int)p, (int)t, (double)T * T * (double)q.k});
     }
     // Longest-first: the dynamic queue then finishes with the short items.
@@ -374,6 +524,20 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
     }
+    // A persistent grid never asks for more workgroups per CU than the kernel
+    // it launches can keep resident (late workgroups would only find the
+    // queue drained).
+    const bool mixed = plan->class_count[1] > 0;
+    const void* kfn = nullptr;
+    if (!mixed && plan->lean)
+        kfn = precision == MBH_PREC_BF16 ? (const void*)vb_gemm_lean_kernel<true> : (const void*)vb_gemm_lean_kernel<false>;
+    else if (precision == MBH_PREC_BF16)
+        kfn = mixed ? (const void*)vb_gemm_bf16_kernel<1> : (const void*)vb_gemm_bf16_kernel<0>;
+    else
+        kfn = mixed ? (const void*)vb_gemm_f32_kernel<1> : (const void*)vb_gemm_f32_kernel<0>;
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kfn, 256, 0) == hipSuccess && resident > 0)
+        wgs = std::min(wgs, resident);
     plan->grid = std::max(1, std::min(plan->nitems, ncu * wgs));
     auto fail = [&](hipError_t err, const char* what) {
         mbh_vbatch_plan_destroy((mbh_vbatch_plan)plan);
@@ -435,7 +599,14 @@ int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* co
     vb_reset_kernel<<<1, 1, 0, s>>>(plan->ctr);
     const dim3 grid((unsigned)plan->grid), block(256);
     const bool mixed = plan->class_count[1] > 0;
-    if (plan->prec == MBH_PREC_BF16) {
+    if (!mixed && plan->lean) {
+        if (plan->prec == MBH_PREC_BF16)
+            vb_gemm_lean_kernel<true><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
+                                                             beta, a, b, c);
+        else
+            vb_gemm_lean_kernel<false><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
+                                                              beta, a, b, c);
+    } else if (plan->prec == MBH_PREC_BF16) {
         if (mixed)
             vb_gemm_bf16_kernel<1><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
                                                           beta, a, b, c);

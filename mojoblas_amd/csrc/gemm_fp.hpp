@@ -95,15 +95,21 @@ struct PanelLoader {
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) r[v][e] = x[e];
             } else {
+                // Branch-free masked loads: a masked element reads src[0]
+                // (the panel has at least one element whenever it loads) and
+                // is replaced by zero, so the loads issue back to back
+                // (guarded loads each became a branch and were serialized).
+                const MBH_GLOBAL TIN* g = (const MBH_GLOBAL TIN*)src;
+                bool ok[VEC];
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) {
                     const int64_t gr = row0 + row + (KCONTIG ? 0 : e);
                     const int64_t gk = k0 + kk + (KCONTIG ? e : 0);
-                    TIN x = TIN(0);
-                    const MBH_GLOBAL TIN* g = (const MBH_GLOBAL TIN*)src;
-                    if (gr < nrows && gk < ktot) x = KCONTIG ? g[gk + gr * ld] : g[gr + gk * ld];
-                    r[v][e] = x;
+                    ok[e] = gr < nrows && gk < ktot;
+                    r[v][e] = g[ok[e] ? (KCONTIG ? gk + gr * ld : gr + gk * ld) : 0];
                 }
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) r[v][e] = ok[e] ? r[v][e] : TIN(0);
             }
         }
     }

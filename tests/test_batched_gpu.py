@@ -53,8 +53,26 @@ def _check_problems(prec, m, n, k, bufs, offs, alpha, beta, c0, which, sample=No
         assert (err <= bound).all(), (prec, p, mp, np_, kp, float((err / bound).max()))
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_ragged_batch_vs_oracle(prec):
+# Tile-class tables: "all64" selects the lean 64x64-only persistent kernel,
+# "all128" the two-class kernel; "default" is the compiled (tuned) table.
+TABLES = {"default": None, "all64": np.zeros((6, 6), np.int32), "all128": np.ones((6, 6), np.int32)}
+
+
+@pytest.fixture
+def table(request):
+    name, prec = request.param
+    if TABLES[name] is not None:
+        batched.set_tuning(TABLES[name], 0, prec)
+    yield name
+    batched.set_tuning(None, 0, prec)
+
+
+def _cases(tables):
+    return [pytest.param(p, (t, p), id="%s-%s" % (p, t)) for p in ("fp32", "bf16") for t in tables]
+
+
+@pytest.mark.parametrize("prec,table", _cases(TABLES), indirect=["table"])
+def test_ragged_batch_vs_oracle(prec, table):
     rng = np.random.default_rng(40)
     count = 96
     m = np.floor(np.exp(rng.uniform(0, np.log(300), count))).astype(np.int64)
@@ -74,8 +92,8 @@ def test_ragged_batch_vs_oracle(prec):
     plan.close()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_config4_4096_problems(prec):
+@pytest.mark.parametrize("prec,table", _cases(("default", "all64")), indirect=["table"])
+def test_config4_4096_problems(prec, table):
     """Config size: 4096 N,N problems, m, n, k log-uniform in [16, 512],
     U[-1,1], alpha=1, beta=0, ld = rows.  Every problem is priced on 8 sampled
     entries by the oracle, every 64th problem in full."""
@@ -119,6 +137,48 @@ def test_graph_capture_replays_match_eager(prec):
         # eager work between replays
         bufs[2].mul_(2.0)
         torch.cuda.synchronize()
+    plan.close()
+
+
+@pytest.mark.parametrize("prec,table", _cases(("all64", "all128")), indirect=["table"])
+def test_padded_leading_dimensions(prec, table):
+    """lda > m, ldb > k, ldc > m with odd paddings (columns off 16-byte
+    alignment); the padding rows of C must stay untouched."""
+    rng = np.random.default_rng(44)
+    count = 40
+    m = rng.integers(1, 200, count)
+    n = rng.integers(1, 200, count)
+    k = rng.integers(1, 200, count)
+    lda, ldb, ldc = m + rng.integers(1, 6, count), k + rng.integers(1, 6, count), m + rng.integers(1, 6, count)
+    g = torch.Generator(device="cuda").manual_seed(45)
+    As = [torch.rand(int(lda[p] * k[p]), device="cuda", generator=g) * 2 - 1 for p in range(count)]
+    Bs = [torch.rand(int(ldb[p] * n[p]), device="cuda", generator=g) * 2 - 1 for p in range(count)]
+    Cs = [torch.rand(int(ldc[p] * n[p]), device="cuda", generator=g) * 2 - 1 for p in range(count)]
+    C0 = [c.cpu().numpy().copy() for c in Cs]
+    ptr = lambda ts: torch.tensor([t.data_ptr() for t in ts], dtype=torch.int64, device="cuda")
+    plan = batched.VBatchPlan(m, n, k, lda=lda, ldb=ldb, ldc=ldc, precision=prec)
+    alpha, beta = 0.75, 1.5
+    plan.exec(alpha, ptr(As), ptr(Bs), beta, ptr(Cs))
+    torch.cuda.synchronize()
+    for p in range(count):
+        mp, np_, kp = int(m[p]), int(n[p]), int(k[p])
+        a = As[p].cpu().numpy().astype(np.float64)
+        b = Bs[p].cpu().numpy().astype(np.float64)
+        want = C0[p].astype(np.float64).copy()
+        oracle.dgemm("N", "N", mp, np_, kp, alpha, a, int(lda[p]), b, int(ldb[p]), beta, want, int(ldc[p]))
+        got = Cs[p].cpu().numpy().astype(np.float64)
+        am = a.reshape(kp, int(lda[p])).T[:mp]
+        bm = b.reshape(np_, int(ldb[p])).T[:kp]
+        absab = np.zeros((int(ldc[p]), np_))
+        absab[:mp] = np.abs(am) @ np.abs(bm)
+        absab = absab.T.ravel()
+        e = E_REP[prec] + (2.0 * kp + 2.0) * H.U32
+        bound = e * abs(alpha) * absab + 4 * H.U32 * np.abs(beta * C0[p].astype(np.float64)) + 1e-30
+        err = np.abs(got - want)
+        assert (err <= bound).all(), (prec, p, mp, np_, kp, float((err / bound).max()))
+        pad = np.ones((int(ldc[p]), np_), bool)
+        pad[:mp] = False
+        assert np.array_equal(got.reshape(np_, int(ldc[p])).T[pad], C0[p].reshape(np_, int(ldc[p])).T[pad])
     plan.close()
 
 
