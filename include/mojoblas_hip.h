@@ -1,5 +1,6 @@
 /*
- * mojoblas_hip: C-ABI of the MI355X (gfx950) level-3 BLAS hot path.
+ * mojoblas_hip: C-ABI of the MI355X (gfx950) level-3 BLAS hot path (and the
+ * level-2 gemv / trsv next to it).
  *
  * Drop-in boundary for mojoBLAS's src/level3 Mojo functions
  * (docs/reference.md:264-327).  A Mojo shim keeps the reference signature
@@ -156,6 +157,37 @@ int mbh_ssymm_dev(char side, char uplo, int64_t m, int64_t n, float alpha, const
 int mbh_dsymm_dev(char side, char uplo, int64_t m, int64_t n, double alpha, const double* a,
                   int64_t lda, const double* b, int64_t ldb, double beta, double* c, int64_t ldc,
                   void* stream);
+
+/* ---- level 2: gemv  y := alpha*op(A)*x + beta*y -------------------------
+ * Replaces gemv[dtype](trans, m, n, alpha, a, lda, x, incx, beta, y, incy),
+ * src/level2/gemv.mojo:23-212.  trans is 'N', 'T' or 'C' (uppercase only:
+ * the reference compares against "N"/"T"/"C", gemv.mojo:77); info 1,2,3,6,8,11
+ * (gemv.mojo:76-91).  x and y are strided (incx, incy != 0; negative
+ * increments walk the vector from its far end).  Deterministic fixed-order
+ * sums; the arithmetic is that of the data type.  beta == 0 does not read y. */
+int mbh_sgemv_host(char trans, int64_t m, int64_t n, float alpha, const float* a, int64_t lda,
+                   const float* x, int64_t incx, float beta, float* y, int64_t incy);
+int mbh_dgemv_host(char trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+                   const double* x, int64_t incx, double beta, double* y, int64_t incy);
+int mbh_sgemv_dev(char trans, int64_t m, int64_t n, float alpha, const float* a, int64_t lda,
+                  const float* x, int64_t incx, float beta, float* y, int64_t incy, void* stream);
+int mbh_dgemv_dev(char trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+                  const double* x, int64_t incx, double beta, double* y, int64_t incy, void* stream);
+
+/* ---- level 2: trsv  x := inv(op(A))*x, A triangular -----------------------
+ * Replaces trsv[dtype](uplo, trans, diag, n, a, lda, x, incx),
+ * src/level2/trsv.mojo:20-214; info 1,2,3,4,6,8 (trsv.mojo:57-80).  Runs as
+ * the n x 1 trsm (fp64 diagonal-block inverses; fp32 data gets one
+ * fp64-residual refinement step); `precision` selects the gemm of the updates
+ * (device entry; the host entry uses the library default). */
+int mbh_strsv_host(char uplo, char trans, char diag, int64_t n, const float* a, int64_t lda,
+                   float* x, int64_t incx);
+int mbh_dtrsv_host(char uplo, char trans, char diag, int64_t n, const double* a, int64_t lda,
+                   double* x, int64_t incx);
+int mbh_strsv_dev(char uplo, char trans, char diag, int64_t n, const float* a, int64_t lda,
+                  float* x, int64_t incx, int precision, void* stream);
+int mbh_dtrsv_dev(char uplo, char trans, char diag, int64_t n, const double* a, int64_t lda,
+                  double* x, int64_t incx, void* stream);
 
 /* ---- variable-size batched gemm (no counterpart in the reference) --------
  * count independent problems C_p := alpha*A_p*B_p + beta*C_p (each with the

@@ -1,7 +1,7 @@
 """mojoblas_amd: MI355X (gfx950) level-3 BLAS hot path behind mojoBLAS's API.
 
 The functions here mirror the reference's src/level3 Mojo functions
-(docs/reference.md:264-327): same names, same Netlib argument order and
+(docs/reference.md:264-327), and its level-2 gemv / trsv (src/level2): same names, same Netlib argument order and
 meaning, flags as strings compared whole ("N"/"n"/"T"/"t"/"C"/"c", ...), and
 the same error behaviour -- an invalid argument prints "<name>: Info <k>"
 and returns without touching any output (gemm.mojo:211-213).  They return
@@ -204,6 +204,22 @@ def symm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, precision=None, 
     return _call("symm", [_flag(side), _flag(uplo), m, n, alpha],
                  [_p(a), lda, _p(b), ldb, beta, _p(c), ldc], (a, b, c), precision, stream,
                  dev_extra_float=[_prec(precision)])
+
+
+def gemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, stream=None):
+    """y := alpha*op(A)*x + beta*y (src/level2/gemv.mojo:23-212): trans is
+    "N", "T" or "C" (uppercase only, as the reference compares them), x and y
+    strided by incx / incy (negative increments as in BLAS)."""
+    return _call("gemv", [_flag(trans), m, n, alpha], [_p(a), lda, _p(x), incx, beta, _p(y), incy], (a, x, y),
+                 None, stream)
+
+
+def trsv(uplo, trans, diag, n, a, lda, x, incx, precision=None, stream=None):
+    """x := inv(op(A))*x, A triangular (src/level2/trsv.mojo:20-214); runs as
+    the n x 1 trsm (fp64 diagonal blocks, one fp64-residual refinement step for
+    float32); `precision` selects the gemm of the updates for float32."""
+    return _call("trsv", [_flag(uplo), _flag(trans), _flag(diag), n], [_p(a), lda, _p(x), incx], (a, x),
+                 precision, stream, dev_extra_float=[_prec(precision)])
 
 
 # The reference exports its gemm generations under their own names

@@ -16,6 +16,9 @@
   l3     the level-3 routines without a sized workload of their own: ssymm
          L,L m=n=8192, strmm L,L,N,N m=n=4096, ssyr2k L,N n=8192 k=2048
   dgemm  DGEMM N,N 8192^3 on the fp64 MFMA (alpha=1, beta=0)
+  l2     level-2: sgemv N and T at 16384 x 16384 (1 GiB matrix) and dgemv N at
+         8192 x 8192 (HBM-bound: algorithmic bytes / time against 8 TB/s);
+         strsv L,N,N at n = 4096 and 16384, dtrsv at 4096
 
     python -m mojoblas_amd.bench_configs [--only cfg4,cfg3a] [--reps 10]
 
@@ -322,8 +325,47 @@ def dgemm(torch, mb, reps):
           "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK["fp64"], "diff_vs_torch_fp64": e})
 
 
+def l2(torch, mb, reps):
+    g = torch.Generator(device="cuda").manual_seed(10)
+    for dt, n in ((torch.float32, 16384), (torch.float64, 8192)):
+        m = n
+        es = 4 if dt == torch.float32 else 8
+        A = torch.randn(m * n, device="cuda", generator=g, dtype=dt)
+        x = torch.randn(max(m, n), device="cuda", generator=g, dtype=dt)
+        y = torch.empty(max(m, n), device="cuda", dtype=dt)
+        A64 = A.view(n, m).t().double()
+        for trans in (("N", "T") if dt == torch.float32 else ("N",)):
+            ms = timed(torch, lambda: mb.gemv(trans, m, n, 1.0, A, m, x, 1, 0.0, y, 1), reps)
+            ref = (A64 if trans == "N" else A64.t()) @ x.double()[: (n if trans == "N" else m)]
+            e = err_stats(torch, y[: (m if trans == "N" else n)].double(), ref)
+            byt = es * (m * n + m + n)
+            emit({"config": "l2", "routine": "%sgemv %s %dx%d" % ("s" if es == 4 else "d", trans, m, n),
+                  "ms": ms, "gbs": byt / ms / 1e6, "roofline_frac": byt / ms / 1e6 / (HBM / 1e9),
+                  "bytes": byt, "error_vs_fp64": e})
+        del A, A64
+    for dt, n in ((torch.float32, 4096), (torch.float32, 16384), (torch.float64, 4096)):
+        a = (torch.rand(n, n, device="cuda", generator=g, dtype=torch.float64) * 2 - 1) / n
+        a = torch.tril(a, -1) + torch.diag(1 + torch.rand(n, device="cuda", generator=g, dtype=torch.float64))
+        A = a.t().contiguous().to(dt).reshape(-1)
+        b = torch.rand(n, device="cuda", generator=g, dtype=dt) * 2 - 1
+        x = torch.empty_like(b)
+        ref = torch.linalg.solve_triangular(A.view(n, n).t().double(), b.double().view(n, 1), upper=False).view(-1)
+
+        def run():
+            x.copy_(b)
+            mb.trsv("L", "N", "N", n, A, n, x, 1)
+        ms = timed(torch, run, reps)
+        run()
+        e = err_stats(torch, x.double(), ref)
+        es = 4 if dt == torch.float32 else 8
+        byt = es * (n * (n + 1) // 2 + 2 * n)
+        emit({"config": "l2", "routine": "%strsv L,N,N n=%d" % ("s" if es == 4 else "d", n), "ms": ms,
+              "gbs": byt / ms / 1e6, "roofline_frac": byt / ms / 1e6 / (HBM / 1e9), "error_vs_fp64": e})
+        del a, A
+
+
 CONFIGS = {"cfg1": cfg1, "cfg2b": cfg2b, "cfg2h": cfg2h, "cfg3a": cfg3a, "cfg3b": cfg3b, "cfg4": cfg4, "cfg5": cfg5,
-           "l3": l3, "dgemm": dgemm}
+           "l3": l3, "dgemm": dgemm, "l2": l2}
 
 
 def main():

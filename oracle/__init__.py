@@ -43,6 +43,8 @@ def lib():
         L.orc_dsyrk.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _dbl, _pd, _i64]
         L.orc_dsymm.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
         L.orc_dsyr2k.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
+        L.orc_dgemv.argtypes = [_cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
+        L.orc_dtrsv.argtypes = [_cs, _cs, _cs, _i64, _pd, _i64, _pd, _i64]
         for f in (L.orc_dgemm, L.orc_dtrsm, L.orc_dtrmm, L.orc_dsyrk, L.orc_dsymm, L.orc_dsyr2k):
             f.restype = ctypes.c_int
         L.orc_sgemm_sample_f64.argtypes = [
@@ -113,7 +115,16 @@ def dsyr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     return lib().orc_dsyr2k(_s(uplo), _s(trans), n, k, alpha, _d(a), lda, _d(b), ldb, beta, _d(c), ldc)
 
 
-ROUTINES = {"gemm": dgemm, "trsm": dtrsm, "trmm": dtrmm, "syrk": dsyrk, "symm": dsymm, "syr2k": dsyr2k}
+def dgemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+    return lib().orc_dgemv(_s(trans), m, n, alpha, _d(a), lda, _d(x), incx, beta, _d(y), incy)
+
+
+def dtrsv(uplo, trans, diag, n, a, lda, x, incx):
+    return lib().orc_dtrsv(_s(uplo), _s(trans), _s(diag), n, _d(a), lda, _d(x), incx)
+
+
+ROUTINES = {"gemm": dgemm, "trsm": dtrsm, "trmm": dtrmm, "syrk": dsyrk, "symm": dsymm, "syr2k": dsyr2k,
+            "gemv": dgemv, "trsv": dtrsv}
 
 
 # ---- sampled fp64 pricing of full-size fp32 results ----

@@ -286,6 +286,80 @@ int orc_dsyr2k(const char* uplo, const char* trans, int64_t n, int64_t k, double
 #undef B_
 #undef C_
 
+/* ---- level 2 ------------------------------------------------------------ */
+
+/* Logical element i of a strided vector of length len (BLAS convention). */
+static inline int64_t vidx(int64_t i, int64_t len, int64_t inc) { return inc > 0 ? i * inc : (len - 1 - i) * (-inc); }
+
+int orc_dgemv(const char* trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+              const double* x, int64_t incx, double beta, double* y, int64_t incy) {
+    /* gemv.mojo:76-91: trans compared against "N", "T", "C" only */
+    const int tn = eq(trans, "N"), tt = eq(trans, "T") || eq(trans, "C");
+    int info = 0;
+    if (!tn && !tt) info = 1;
+    else if (m < 0) info = 2;
+    else if (n < 0) info = 3;
+    else if (lda < max1(m)) info = 6;
+    else if (incx == 0) info = 8;
+    else if (incy == 0) info = 11;
+    if (info) return info;
+    if (m == 0 || n == 0 || (alpha == 0.0 && beta == 1.0)) return 0;
+    const int64_t lenx = tn ? n : m, leny = tn ? m : n;
+    for (int64_t i = 0; i < leny; ++i) {
+        double* yi = y + vidx(i, leny, incy);
+        *yi = (beta == 0.0) ? 0.0 : beta * *yi;
+    }
+    if (alpha == 0.0) return 0;
+    if (tn) {
+        for (int64_t j = 0; j < n; ++j) {
+            const double t = alpha * x[vidx(j, lenx, incx)];
+            for (int64_t i = 0; i < m; ++i) y[vidx(i, leny, incy)] += t * a[i + j * lda];
+        }
+    } else {
+        for (int64_t j = 0; j < n; ++j) {
+            double t = 0.0;
+            for (int64_t i = 0; i < m; ++i) t += a[i + j * lda] * x[vidx(i, lenx, incx)];
+            y[vidx(j, leny, incy)] += alpha * t;
+        }
+    }
+    return 0;
+}
+
+int orc_dtrsv(const char* uplo, const char* trans, const char* diag, int64_t n, const double* a,
+              int64_t lda, double* x, int64_t incx) {
+    int info = 0;
+    if (!flag_u(uplo) && !flag_l(uplo)) info = 1;
+    else if (!flag_n(trans) && !flag_t(trans)) info = 2;
+    else if (!flag_u(diag) && !flag_n(diag)) info = 3;
+    else if (n < 0) info = 4;
+    else if (lda < max1(n)) info = 6;
+    else if (incx == 0) info = 8;
+    if (info) return info;
+    if (n == 0) return 0;
+    const int upper = flag_u(uplo), unit = flag_u(diag), tr = flag_t(trans);
+    /* op(A) is lower iff (lower and 'N') or (upper and 'T'); solve op(A) x = b by
+     * forward (lower) or backward (upper) substitution on op(A)(i,j). */
+    const int eff_lower = (!upper) != tr;
+#define OPA(i, j) (tr ? a[(j) + (i) * lda] : a[(i) + (j) * lda])
+#define X(i) x[vidx((i), n, incx)]
+    if (eff_lower) {
+        for (int64_t i = 0; i < n; ++i) {
+            double t = X(i);
+            for (int64_t j = 0; j < i; ++j) t -= OPA(i, j) * X(j);
+            X(i) = unit ? t : t / OPA(i, i);
+        }
+    } else {
+        for (int64_t i = n - 1; i >= 0; --i) {
+            double t = X(i);
+            for (int64_t j = i + 1; j < n; ++j) t -= OPA(i, j) * X(j);
+            X(i) = unit ? t : t / OPA(i, i);
+        }
+    }
+#undef OPA
+#undef X
+    return 0;
+}
+
 static inline double opa_f(char ta, const float* a, int64_t lda, int64_t i, int64_t l) {
     return (ta == 'N' || ta == 'n') ? (double)a[i + l * lda] : (double)a[l + i * lda];
 }

@@ -1,6 +1,6 @@
 /*
  * FP64 CPU oracle for the level-3 hot path (gemm x4 transposes, trsm, trmm,
- * syrk, symm, syr2k).
+ * syrk, symm, syr2k) and the level-2 gemv / trsv built next to it.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
@@ -15,6 +15,8 @@
  *   syrk  src/level3/syrk.mojo:20-200
  *   symm  src/level3/symm.mojo:20-241
  *   syr2k src/level3/syr2k.mojo:20-190
+ *   gemv  src/level2/gemv.mojo:23-212   (trans is "N"/"T"/"C" only, :77)
+ *   trsv  src/level2/trsv.mojo:20-214
  * Flags are compared as whole strings exactly as the reference does
  * ("N"/"n"/"T"/"t"/"C"/"c", gemm.mojo:180-197), so "No" is info 1.
  * Deliberate divergences from the reference's arithmetic (documented where
@@ -56,6 +58,20 @@ int orc_dsymm(const char* side, const char* uplo, int64_t m, int64_t n, double a
 int orc_dsyr2k(const char* uplo, const char* trans, int64_t n, int64_t k, double alpha,
                const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                double* c, int64_t ldc);
+
+/* y := alpha*op(A)*x + beta*y, A m x n, strided x / y (incx, incy != 0;
+ * negative increments walk the vector from its far end, as in BLAS).  Info
+ * numbering of gemv.mojo:76-91.  Divergence: no zero-skip on x(j)
+ * (gemv.mojo:136-141). */
+int orc_dgemv(const char* trans, int64_t m, int64_t n, double alpha, const double* a, int64_t lda,
+              const double* x, int64_t incx, double beta, double* y, int64_t incy);
+
+/* x := op(A)^-1 x, A n x n triangular.  Info numbering of trsv.mojo:57-80.
+ * Divergences: no zero-skip on x(j) (trsv.mojo:95,120), and for upper, 'N',
+ * incx != 1 element j of x is x((j)*incx) as in BLAS (trsv.mojo:108-111
+ * decrements the index before its first use, reading one stride off). */
+int orc_dtrsv(const char* uplo, const char* trans, const char* diag, int64_t n, const double* a,
+              int64_t lda, double* x, int64_t incx);
 
 /* Sampled FP64 gemm: out[s] = alpha*sum_l op(A)(rows[s],l)*op(B)(l,cols[s]) + beta*c0[s]
  * (c0 may be NULL when beta == 0).  Inputs are fp32 (the hot path's data type);

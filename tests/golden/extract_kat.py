@@ -1,20 +1,24 @@
-"""Extract the level-3 known-answer tests of mojoBLAS into a JSON fixture.
+"""Extract the known-answer tests of mojoBLAS's level-3 routines and of
+its level-2 gemv / trsv into JSON fixtures.
 
-Reads the reference's ``tests/test_level3.mojo`` as text (it is never run)
-and records, for each ``def test_*``: the routine called, its arguments, the
-input arrays as the test fills them, and every ``assert_almost_equal``
-(array, index, expected value, atol, source line).  The output
-``level3_kat.json`` is data: inputs and expected outputs.
+Reads the reference's ``tests/test_level3.mojo``, ``tests/test_level2.mojo``
+and ``tests/test_level2_extended.mojo`` as text (they are never run) and
+records, for each ``def test_*`` that calls one of the routines: the routine
+called, its arguments, the input arrays as the test fills them, and every
+``assert_almost_equal`` (array, index, expected value, atol, source line).
+The outputs ``level3_kat.json`` and ``level2_kat.json`` are data: inputs and
+expected outputs.
 
 Usage (in a checkout where the reference is mounted):
-    python tests/golden/extract_kat.py /root/reference/tests/test_level3.mojo
+    python tests/golden/extract_kat.py /root/reference/tests
 """
 import json
 import os
 import re
 import sys
 
-ROUTINES = ("gemm", "syrk", "syr2k", "symm", "trmm", "trsm")
+LEVEL3 = ("gemm", "syrk", "syr2k", "symm", "trmm", "trsm")
+LEVEL2 = ("gemv", "trsv")
 # Mojo std.testing.assert_almost_equal defaults for floating point.
 DEFAULT_ATOL = 1e-8
 DEFAULT_RTOL = 1e-5
@@ -41,7 +45,9 @@ def _split_args(s):
     return out
 
 
-def parse(path):
+def parse(path, routines=LEVEL3):
+    """Tests of `path` that call one of `routines`; others are skipped."""
+    ROUTINES = routines
     lines = open(path).read().split("\n")
     tests, i = [], 0
     while i < len(lines):
@@ -50,6 +56,13 @@ def parse(path):
             i += 1
             continue
         name, def_line = m.group(1), i + 1
+        end = i + 1
+        while end < len(lines) and not lines[end].startswith("def "):
+            end += 1
+        body = "\n".join(lines[i + 1:end])
+        if not re.search(r"^\s*(%s)(\[DType\.\w+\])?\(" % "|".join(ROUTINES), body, re.M):
+            i = end
+            continue
         env, arrays, dtype = {}, {}, None
         call, expects = None, []
         i += 1
@@ -129,12 +142,10 @@ def parse(path):
     return tests
 
 
-def main():
-    src = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/tests/test_level3.mojo"
-    tests = parse(src)
-    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "level3_kat.json")
+def write(tests, sources, name):
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), name)
     doc = {
-        "source": "tests/test_level3.mojo (mojoBLAS)",
+        "source": sources,
         "assert_defaults": {"atol": DEFAULT_ATOL, "rtol": DEFAULT_RTOL},
         "count": len(tests),
         "tests": tests,
@@ -142,6 +153,17 @@ def main():
     with open(out, "w") as f:
         json.dump(doc, f, indent=1)
     print("wrote %d tests to %s" % (len(tests), out))
+
+
+def main():
+    src = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/tests"
+    write(parse(os.path.join(src, "test_level3.mojo")), "tests/test_level3.mojo (mojoBLAS)", "level3_kat.json")
+    l2 = []
+    for fn in ("test_level2.mojo", "test_level2_extended.mojo"):
+        for t in parse(os.path.join(src, fn), LEVEL2):
+            t["file"] = "tests/" + fn
+            l2.append(t)
+    write(l2, "tests/test_level2.mojo, tests/test_level2_extended.mojo (mojoBLAS)", "level2_kat.json")
 
 
 if __name__ == "__main__":

@@ -110,3 +110,62 @@ def test_batched_plan_validates_every_problem_without_gpu():
         batched.VBatchPlan([4], [4], [4], precision="bf16x3")
     with pytest.raises(mb.HipBlasError, match="classes are 0 and 1"):
         batched.set_tuning([2] * 36)
+
+
+# (args, info) for gemv, gemv.mojo:76-91 (trans uppercase only).
+GEMV_INFO = [
+    ((b"n", 2, 2, 2, 1, 1), 1),
+    ((b"\x00", 2, 2, 2, 1, 1), 1),
+    ((b"N", -1, 2, 2, 1, 1), 2),
+    ((b"T", 2, -1, 2, 1, 1), 3),
+    ((b"C", 3, 2, 2, 1, 1), 6),
+    ((b"N", 2, 2, 2, 0, 1), 8),
+    ((b"N", 2, 2, 2, -1, 0), 11),
+]
+
+
+@pytest.mark.parametrize("args,info", GEMV_INFO)
+@pytest.mark.parametrize("entry", ["host_s", "host_d", "dev_s", "dev_d"])
+def test_gemv_info_codes_without_gpu(args, info, entry):
+    L = _lib.lib()
+    trans, m, n, lda, incx, incy = args
+    dt = np.float32 if entry.endswith("s") else np.float64
+    a, x, y = np.full(16, 7, dt), np.full(16, 7, dt), np.full(16, 7, dt)
+    P = lambda v: ctypes.c_void_p(v.ctypes.data)
+    fn = getattr(L, "mbh_%sgemv_%s" % (entry[-1], entry[:-2]))
+    extra = [None] if entry.startswith("dev") else []
+    assert fn(trans, m, n, 1.0, P(a), lda, P(x), incx, 0.0, P(y), incy, *extra) == info
+    assert (y == 7).all()
+
+
+# (args, info) for trsv, trsv.mojo:57-80 (both cases accepted).
+TRSV_INFO = [
+    ((b"X", b"N", b"N", 2, 2, 1), 1),
+    ((b"u", b"X", b"n", 2, 2, 1), 2),
+    ((b"U", b"t", b"X", 2, 2, 1), 3),
+    ((b"L", b"N", b"N", -1, 2, 1), 4),
+    ((b"l", b"c", b"u", 3, 2, 1), 6),
+    ((b"U", b"N", b"N", 2, 2, 0), 8),
+]
+
+
+@pytest.mark.parametrize("args,info", TRSV_INFO)
+@pytest.mark.parametrize("entry", ["host_s", "host_d", "dev_s", "dev_d"])
+def test_trsv_info_codes_without_gpu(args, info, entry):
+    L = _lib.lib()
+    uplo, trans, diag, n, lda, incx = args
+    dt = np.float32 if entry.endswith("s") else np.float64
+    a, x = np.full(16, 7, dt), np.full(16, 7, dt)
+    P = lambda v: ctypes.c_void_p(v.ctypes.data)
+    fn = getattr(L, "mbh_%strsv_%s" % (entry[-1], entry[:-2]))
+    extra = ([0] if entry == "dev_s" else []) + ([None] if entry.startswith("dev") else [])
+    assert fn(uplo, trans, diag, n, P(a), lda, P(x), incx, *extra) == info
+    assert (x == 7).all()
+
+
+def test_level2_python_mirror_prints_info(capsys):
+    a = np.zeros(4, np.float32)
+    assert mb.gemv("n", 2, 2, 1.0, a, 2, a, 1, 0.0, a.copy(), 1) == 1
+    assert "gemv: Info 1" in capsys.readouterr().out
+    assert mb.trsv("U", "N", "N", 2, a, 1, a.copy(), 1) == 6
+    assert "trsv: Info 6" in capsys.readouterr().out
