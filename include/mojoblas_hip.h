@@ -176,16 +176,17 @@ int mbh_dgemv_dev(char trans, int64_t m, int64_t n, double alpha, const double* 
 
 /* ---- level 2: trsv  x := inv(op(A))*x, A triangular -----------------------
  * Replaces trsv[dtype](uplo, trans, diag, n, a, lda, x, incx),
- * src/level2/trsv.mojo:20-214; info 1,2,3,4,6,8 (trsv.mojo:57-80).  Runs as
- * the n x 1 trsm (fp64 diagonal-block inverses; fp32 data gets one
- * fp64-residual refinement step); `precision` selects the gemm of the updates
- * (device entry; the host entry uses the library default). */
+ * src/level2/trsv.mojo:20-214; info 1,2,3,4,6,8 (trsv.mojo:57-80).  One
+ * kernel: blocked substitution over 128-row blocks with fp64 residual
+ * accumulation and fp64 diagonal-block inverses (every arithmetic step in
+ * fp64, results rounded to the data type).  The host entry returns
+ * MBH_E_HIP if a block's bounded wait for an earlier block ran out. */
 int mbh_strsv_host(char uplo, char trans, char diag, int64_t n, const float* a, int64_t lda,
                    float* x, int64_t incx);
 int mbh_dtrsv_host(char uplo, char trans, char diag, int64_t n, const double* a, int64_t lda,
                    double* x, int64_t incx);
 int mbh_strsv_dev(char uplo, char trans, char diag, int64_t n, const float* a, int64_t lda,
-                  float* x, int64_t incx, int precision, void* stream);
+                  float* x, int64_t incx, void* stream);
 int mbh_dtrsv_dev(char uplo, char trans, char diag, int64_t n, const double* a, int64_t lda,
                   double* x, int64_t incx, void* stream);
 

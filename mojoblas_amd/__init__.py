@@ -214,12 +214,12 @@ def gemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, stream=None):
                  None, stream)
 
 
-def trsv(uplo, trans, diag, n, a, lda, x, incx, precision=None, stream=None):
-    """x := inv(op(A))*x, A triangular (src/level2/trsv.mojo:20-214); runs as
-    the n x 1 trsm (fp64 diagonal blocks, one fp64-residual refinement step for
-    float32); `precision` selects the gemm of the updates for float32."""
+def trsv(uplo, trans, diag, n, a, lda, x, incx, stream=None):
+    """x := inv(op(A))*x, A triangular (src/level2/trsv.mojo:20-214): one
+    blocked-substitution kernel with fp64 residuals and fp64 diagonal-block
+    inverses."""
     return _call("trsv", [_flag(uplo), _flag(trans), _flag(diag), n], [_p(a), lda, _p(x), incx], (a, x),
-                 precision, stream, dev_extra_float=[_prec(precision)])
+                 None, stream)
 
 
 # The reference exports its gemm generations under their own names

@@ -60,7 +60,7 @@ SIGNATURES = {
     "mbh_dgemv_dev": ([ch, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp], i32),
     "mbh_strsv_host": ([ch, ch, ch, i64, vp, i64, vp, i64], i32),
     "mbh_dtrsv_host": ([ch, ch, ch, i64, vp, i64, vp, i64], i32),
-    "mbh_strsv_dev": ([ch, ch, ch, i64, vp, i64, vp, i64, i32, vp], i32),
+    "mbh_strsv_dev": ([ch, ch, ch, i64, vp, i64, vp, i64, vp], i32),
     "mbh_dtrsv_dev": ([ch, ch, ch, i64, vp, i64, vp, i64, vp], i32),
     "mbh_vbatch_plan_create": ([i64, vp, vp, vp, vp, vp, vp, i32, vp], i32),
     "mbh_sgemm_vbatched_exec": ([vp, f32, vp, vp, f32, vp, vp], i32),

@@ -13,9 +13,13 @@
 //             of rows; lanes stride the rows (coalesced), x(i) is read once for
 //             the four columns, each column is reduced across the wave by
 //             shuffles; row chunks give the workgroup count, reduced as above.
-//   trsv      x := op(A)^-1 x, the n x 1 case of the blocked trsm (fp64
-//             diagonal-block inverses, gemm updates, one fp64-residual
-//             refinement for fp32 data), on a contiguous copy when incx != 1.
+//   trsv      x := op(A)^-1 x in one kernel: blocked substitution with
+//             decoupled look-back.  One workgroup per 128-row block, in solve
+//             order by an atomic ticket; it streams its off-diagonal blocks of
+//             op(A) while earlier blocks are solved, waits on each earlier
+//             block's flag, accumulates the residual in fp64, applies the fp64
+//             inverse of its diagonal block and publishes its part of x and its
+//             flag.  On a contiguous copy of x when incx != 1.
 // Strided vectors follow BLAS: for inc < 0 logical element i is stored at
 // (len-1-i)*|inc|.  The results are deterministic (fixed summation order).
 #include <algorithm>
@@ -154,6 +158,118 @@ __global__ void vec_copy_kernel(int64_t len, const T* src, int64_t incs, T* dst,
         dst[vidx(i, len, incd)] = src[vidx(i, len, incs)];
 }
 
+// ---- trsv ---------------------------------------------------------------------
+// A workgroup waits only on blocks with smaller tickets, which were taken by
+// workgroups that had already started, so the waits cannot deadlock.  Blocks
+// of x and the flags are written and read with device-scope (agent) atomics:
+// the XCDs' L2 caches are not coherent with each other for plain accesses.
+// The wait is bounded (TV_SPIN polls, and no longer once any workgroup has
+// given up); a workgroup that gives up sets *fail and finishes with what it
+// has, so the kernel always terminates.
+constexpr int TV_NB = TRI_NB;  // rows per block (the fp64 inverses' blocks)
+constexpr int TV_NT = 256;     // thread t: row t % 128, column half t / 128
+constexpr int TV_H = TV_NB / 2;
+constexpr int TV_SPIN = 1 << 22;  // about 0.2 s of 128-cycle sleeps
+
+__global__ void trsv_reset_kernel(int* w, int count) {
+    for (int i = threadIdx.x; i < count; i += blockDim.x) w[i] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TV_NT) void trsv_lookback_kernel(int64_t n, const T* __restrict__ A, int64_t lda,
+                                                              int eff_lower, int trans, int avec,
+                                                              const double* __restrict__ inv, T* x, int* flags,
+                                                              int* ticket, int nblk, int* fail) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
+    __shared__ double xs[TV_NB];
+    __shared__ double red[2][TV_NB];
+    __shared__ int s_sh;
+    const int tid = threadIdx.x, row = tid & (TV_NB - 1), h = tid >> 7;
+    if (tid == 0) s_sh = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int s = s_sh;  // solve-order index (grid = nblk, so s < nblk)
+    const int b = eff_lower ? s : nblk - 1 - s;
+    const int64_t r0 = (int64_t)b * TV_NB;
+    const int nb = (n - r0 < TV_NB) ? (int)(n - r0) : TV_NB;
+    const bool rok = row < nb;
+    // This thread's row of the diagonal block's inverse, its half of the k's.
+    const double* ib = inv + (size_t)b * TV_NB * TV_NB;
+    double iv[TV_H];
+#pragma unroll
+    for (int q = 0; q < TV_H; ++q) iv[q] = ib[row + (size_t)(h * TV_H + q) * TV_NB];
+    const MBH_GLOBAL T* Ag = (const MBH_GLOBAL T*)A;
+    double acc = 0.0;
+    for (int sp = 0; sp < s; ++sp) {
+        const int c = eff_lower ? sp : nblk - 1 - sp;
+        const int64_t c0 = (int64_t)c * TV_NB;
+        const int nc = (n - c0 < TV_NB) ? (int)(n - c0) : TV_NB;
+        // av[q] = op(A)(r0 + row, c0 + h*64 + q), masked loads without
+        // branches, issued before the wait so that they overlap it
+        T av[TV_H];
+        if (!trans) {  // A[(r0+row) + (c0+j)*lda]: the lanes read consecutive rows
+#pragma unroll
+            for (int q = 0; q < TV_H; ++q) {
+                const int j = h * TV_H + q;
+                const bool ok = rok && j < nc;
+                av[q] = Ag[ok ? (r0 + row) + (c0 + j) * lda : 0];
+                av[q] = ok ? av[q] : T(0);
+            }
+        } else if (avec && nc == TV_NB) {  // A[(c0+j) + (r0+row)*lda]: a lane reads its row in 16-byte vectors
+#pragma unroll
+            for (int q = 0; q < TV_H; q += VEC) {
+                const vec_t v = *(const MBH_GLOBAL vec_t*)(Ag + (rok ? (c0 + h * TV_H + q) + (r0 + row) * lda : 0));
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) av[q + e] = rok ? v[e] : T(0);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < TV_H; ++q) {
+                const int j = h * TV_H + q;
+                const bool ok = rok && j < nc;
+                av[q] = Ag[ok ? (c0 + j) + (r0 + row) * lda : 0];
+                av[q] = ok ? av[q] : T(0);
+            }
+        }
+        if (tid == 0) {
+            int polls = 0;
+            while (__hip_atomic_load(flags + c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                // give up past the limit, or at once when another block has
+                if (++polls > TV_SPIN ||
+                    ((polls & 255) == 0 && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    atomicOr(fail, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        if (tid < TV_NB)
+            xs[tid] = tid < nc ? (double)__hip_atomic_load(x + c0 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TV_H; ++q) acc += (double)av[q] * xs[h * TV_H + q];
+        __syncthreads();  // xs is refilled by the next predecessor
+    }
+    red[h][row] = acc;
+    __syncthreads();
+    // residual of the block (its right-hand side is still in x: only this
+    // workgroup writes these entries), then x_b = inv_b * residual
+    if (tid < TV_NB) xs[tid] = tid < nb ? (double)x[r0 + tid] - (red[0][tid] + red[1][tid]) : 0.0;
+    __syncthreads();
+    double yv = 0.0;
+#pragma unroll
+    for (int q = 0; q < TV_H; ++q) yv += iv[q] * xs[h * TV_H + q];
+    red[h][row] = yv;
+    __syncthreads();
+    if (tid < nb)
+        __hip_atomic_store(x + r0 + tid, (T)(red[0][tid] + red[1][tid]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flags + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 dim3 grid1(int64_t len) {
     int64_t b = (len + 255) / 256;
     return dim3((unsigned)(b < 1 ? 1 : (b > 16384 ? 16384 : b)));
@@ -200,21 +316,39 @@ hipError_t gemv(bool trans, int64_t m, int64_t n, T alpha, const T* A, int64_t l
     return hipGetLastError();
 }
 
+// Workspace (slot 3): fp64 block inverses | flags, ticket, fail | contiguous x.
 template <typename T>
-hipError_t trsv(char uplo, char trans, char diag, int64_t n, const T* A, int64_t lda, T* x, int64_t incx, int prec,
-                hipStream_t s) {
+hipError_t trsv(char uplo, char trans, char diag, int64_t n, const T* A, int64_t lda, T* x, int64_t incx,
+                hipStream_t s, int** fail_out = nullptr) {
     if (n <= 0) return hipSuccess;
-    T* xc = x;
+    const bool upper = flag_u(uplo), tr = !flag_n(trans), unit = flag_u(diag);
+    const bool eff_lower = (!upper) != tr;
+    const int64_t nblk = (n + TV_NB - 1) / TV_NB;
+    if (nblk > INT_MAX - 3) return hipErrorInvalidValue;
+    auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t inv_b = (size_t)nblk * TV_NB * TV_NB * sizeof(double), fl_b = (size_t)(nblk + 3) * sizeof(int);
+    const size_t xc_b = incx != 1 ? (size_t)n * sizeof(T) : 0;
+    void* ws = nullptr;
+    hipError_t e = workspace(s, 3, al(inv_b) + al(fl_b) + al(xc_b), &ws);
+    if (e != hipSuccess) return e;
+    double* inv = (double*)ws;
+    int* flags = (int*)((char*)ws + al(inv_b));
+    int* ticket = flags + nblk;
+    int* fail = ticket + 1;
+    T* xc = incx != 1 ? (T*)((char*)ws + al(inv_b) + al(fl_b)) : x;
+    trsv_reset_kernel<<<1, 256, 0, s>>>(flags, (int)nblk + 3);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = tri_block_inverses<T>(n, A, lda, eff_lower, tr, unit, inv, s)) != hipSuccess) return e;
     if (incx != 1) {
-        void* ws = nullptr;
-        hipError_t e = workspace(s, 3, (size_t)n * sizeof(T), &ws);
-        if (e != hipSuccess) return e;
-        xc = (T*)ws;
         vec_copy_kernel<T><<<grid1(n), 256, 0, s>>>(n, x, incx, xc, 1);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipError_t e = trsm<T>('L', uplo, trans, diag, n, 1, T(1), A, lda, xc, n, prec, sizeof(T) == 4 ? 1 : 0, s);
-    if (e != hipSuccess || incx == 1) return e;
+    const int avec = ((uintptr_t)A % 16 == 0) && (lda % (16 / (int64_t)sizeof(T)) == 0);
+    trsv_lookback_kernel<T><<<dim3((unsigned)nblk), TV_NT, 0, s>>>(n, A, lda, eff_lower ? 1 : 0, tr ? 1 : 0, avec, inv,
+                                                                  xc, flags, ticket, (int)nblk, fail);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (fail_out) *fail_out = fail;
+    if (incx == 1) return hipSuccess;
     vec_copy_kernel<T><<<grid1(n), 256, 0, s>>>(n, xc, 1, x, incx);
     return hipGetLastError();
 }
@@ -283,15 +417,10 @@ int gemv_host(char trans, int64_t m, int64_t n, T alpha, const T* a, int64_t lda
 }
 
 template <typename T>
-int trsv_dev(char uplo, char trans, char diag, int64_t n, const T* a, int64_t lda, T* x, int64_t incx, int prec,
-             void* stream) {
+int trsv_dev(char uplo, char trans, char diag, int64_t n, const T* a, int64_t lda, T* x, int64_t incx, void* stream) {
     int info = check_trsv(uplo, trans, diag, n, lda, incx);
     if (info) return info;
-    if (sizeof(T) == 4 && !gemm_precision_ok(prec)) {
-        set_error("trsv: invalid precision %d", prec);
-        return MBH_E_PRECISION;
-    }
-    return finish(trsv<T>(uplo, trans, diag, n, a, lda, x, incx, prec, (hipStream_t)stream), "trsv");
+    return finish(trsv<T>(uplo, trans, diag, n, a, lda, x, incx, (hipStream_t)stream), "trsv");
 }
 
 template <typename T>
@@ -307,10 +436,18 @@ int trsv_host(char uplo, char trans, char diag, int64_t n, const T* a, int64_t l
     if ((e = put_matrix(dA, a, n, n, lda, sizeof(T))) != hipSuccess) return hip_fail(e, "trsv: copy A");
     if ((e = slot_get(1, (size_t)sx * sizeof(T), &dx)) != hipSuccess) return hip_fail(e, "trsv: x");
     if ((e = put_matrix(dx, x, sx, 1, sx, sizeof(T))) != hipSuccess) return hip_fail(e, "trsv: copy x");
-    if ((e = trsv<T>(uplo, trans, diag, n, (const T*)dA, max1(n), (T*)dx, incx, default_precision(), nullptr)) !=
-        hipSuccess)
+    int* fail = nullptr;
+    if ((e = trsv<T>(uplo, trans, diag, n, (const T*)dA, max1(n), (T*)dx, incx, nullptr, &fail)) != hipSuccess)
         return hip_fail(e, "trsv");
-    return finish(get_matrix(x, dx, sx, 1, sx, sizeof(T)), "trsv: copy x back");
+    if ((e = get_matrix(x, dx, sx, 1, sx, sizeof(T))) != hipSuccess) return hip_fail(e, "trsv: copy x back");
+    int failed = 0;
+    if ((e = hipMemcpy(&failed, fail, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "trsv: status");
+    if (failed) {
+        set_error("trsv: a block waited past its limit for an earlier block");
+        return MBH_E_HIP;
+    }
+    return 0;
 }
 
 }  // namespace
@@ -344,12 +481,12 @@ int mbh_dtrsv_host(char uplo, char trans, char diag, int64_t n, const double* a,
     return trsv_host<double>(uplo, trans, diag, n, a, lda, x, incx);
 }
 int mbh_strsv_dev(char uplo, char trans, char diag, int64_t n, const float* a, int64_t lda, float* x, int64_t incx,
-                  int precision, void* stream) {
-    return trsv_dev<float>(uplo, trans, diag, n, a, lda, x, incx, precision, stream);
+                  void* stream) {
+    return trsv_dev<float>(uplo, trans, diag, n, a, lda, x, incx, stream);
 }
 int mbh_dtrsv_dev(char uplo, char trans, char diag, int64_t n, const double* a, int64_t lda, double* x, int64_t incx,
                   void* stream) {
-    return trsv_dev<double>(uplo, trans, diag, n, a, lda, x, incx, 0, stream);
+    return trsv_dev<double>(uplo, trans, diag, n, a, lda, x, incx, stream);
 }
 
 }  // extern "C"

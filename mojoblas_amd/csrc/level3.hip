@@ -351,6 +351,21 @@ hipError_t launch_scale_tri(int64_t m, int64_t n, T beta, T* C, int64_t ldc, hip
 
 }  // namespace
 
+template <typename T>
+hipError_t tri_block_inverses(int64_t na, const T* A, int64_t lda, bool eff_lower, bool trans, bool unit, double* inv,
+                              hipStream_t s) {
+    static_assert(TRI_NB == NB, "one diagonal block size");
+    const int64_t nblk = (na + NB - 1) / NB;
+    if (nblk <= 0) return hipSuccess;
+    if (nblk > INT_MAX) return hipErrorInvalidValue;
+    tri_inv_kernel<T><<<dim3((unsigned)nblk, TI_GROUPS), dim3(TI_NT), 0, s>>>(na, A, lda, eff_lower ? 1 : 0,
+                                                                             trans ? 1 : 0, unit ? 1 : 0, inv);
+    return hipGetLastError();
+}
+template hipError_t tri_block_inverses<float>(int64_t, const float*, int64_t, bool, bool, bool, double*, hipStream_t);
+template hipError_t tri_block_inverses<double>(int64_t, const double*, int64_t, bool, bool, bool, double*,
+                                               hipStream_t);
+
 // ---------------------------------------------------------------------------
 // trsm (trsm.mojo:20-275, BLAS semantics)
 
@@ -382,9 +397,7 @@ hipError_t trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t
     T* tri = (T*)(base + al(inv_b) + al(b0_b));
     T* d = (T*)(base + al(inv_b) + al(b0_b) + al(tri_b));
 
-    tri_inv_kernel<T><<<dim3((unsigned)nblk, TI_GROUPS), dim3(TI_NT), 0, s>>>(na, A, lda, eff_lower ? 1 : 0,
-                                                                             trans ? 1 : 0, unit ? 1 : 0, inv);
-    MBH_TRY(hipGetLastError());
+    MBH_TRY(tri_block_inverses<T>(na, A, lda, eff_lower, trans, unit, inv, s));
     if (do_ir) {
         copy_kernel<T, double><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, B, ldb, b0, m, (double)alpha);
         tri_copy_kernel<T, T><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0, tri,

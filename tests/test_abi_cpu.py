@@ -158,7 +158,7 @@ def test_trsv_info_codes_without_gpu(args, info, entry):
     a, x = np.full(16, 7, dt), np.full(16, 7, dt)
     P = lambda v: ctypes.c_void_p(v.ctypes.data)
     fn = getattr(L, "mbh_%strsv_%s" % (entry[-1], entry[:-2]))
-    extra = ([0] if entry == "dev_s" else []) + ([None] if entry.startswith("dev") else [])
+    extra = [None] if entry.startswith("dev") else []
     assert fn(uplo, trans, diag, n, P(a), lda, P(x), incx, *extra) == info
     assert (x == 7).all()
 
