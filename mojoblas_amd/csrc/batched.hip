@@ -80,96 +80,211 @@ __device__ inline void vb_walk(int* ctr, int nitems, int* slot, F&& body) {
 // ---- lean 64x64 tile ---------------------------------------------------------
 // One 64x64 C tile of a batched problem on 256 threads (2 x 2 waves of 32x32),
 // 32-deep k-tiles double-buffered in LDS through registers.  The batched
-// operands are column-major with ld = rows, so most columns start off 16-byte
-// alignment; the loads are therefore single dwords at 32-bit element offsets
-// from the operand's (uniform) base, with the lanes of an instruction on
-// consecutive addresses, which coalesces for any ld and alignment.  Masks are
-// formed once per tile (rows, columns) and once per k-tile (depth) instead of
-// per element with 64-bit addresses, which keeps the register count low enough
-// for several workgroups per CU.  The plan takes this path only when every
-// problem's offsets fit in int32 (vb_lean_ok).
+// operands are column-major with ld = rows, so columns start at any 4-byte
+// offset; the loads are 16-byte loads at dword alignment (which the hardware
+// serves as one vector load) at 32-bit element offsets from the problem's
+// (uniform) base.  A 16-byte load moves four times the bytes of a dword load
+// per address-unit slot, which is what bounds these short k-loops.  The plan
+// takes this path only when every problem's offsets fit in int32
+// (vb_lean_ok).
 //
-//   A staging: thread t holds A(m0 + t%64, k0 + 8*(t/64) + q), q < 8
-//              (instruction q: 64 lanes read 64 consecutive rows of one column);
-//   B staging: thread t holds B(k0 + t%32, n0 + t/32 + 8q), q < 8
-//              (instruction q: two runs of 32 consecutive k).
+//   A staging: thread t holds rows m0 + 4(t%16) + e, e < 4 (one 16-byte
+//              load), at k = k0 + t/16 and k0 + t/16 + 16;
+//   B staging: thread t holds k = k0 + 4(t%8) + e, e < 4 (one 16-byte load),
+//              in columns n0 + t/8 and n0 + t/8 + 32.
+// Edges: a group of four that runs past the last row m (or past k) is
+// loaded from m - 4 (k - 4) instead and shifted into place, so every load
+// stays inside the operand.  Rows >= m of A and columns >= n of B only reach
+// C entries that are never stored, so they may hold any finite or non-finite
+// data of the operand; k >= K is zeroed in both operands.  Problems with
+// m < 4 or k < 4 load the same layout one dword at a time.
 // fp32 mode: LDS [k][64 + 1] per operand, v_mfma_f32_32x32x2_f32.
 // bf16 mode: the packed swizzled 64-byte-row layout of the split-precision
-//            kernels (A: one 16-byte write of 8 k per thread; B: 2-byte
-//            writes), fragments and MFMAs of packed_ktile_mfma.
+//            kernels (2-byte writes for A, 8-byte writes for B), fragments and
+//            MFMAs of packed_ktile_mfma.
 template <bool BF16>
 struct VbLean {
-    static constexpr int S = 65;                      // fp32 row stride (floats)
+    static constexpr int S = 65;                                     // fp32 row stride (floats)
     static constexpr int BUF = BF16 ? 2 * 64 * 64 : 2 * 32 * S * 4;  // bytes per buffer, A + B
     static constexpr int BYTES = 2 * BUF;
 };
 
+// One work item of the lean path (a 64x64 tile of one problem), read once
+// from the plan; the item index is wave-uniform, so this sits in scalar
+// registers.
+struct VbLeanItem {
+    const MBH_GLOBAL float* A;
+    const MBH_GLOBAL float* B;
+    MBH_GLOBAL float* C;
+    int m, n, K, lda, ldb, ldc, m0, n0, nkt;
+};
+
+__device__ inline VbLeanItem vb_lean_item(const VbProblem* __restrict__ probs, const VbItem* __restrict__ items,
+                                          int it, const float* const* A, const float* const* B, float* const* C) {
+    const VbItem w = items[it];
+    const VbProblem pr = probs[w.prob];
+    VbLeanItem d;
+    d.A = (const MBH_GLOBAL float*)A[w.prob];
+    d.B = (const MBH_GLOBAL float*)B[w.prob];
+    d.C = (MBH_GLOBAL float*)C[w.prob];
+    d.m = (int)pr.m;
+    d.n = (int)pr.n;
+    d.K = (int)pr.k;
+    d.lda = (int)pr.lda;
+    d.ldb = (int)pr.ldb;
+    d.ldc = (int)pr.ldc;
+    d.m0 = (w.tile / pr.tiles_n) * 64;
+    d.n0 = (w.tile % pr.tiles_n) * 64;
+    d.nkt = (d.K + 31) / 32;
+    return d;
+}
+
+// 16 bytes at byte offset off (a multiple of 4) from a uniform base: one
+// global_load_dwordx4 with the base in scalar registers and a 32-bit offset.
+__device__ inline float4 vb_load4(const MBH_GLOBAL float* base, unsigned off) {
+    float4 v;
+    __builtin_memcpy(&v, (const MBH_GLOBAL unsigned char*)base + off, 16);
+    return v;
+}
+
+// v shifted down by s elements: out[e] = v[min(e + s, 3)], as a two-stage
+// shifter of selects (shift by 2, then by 1; s >= 3 shifts by 3).
+__device__ inline void vb_shift4(const float4& v, int s, float* out) {
+    const bool b2 = s >= 2, b1 = (s & 1) || s >= 3;
+    const float t0 = b2 ? v.z : v.x, t1 = b2 ? v.w : v.y, t2 = b2 ? v.w : v.z;
+    out[0] = b1 ? t1 : t0;
+    out[1] = b1 ? t2 : t1;
+    out[2] = b1 ? v.w : t2;
+    out[3] = v.w;
+}
+
+// Persistent walk over the work queue with the k-loop pipelined across
+// items: the last k-step of an item loads the first k-tile of the next item
+// (claimed one item ahead), so a workgroup moves from tile to tile without
+// an exposed load.  Two slots in LDS alternate between items: the slot an
+// item reads was written before the previous item's last barrier, and the
+// other slot is rewritten only after every thread has passed a barrier since
+// reading it (each item passes at least one).  Thread 0 issues the claim
+// when an item starts and publishes it before the item's last barrier.
 template <bool BF16>
-__device__ inline void vb_lean_tile(const VbProblem& pr, const float* Ag, const float* Bg, float* Cg, float alpha,
-                                    float beta, int tm, int tn, unsigned char* lds) {
+__global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* __restrict__ probs,
+                                                           const VbItem* __restrict__ items, int nitems, int* ctr,
+                                                           float alpha, float beta, const float* const* A,
+                                                           const float* const* B, float* const* C) {
     typedef VbLean<BF16> L;
     constexpr int S = L::S;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
+    __shared__ int slot[2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m = (int)pr.m, n = (int)pr.n, K = (int)pr.k, lda = (int)pr.lda, ldb = (int)pr.ldb;
-    const int m0 = tm * 64, n0 = tn * 64;
-    const MBH_GLOBAL float* const A = (const MBH_GLOBAL float*)Ag;
-    const MBH_GLOBAL float* const B = (const MBH_GLOBAL float*)Bg;
-    const int arow = m0 + lane;
-    const bool arow_ok = arow < m;
-    const int ak = 8 * wave;
-    const int bk = tid & 31, bc = tid >> 5;
-    // Branch-free masked loads: a masked lane reads element 0 (every problem
-    // on this path has k >= 1 when it loads) and selects zero, so the 16
-    // loads of a k-tile issue back to back (guarded loads each became a
-    // branch, and the register reuse across them serialized the loads).
-    float ra[8], rb[8];
-    auto load = [&](int k0) {
-        bool va[8], vb[8];
+    const int ga = tid & 15, ka = tid >> 4;  // A: row group, k
+    const int gb = tid & 7, cb = tid >> 3;   // B: k group, column
+    const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
+    // Staged k-tile: va[q] holds the four elements of A from linear index
+    // r + k lda - sa[q] (r = m0 + 4ga, k = k0 + ka + 16q), vb[q] those of B
+    // from kk + col ldb - sb[q] (kk = k0 + 4gb, col = n0 + cb + 32q); the
+    // store shifts them back by sa / sb and zeroes k >= K.  A group of four
+    // that would run past the operand's last element is loaded from its last
+    // four instead, so every load stays inside the operand.  The loads only
+    // issue here and are consumed by the store after the k-tile's MFMAs.
+    float4 va[2], vb[2];
+    int sa[2], sb[2];
+    auto load = [&](const VbLeanItem& d, int k0) {
+        const int r = d.m0 + 4 * ga;
+        const int kk = k0 + 4 * gb;
+        // Last element of each operand (column-major, the final column
+        // holds m (A) or k (B) elements).
+        const int alast = (d.K - 1) * d.lda + d.m - 1, blast = (d.n - 1) * d.ldb + d.K - 1;
+        if (alast >= 3 && blast >= 3) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int k = k0 + ak + q;
-            va[q] = arow_ok && k < K;
-            ra[q] = A[va[q] ? arow + k * lda : 0];
-        }
-        const int k = k0 + bk;
-        const bool kok = k < K;
+            for (int q = 0; q < 2; ++q) {
+                const int ia = r + (k0 + ka + 16 * q) * d.lda, ja = min(ia, alast - 3);
+                sa[q] = ia - ja;
+                va[q] = vb_load4(d.A, 4u * (unsigned)ja);
+                const int ib = kk + min(d.n0 + cb + 32 * q, d.n - 1) * d.ldb, jb = min(ib, blast - 3);
+                sb[q] = ib - jb;
+                vb[q] = vb_load4(d.B, 4u * (unsigned)jb);
+            }
+        } else {
+            // Operands of fewer than four elements: single elements, relaxed
+            // atomic loads so that they stay apart from the 16-byte loads.
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int col = n0 + bc + 8 * q;
-            vb[q] = kok && col < n;
-            rb[q] = B[vb[q] ? k + col * ldb : 0];
-        }
+            for (int q = 0; q < 2; ++q) {
+                sa[q] = 0;
+                sb[q] = 0;
+                const int k = k0 + ka + 16 * q, col = d.n0 + cb + 32 * q;
+                float a4[4], b4[4];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            ra[q] = va[q] ? ra[q] : 0.0f;
-            rb[q] = vb[q] ? rb[q] : 0.0f;
+                for (int e = 0; e < 4; ++e) {
+                    const int ia = r + e < d.m && k < d.K ? r + e + k * d.lda : 0;
+                    const int ib = kk + e < d.K && col < d.n ? kk + e + col * d.ldb : 0;
+                    a4[e] = __hip_atomic_load((const float*)d.A + ia, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    b4[e] = __hip_atomic_load((const float*)d.B + ib, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                va[q] = float4{a4[0], a4[1], a4[2], a4[3]};
+                vb[q] = float4{b4[0], b4[1], b4[2], b4[3]};
+            }
         }
     };
-    auto store = [&](MBH_LDS unsigned char* buf) {
-        if constexpr (BF16) {
-            MBH_LDS u32x4* a4 = (MBH_LDS u32x4*)buf;
-            MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + 64 * 64);
-            a4[lane * 4 + (wave ^ lds_swz(lane))] =
-                u32x4{bf16_rn_bits(ra[0]) | (bf16_rn_bits(ra[1]) << 16), bf16_rn_bits(ra[2]) | (bf16_rn_bits(ra[3]) << 16),
-                      bf16_rn_bits(ra[4]) | (bf16_rn_bits(ra[5]) << 16), bf16_rn_bits(ra[6]) | (bf16_rn_bits(ra[7]) << 16)};
+    // krem = K - k0 of the staged k-tile.  A k-tile with krem > 32 holds
+    // neither operand's last element, so its groups need no shift and no
+    // k mask.
+    auto store = [&](MBH_LDS unsigned char* buf, int krem) {
+        float ra[8], rb[8];
+        if (krem > 32) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int j = bc + 8 * q;
-                b16[j * 32 + (((bk >> 3) ^ lds_swz(j)) << 3) + (bk & 7)] = (uint16_t)bf16_rn_bits(rb[q]);
+            for (int q = 0; q < 2; ++q) {
+                ra[4 * q] = va[q].x, ra[4 * q + 1] = va[q].y, ra[4 * q + 2] = va[q].z, ra[4 * q + 3] = va[q].w;
+                rb[4 * q] = vb[q].x, rb[4 * q + 1] = vb[q].y, rb[4 * q + 2] = vb[q].z, rb[4 * q + 3] = vb[q].w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                float a4[4], b4[4];
+                vb_shift4(va[q], sa[q], a4);
+                vb_shift4(vb[q], sb[q], b4);
+                const bool kva = ka + 16 * q < krem;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    ra[4 * q + e] = kva ? a4[e] : 0.0f;
+                    rb[4 * q + e] = 4 * gb + e < krem ? b4[e] : 0.0f;
+                }
+            }
+        }
+        if constexpr (BF16) {
+            MBH_LDS uint16_t* a16 = (MBH_LDS uint16_t*)buf;
+            MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + 64 * 64);
+            typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int k = ka + 16 * q;
+#pragma unroll
+                for (int e = 0; e < 4; e += 2) {
+                    const uint32_t w = bf16x2_rn_bits(ra[4 * q + e], ra[4 * q + e + 1]);
+                    const int r0 = 4 * ga + e, r1 = r0 + 1;
+                    a16[r0 * 32 + (((k >> 3) ^ lds_swz(r0)) << 3) + (k & 7)] = (uint16_t)w;
+                    a16[r1 * 32 + (((k >> 3) ^ lds_swz(r1)) << 3) + (k & 7)] = (uint16_t)(w >> 16);
+                }
+                const int j = cb + 32 * q, k4 = 4 * gb;
+                *(MBH_LDS u32x2*)(b16 + j * 32 + (((k4 >> 3) ^ lds_swz(j)) << 3) + (k4 & 7)) =
+                    u32x2{bf16x2_rn_bits(rb[4 * q], rb[4 * q + 1]), bf16x2_rn_bits(rb[4 * q + 2], rb[4 * q + 3])};
             }
         } else {
             MBH_LDS float* as = (MBH_LDS float*)buf;
             MBH_LDS float* bs = as + 32 * S;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) as[(ak + q) * S + lane] = ra[q];
+            for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int q = 0; q < 8; ++q) bs[bk * S + bc + 8 * q] = rb[q];
+                for (int e = 0; e < 4; ++e) {
+                    as[(ka + 16 * q) * S + 4 * ga + e] = ra[4 * q + e];
+                    bs[(4 * gb + e) * S + cb + 32 * q] = rb[4 * q + e];
+                }
         }
     };
-    const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
     f32x16 acc[1][1];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[0][0][r] = 0.0f;
+    // The scheduling barrier keeps the staging of the next k-tile (which
+    // waits for its loads) behind this k-tile's MFMAs.
     auto compute = [&](const MBH_LDS unsigned char* buf) {
         if constexpr (BF16) {
             const MBH_LDS u32x4* a4 = (const MBH_LDS u32x4*)buf;
@@ -182,47 +297,89 @@ __device__ inline void vb_lean_tile(const VbProblem& pr, const float* Ag, const 
             for (int kk = 0; kk < 32; kk += 2)
                 acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(bs[kk * S], as[kk * S], acc[0][0], 0, 0, 0);
         }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    // C^T accumulator: register r of lane l is C(row0 + l%32, col0 + acc_col(r, l)).
+    auto epilogue = [&](const VbLeanItem& d) {
+        const int row = d.m0 + wm0 + (lane & 31);
+        const bool rok = row < d.m;
+        const int col0 = d.n0 + wn0;
+        if (beta == 0.0f) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int col = col0 + acc_col<32>(r, lane);
+                if (rok && col < d.n) d.C[row + col * d.ldc] = alpha * acc[0][0][r];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int col = col0 + acc_col<32>(r, lane);
+                if (rok && col < d.n) {
+                    MBH_GLOBAL float* cp = d.C + (row + col * d.ldc);
+                    *cp = alpha * acc[0][0][r] + beta * *cp;
+                }
+            }
+        }
     };
     MBH_LDS unsigned char* const L0 = (MBH_LDS unsigned char*)lds;
-    MBH_LDS unsigned char* const L1 = L0 + L::BUF;
-    const int nkt = (K + 31) / 32;
-    if (nkt > 0) {
-        load(0);
-        store(L0);
+    auto lbuf = [&](int b) { return L0 + b * L::BUF; };
+
+    if (tid == 0) {
+        slot[0] = atomicAdd(ctr, 1);
+        slot[1] = atomicAdd(ctr, 1);
     }
     __syncthreads();
-    for (int kt = 0; kt < nkt; ++kt) {
-        MBH_LDS unsigned char* cur = (kt & 1) ? L1 : L0;
-        MBH_LDS unsigned char* nxt = (kt & 1) ? L0 : L1;
-        if (kt + 1 < nkt) load((kt + 1) * 32);
-        compute(cur);
-        if (kt + 1 < nkt) store(nxt);
-        __syncthreads();
+    const int it0 = __builtin_amdgcn_readfirstlane(slot[0]);
+    if (it0 >= nitems) return;
+    VbLeanItem cur = vb_lean_item(probs, items, it0, A, B, C);
+    int sel = 1, buf = 0;
+    if (cur.nkt > 0) {
+        load(cur, 0);
+        store(lbuf(0), cur.K);
     }
-    epilogue_32x32<1, 1>(acc, Cg, pr.ldc, pr.m, pr.n, m0 + wm0, n0 + wn0, lane, alpha, beta, TRI_NONE);
-}
-
-template <bool BF16>
-__global__ __launch_bounds__(256, BF16 ? 4 : 2) void vb_gemm_lean_kernel(const VbProblem* __restrict__ probs,
-                                                           const VbItem* __restrict__ items, int nitems, int* ctr,
-                                                           float alpha, float beta, const float* const* A,
-                                                           const float* const* B, float* const* C) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds[VbLean<BF16>::BYTES];
-    __shared__ int slot;
-    vb_walk(ctr, nitems, &slot, [&](int it) {
-        const VbItem w = items[it];
-        const VbProblem pr = probs[w.prob];
-        vb_lean_tile<BF16>(pr, A[w.prob], B[w.prob], C[w.prob], alpha, beta, w.tile / pr.tiles_n,
-                           w.tile % pr.tiles_n, lds);
-    });
+    __syncthreads();
+    for (;;) {
+        // Invariant: the current item's first k-tile is in lbuf(buf).
+        const int itn = __builtin_amdgcn_readfirstlane(slot[sel]);
+        const bool has_next = itn < nitems;
+        int claim = 0;
+        if (tid == 0 && has_next) claim = atomicAdd(ctr, 1);
+        VbLeanItem nx = cur;
+        if (has_next) nx = vb_lean_item(probs, items, itn, A, B, C);
+        const bool pre = has_next && nx.nkt > 0;
+        // k-steps but the last, then the last one, which loads the next
+        // item's first k-tile instead (k = 0: C := beta C, no k-steps).
+        for (int kt = 0; kt + 1 < cur.nkt; ++kt) {
+            load(cur, (kt + 1) * 32);
+            compute(lbuf(buf));
+            store(lbuf(buf ^ 1), cur.K - (kt + 1) * 32);
+            __syncthreads();
+            buf ^= 1;
+        }
+        if (pre) load(nx, 0);
+        if (cur.nkt > 0) compute(lbuf(buf));
+        const int nbuf = cur.nkt > 0 ? buf ^ 1 : buf;
+        if (pre) store(lbuf(nbuf), nx.K);
+        if (tid == 0 && has_next) slot[sel ^ 1] = claim;
+        __syncthreads();
+        buf = nbuf;
+        epilogue(cur);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[0][0][r] = 0.0f;
+        if (!has_next) break;
+        cur = nx;
+        sel ^= 1;
+    }
 }
 
 // A problem can take the lean tile when its element offsets (with the
-// k-tile and tile overhang the tile computes before masking) fit in int32.
+// k-tile and tile overhang the tile computes before clamping) fit in int32
+// and the byte offsets of the elements it loads fit in 32 bits.
 bool vb_lean_ok(const VbProblem& q) {
-    const int64_t lim = (int64_t)1 << 31;
-    return q.m < lim && q.n < lim && q.k < lim && q.lda * (q.k + 64) < lim && q.ldb * (q.n + 64) + q.k + 64 < lim &&
-           q.lda + q.m + 64 < lim;
+    const int64_t l31 = (int64_t)1 << 31, l30 = (int64_t)1 << 30;
+    return q.m < l30 && q.n < l30 && q.k < l30 && q.lda * (q.k + 64) + q.m + 64 < l31 &&
+           q.ldb * (q.n + 64) + q.k + 64 < l31 && q.ldc * (q.n + 64) + q.m + 64 < l31 &&
+           q.lda * q.k < l30 && q.ldb * q.n < l30;  // byte offsets of loaded elements fit 32 bits
 }
 
 // MAXCLS: highest tile class compiled in.  The 64x64-only variant needs far
@@ -489,7 +646,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         int prob, tile;
         double cost;
     };
-    std::vector<Item> items;
+    int64_t ntiles = 0;
     for (int64_t p = 0; p < count; ++p) {
         VbProblem& q = probs[(size_t)p];
         q.m = m[p];
@@ -506,17 +663,35 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         if (q.m == 0 || q.n == 0) continue;
         plan->class_count[q.cls]++;
         if (!vb_lean_ok(q)) plan->lean = false;
-        for (int64_t t = 0; t < tm * tn; ++t) items.push_back({(int)p, (int)t, (double)T * T * (double)q.k});
+        ntiles += tm * tn;
     }
-    // Longest-first: the dynamic queue then finishes with the short items.
-    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-    std::vector<VbItem> ditems(items.size());
-    for (size_t i = 0; i < items.size(); ++i) ditems[i] = {items[i].prob, items[i].tile};
-    if (items.size() > (size_t)INT32_MAX) {
+    if (ntiles > (int64_t)INT32_MAX) {
         delete plan;
         set_error("mbh_vbatch_plan_create: too many tiles");
         return MBH_E_ARG;
     }
+    const bool mixed = plan->class_count[1] > 0;
+    const bool use_lean = !mixed && plan->lean;
+    // Cost of one tile of problem p (a k == 0 tile still scales C).
+    auto tile_cost = [&](int64_t p) {
+        const int T = vb_tile(probs[(size_t)p].cls);
+        return (double)T * T * (double)std::max<int64_t>(probs[(size_t)p].k, 1);
+    };
+    auto tiles_of = [&](int64_t p) {
+        const VbProblem& q = probs[(size_t)p];
+        const int T = vb_tile(q.cls);
+        return ((q.m + T - 1) / T) * ((q.n + T - 1) / T);
+    };
+    std::vector<Item> items;
+    items.reserve((size_t)ntiles);
+    for (int64_t p = 0; p < count; ++p)
+        if (probs[(size_t)p].m > 0 && probs[(size_t)p].n > 0)
+            for (int64_t t = 0; t < tiles_of(p); ++t) items.push_back({(int)p, (int)t, tile_cost(p)});
+    // Longest-first (the tiles of a problem stay adjacent): the dynamic
+    // queue then finishes with the short items.
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+    std::vector<VbItem> ditems(items.size());
+    for (size_t i = 0; i < items.size(); ++i) ditems[i] = {items[i].prob, items[i].tile};
     plan->nitems = (int)ditems.size();
     int dev = 0, ncu = 256;
     hipError_t e = hipGetDevice(&dev);
@@ -527,9 +702,8 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
     // A persistent grid never asks for more workgroups per CU than the kernel
     // it launches can keep resident (late workgroups would only find the
     // queue drained).
-    const bool mixed = plan->class_count[1] > 0;
     const void* kfn = nullptr;
-    if (!mixed && plan->lean)
+    if (use_lean)
         kfn = precision == MBH_PREC_BF16 ? (const void*)vb_gemm_lean_kernel<true> : (const void*)vb_gemm_lean_kernel<false>;
     else if (precision == MBH_PREC_BF16)
         kfn = mixed ? (const void*)vb_gemm_bf16_kernel<1> : (const void*)vb_gemm_bf16_kernel<0>;

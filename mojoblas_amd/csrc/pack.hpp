@@ -23,6 +23,15 @@ __device__ inline uint32_t bf16_rn_bits(float x) {
     return u >> 16;
 }
 
+// a and b rounded to bf16 (nearest, ties to even) and packed, a in the low
+// half: one v_cvt_pk_bf16_f32.  Same bits as bf16_rn_bits for every
+// non-NaN input (a NaN stays a quiet NaN).
+__device__ inline uint32_t bf16x2_rn_bits(float a, float b) {
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t{a, b}), bf16x2_t));
+}
+
 __device__ inline void split_bf16(float x, uint32_t& h, uint32_t& l) {
     h = bf16_rn_bits(x);
     const float hf = __uint_as_float(h << 16);
