@@ -6,7 +6,8 @@
 //         product, every off-diagonal update is one large gemm on the fp32
 //         (or BF16x3) matrix cores.  For fp32 data one step of iterative
 //         refinement follows: R = alpha*B - op(A)*X formed in fp64 from the
-//         fp32 operands, op(A)*D = R solved the same way, X += D.
+//         fp32 operands, op(A)*D = R solved the same way but with bf16
+//         trailing updates (D needs only a few digits), X += D.
 //   trmm  op(A) materialized as a full triangle (zeros, unit diagonal), one
 //         gemm whose tiles visit only the k-tiles where op(A) is nonzero, into
 //         scratch, then copied back (trmm is in place).
@@ -422,7 +423,13 @@ hipError_t trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t
     }
     copy_kernel<double, T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, b0, m, d, m, 1.0);
     MBH_TRY(hipGetLastError());
-    MBH_TRY(ts.solve(d, m, 0, na));
+    // The correction d = op(A)^-1 R only needs a few correct digits: R is
+    // of the size of the first solve's error, so a relative error e in d
+    // adds e |R|-sized error to X, far below fp32 rounding for e ~ 1e-2.
+    // Its trailing updates therefore run as single bf16 products (fp32
+    // accumulate); the fp64 residual sets the refined accuracy.
+    TriSolver<T> tc{left, eff_lower, trans, m, n, A, lda, inv, PREC_BF16, s};
+    MBH_TRY(tc.solve(d, m, 0, na));
     add_kernel<T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, d, m, B, ldb);
     return hipGetLastError();
 }
