@@ -146,9 +146,11 @@ namespace mbh {
 // 128-row x 128-k tile.  Packed like the bf16 planes (blocks of PR rows x 64
 // k, one byte per element, k fastest, zero padding), scales as E8M0 bytes
 // (exponent + 127) in a [R/128][K/128] int array: x ~= 2^(e) * q.
-// One workgroup stages its whole 128x128 fp32 tile in LDS, reduces the tile's
-// amax, picks e = ceil(log2(amax / 448)) and writes 16 quantized bytes per
-// store.
+// One workgroup holds its whole 128x128 fp32 tile in registers (64 values
+// per thread, each thread's share being 16 consecutive k of one or four
+// rows), reduces the tile's amax (wave shuffles, then four partials in LDS),
+// picks e = ceil(log2(amax / 448)) and writes its rows' 16-byte runs of
+// quantized k straight from registers.
 struct PackFp8Params {
     const float* X;
     int64_t ld;
@@ -169,53 +171,42 @@ __device__ inline uint32_t fp8_pack4(float a, float b, float c, float d) {
 template <bool KCONTIG, int PR>
 __global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params p) {
     constexpr int T = 128, PK = 64;
-    __shared__ float s[T][T + 1];
-    __shared__ float red[256];
+    __shared__ float red[4];
     const int tid = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.x * T, k0 = (int64_t)blockIdx.y * T;
+    const bool full = p.vec && r0 + T <= p.R && k0 + T <= p.K;  // no edge: unguarded 16-byte loads
+    // KCONTIG ((r, k) at X[k + r ld]): v[j] = X(gr, gk0 + 4j .. 4j + 3), one
+    //   row gr = r0 + tid/2 and 64 consecutive k from gk0 = k0 + 64 (tid%2);
+    // else ((r, k) at X[r + k ld]): v[j] = X(gr0 .. gr0 + 3, gk0 + j), four
+    //   rows from gr0 = r0 + 4 (tid%32) and 16 consecutive k from
+    //   gk0 = k0 + 16 (tid/32).
+    f32x4 v[16];
+    const int64_t gr0 = KCONTIG ? r0 + tid / 2 : r0 + 4 * (tid % 32);
+    const int64_t gk0 = KCONTIG ? k0 + 64 * (tid % 2) : k0 + 16 * (tid / 32);
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            v[j] = KCONTIG ? *reinterpret_cast<const f32x4*>(p.X + gk0 + 4 * j + gr0 * p.ld)
+                           : *reinterpret_cast<const f32x4*>(p.X + gr0 + (gk0 + j) * p.ld);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t gr = KCONTIG ? gr0 : gr0 + e, gk = KCONTIG ? gk0 + 4 * j + e : gk0 + j;
+                v[j][e] = (gr < p.R && gk < p.K) ? p.X[KCONTIG ? gk + gr * p.ld : gr + gk * p.ld] : 0.0f;
+            }
+    }
     float amax = 0.0f;
-#pragma unroll 4
-    for (int it = 0; it < 16; ++it) {
-        const int idx = tid + 256 * it;
-        if (KCONTIG) {  // (r, k) at X[k + r*ld]: 32 float4 per row
-            const int r = idx / 32, kq = (idx % 32) * 4;
-            const int64_t gr = r0 + r, gk = k0 + kq;
-            f32x4 v = {0.f, 0.f, 0.f, 0.f};
-            if (p.vec && gr < p.R && gk + 4 <= p.K) {
-                v = *reinterpret_cast<const f32x4*>(p.X + gk + gr * p.ld);
-            } else {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) v[e] = (gr < p.R && gk + e < p.K) ? p.X[gk + e + gr * p.ld] : 0.0f;
-            }
+    for (int j = 0; j < 16; ++j)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                s[r][kq + e] = v[e];
-                amax = fmaxf(amax, fabsf(v[e]));
-            }
-        } else {  // (r, k) at X[r + k*ld]: 32 float4 per k column
-            const int kk = idx / 32, rq = (idx % 32) * 4;
-            const int64_t gr = r0 + rq, gk = k0 + kk;
-            f32x4 v = {0.f, 0.f, 0.f, 0.f};
-            if (p.vec && gk < p.K && gr + 4 <= p.R) {
-                v = *reinterpret_cast<const f32x4*>(p.X + gr + gk * p.ld);
-            } else {
+        for (int e = 0; e < 4; ++e) amax = fmaxf(amax, fabsf(v[j][e]));
 #pragma unroll
-                for (int e = 0; e < 4; ++e) v[e] = (gr + e < p.R && gk < p.K) ? p.X[gr + e + gk * p.ld] : 0.0f;
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                s[rq + e][kk] = v[e];
-                amax = fmaxf(amax, fabsf(v[e]));
-            }
-        }
-    }
-    red[tid] = amax;
+    for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = amax;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) red[tid] = fmaxf(red[tid], red[tid + w]);
-        __syncthreads();
-    }
-    amax = red[0];
+    amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
     // Smallest power of two s with amax / s <= 448 (e4m3 max); 1 for a zero tile.
     int e = 0;
     if (amax > 0.0f && __builtin_isfinite(amax)) {
@@ -226,18 +217,30 @@ __global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params p) {
     }
     const float inv = ldexpf(1.0f, -e);
     if (tid == 0) p.scale[(r0 / T) * p.kblocks + k0 / T] = e + 127;
+    // Packed byte offset of element (gr, gk), gk a multiple of 16.
+    auto dst = [&](int64_t gr, int64_t gk) {
+        return reinterpret_cast<uint4*>(p.q + (((gr / PR) * p.ktiles + gk / PK) * PR + gr % PR) * PK + gk % PK);
+    };
+    if (KCONTIG) {
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
-        const int idx = tid + 256 * it;
-        const int r = idx / 8, kc = (idx % 8) * 16;
-        const int64_t gr = r0 + r, gk = k0 + kc;
-        const int64_t t = gr / PR, rr = gr % PR, kt = gk / PK, kk = gk % PK;
-        uint4 w;
-        w.x = fp8_pack4(s[r][kc + 0] * inv, s[r][kc + 1] * inv, s[r][kc + 2] * inv, s[r][kc + 3] * inv);
-        w.y = fp8_pack4(s[r][kc + 4] * inv, s[r][kc + 5] * inv, s[r][kc + 6] * inv, s[r][kc + 7] * inv);
-        w.z = fp8_pack4(s[r][kc + 8] * inv, s[r][kc + 9] * inv, s[r][kc + 10] * inv, s[r][kc + 11] * inv);
-        w.w = fp8_pack4(s[r][kc + 12] * inv, s[r][kc + 13] * inv, s[r][kc + 14] * inv, s[r][kc + 15] * inv);
-        *reinterpret_cast<uint4*>(p.q + ((t * p.ktiles + kt) * PR + rr) * PK + kk) = w;
+        for (int c = 0; c < 4; ++c) {
+            uint4 w;
+            w.x = fp8_pack4(v[4 * c][0] * inv, v[4 * c][1] * inv, v[4 * c][2] * inv, v[4 * c][3] * inv);
+            w.y = fp8_pack4(v[4 * c + 1][0] * inv, v[4 * c + 1][1] * inv, v[4 * c + 1][2] * inv, v[4 * c + 1][3] * inv);
+            w.z = fp8_pack4(v[4 * c + 2][0] * inv, v[4 * c + 2][1] * inv, v[4 * c + 2][2] * inv, v[4 * c + 2][3] * inv);
+            w.w = fp8_pack4(v[4 * c + 3][0] * inv, v[4 * c + 3][1] * inv, v[4 * c + 3][2] * inv, v[4 * c + 3][3] * inv);
+            *dst(gr0, gk0 + 16 * c) = w;
+        }
+    } else {
+#pragma unroll
+        for (int e4 = 0; e4 < 4; ++e4) {
+            uint4 w;
+            w.x = fp8_pack4(v[0][e4] * inv, v[1][e4] * inv, v[2][e4] * inv, v[3][e4] * inv);
+            w.y = fp8_pack4(v[4][e4] * inv, v[5][e4] * inv, v[6][e4] * inv, v[7][e4] * inv);
+            w.z = fp8_pack4(v[8][e4] * inv, v[9][e4] * inv, v[10][e4] * inv, v[11][e4] * inv);
+            w.w = fp8_pack4(v[12][e4] * inv, v[13][e4] * inv, v[14][e4] * inv, v[15][e4] * inv);
+            *dst(gr0 + e4, gk0) = w;
+        }
     }
 }
 
