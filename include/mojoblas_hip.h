@@ -92,8 +92,9 @@ int mbh_dgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, dou
  * reference solves the untransposed system there, trsm.mojo:223-273).
  * fp32: `precision` selects the gemm of the off-diagonal updates; the 128x128
  * diagonal blocks are inverted and applied in fp64; refine = 1 adds one step
- * of iterative refinement with an fp64 residual (0 = none).  The _host entry
- * uses the default precision and refine = 1. */
+ * of iterative refinement with an fp64 residual (0 = none), whose correction
+ * solve runs its updates as single bf16 products.  The _host entry uses the
+ * default precision and refine = 1. */
 int mbh_strsm_host(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float alpha,
                    const float* a, int64_t lda, float* b, int64_t ldb);
 int mbh_dtrsm_host(char side, char uplo, char transa, char diag, int64_t m, int64_t n, double alpha,
@@ -193,7 +194,8 @@ int mbh_dtrsv_dev(char uplo, char trans, char diag, int64_t n, const double* a, 
 /* ---- variable-size batched gemm (no counterpart in the reference) --------
  * count independent problems C_p := alpha*A_p*B_p + beta*C_p (each with the
  * semantics of gemm("N","N", m_p, n_p, k_p, ...), gemm.mojo:56-124), run as
- * one persistent kernel that pulls C tiles from a work queue.
+ * persistent kernels that pull C tiles from a work queue per tile class
+ * (128x128 tiles first, then 64x64).
  *  plan_create : host arrays of shapes and leading dimensions; validates every
  *                problem (MBH_E_ARG names the first bad one), assigns tile
  *                classes from the tuning table, orders the tiles longest-first
@@ -208,7 +210,8 @@ int mbh_dtrsv_dev(char uplo, char trans, char diag, int64_t n, const double* a, 
  *  set_tuning  : for one precision, the class table (36 ints,
  *                [bucket(m)][bucket(n)], buckets <=16,32,64,128,256,larger;
  *                0 = 64x64 tiles, 1 = 128x128; NULL = compiled default) and
- *                workgroups per CU (0 = default) for plans created afterwards. */
+ *                workgroups per CU (0 = default; each tile class is capped at
+ *                what its kernel keeps resident) for plans created afterwards. */
 typedef struct mbh_vbatch_plan_s* mbh_vbatch_plan;
 int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, const int64_t* k,
                            const int64_t* lda, const int64_t* ldb, const int64_t* ldc, int precision,

@@ -45,7 +45,7 @@ __host__ __device__ constexpr int vb_tile(int cls) { return cls == 0 ? 64 : 128;
 namespace {
 
 __global__ void vb_reset_kernel(int* ctr) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctr[threadIdx.x] = 0;
 }
 
 // fp32 batched tiles: the 64x64 class runs 32-deep k-tiles with prefetch
@@ -77,41 +77,46 @@ __device__ inline void vb_walk(int* ctr, int nitems, int* slot, F&& body) {
     }
 }
 
-// ---- lean 64x64 tile ---------------------------------------------------------
-// One 64x64 C tile of a batched problem on 256 threads (2 x 2 waves of 32x32),
-// 32-deep k-tiles double-buffered in LDS through registers.  The batched
-// operands are column-major with ld = rows, so columns start at any 4-byte
-// offset; the loads are 16-byte loads at dword alignment (which the hardware
-// serves as one vector load) at 32-bit element offsets from the problem's
-// (uniform) base.  A 16-byte load moves four times the bytes of a dword load
-// per address-unit slot, which is what bounds these short k-loops.  The plan
-// takes this path only when every problem's offsets fit in int32
-// (vb_lean_ok).
+// ---- lean tiles -------------------------------------------------------------
+// One T x T C tile (T = 64 or 128) of a batched problem on 256 threads
+// (2 x 2 waves of T/2 x T/2), 32-deep k-tiles double-buffered in LDS through
+// registers.  The batched operands are column-major with ld = rows, so
+// columns start at any 4-byte offset; the loads are 16-byte loads at dword
+// alignment (which the hardware serves as one vector load) at 32-bit byte
+// offsets from the problem's (uniform) base.  A 16-byte load moves four
+// times the bytes of a dword load per address-unit slot, and T = 128 halves
+// the bytes per flop of T = 64.  The plan takes this path only when every
+// problem's offsets fit in int32 (vb_lean_ok).
 //
-//   A staging: thread t holds rows m0 + 4(t%16) + e, e < 4 (one 16-byte
-//              load), at k = k0 + t/16 and k0 + t/16 + 16;
+//   A staging: thread t holds rows m0 + 4(t % (T/4)) + e, e < 4 (one 16-byte
+//              load), at k = k0 + t/(T/4) + (1024/T) q, q < T/32;
 //   B staging: thread t holds k = k0 + 4(t%8) + e, e < 4 (one 16-byte load),
-//              in columns n0 + t/8 and n0 + t/8 + 32.
-// Edges: a group of four that runs past the last row m (or past k) is
-// loaded from m - 4 (k - 4) instead and shifted into place, so every load
-// stays inside the operand.  Rows >= m of A and columns >= n of B only reach
-// C entries that are never stored, so they may hold any finite or non-finite
-// data of the operand; k >= K is zeroed in both operands.  Problems with
-// m < 4 or k < 4 load the same layout one dword at a time.
-// fp32 mode: LDS [k][64 + 1] per operand, v_mfma_f32_32x32x2_f32.
+//              in columns n0 + t/8 + 32q, q < T/32.
+// Edges: a group of four that would run past the operand's last element is
+// loaded from its last four instead and shifted back, so every load stays
+// inside the operand.  Rows >= m of A and columns >= n of B only reach C
+// entries that are never stored, so they may hold any data of the operand;
+// k >= K is zeroed in both operands.  Operands of fewer than four elements
+// load single elements.
+// fp32 mode: LDS [k][T + 1] per operand, v_mfma_f32_32x32x2_f32.
 // bf16 mode: the packed swizzled 64-byte-row layout of the split-precision
 //            kernels (2-byte writes for A, 8-byte writes for B), fragments and
 //            MFMAs of packed_ktile_mfma.
-template <bool BF16>
+template <bool BF16, int T>
 struct VbLean {
-    static constexpr int S = 65;                                     // fp32 row stride (floats)
-    static constexpr int BUF = BF16 ? 2 * 64 * 64 : 2 * 32 * S * 4;  // bytes per buffer, A + B
+    static_assert(T == 64 || T == 128, "lean tiles are 64 or 128 wide");
+    static constexpr int S = T + 1;                                  // fp32 row stride (floats)
+    static constexpr int BUF = BF16 ? 2 * T * 64 : 2 * 32 * S * 4;  // bytes per buffer, A + B
     static constexpr int BYTES = 2 * BUF;
+    static constexpr int NQ = T / 32;          // 16-byte loads per operand per thread
+    static constexpr int TW = T / 64;          // 32x32 blocks per wave and dimension
+    static constexpr int AG = T / 4;           // A row groups
+    static constexpr int KS = 1024 / T;        // k stride between a thread's A loads
+    static constexpr int MINWG = T == 64 ? 4 : 2;  // resident workgroups per CU (register bound)
 };
 
-// One work item of the lean path (a 64x64 tile of one problem), read once
-// from the plan; the item index is wave-uniform, so this sits in scalar
-// registers.
+// One work item of the lean path (a tile of one problem), read once from the
+// plan; the item index is wave-uniform, so this sits in scalar registers.
 struct VbLeanItem {
     const MBH_GLOBAL float* A;
     const MBH_GLOBAL float* B;
@@ -119,6 +124,7 @@ struct VbLeanItem {
     int m, n, K, lda, ldb, ldc, m0, n0, nkt;
 };
 
+template <int T>
 __device__ inline VbLeanItem vb_lean_item(const VbProblem* __restrict__ probs, const VbItem* __restrict__ items,
                                           int it, const float* const* A, const float* const* B, float* const* C) {
     const VbItem w = items[it];
@@ -133,8 +139,8 @@ __device__ inline VbLeanItem vb_lean_item(const VbProblem* __restrict__ probs, c
     d.lda = (int)pr.lda;
     d.ldb = (int)pr.ldb;
     d.ldc = (int)pr.ldc;
-    d.m0 = (w.tile / pr.tiles_n) * 64;
-    d.n0 = (w.tile % pr.tiles_n) * 64;
+    d.m0 = (w.tile / pr.tiles_n) * T;
+    d.n0 = (w.tile % pr.tiles_n) * T;
     d.nkt = (d.K + 31) / 32;
     return d;
 }
@@ -158,36 +164,34 @@ __device__ inline void vb_shift4(const float4& v, int s, float* out) {
     out[3] = v.w;
 }
 
-// Persistent walk over the work queue with the k-loop pipelined across
-// items: the last k-step of an item loads the first k-tile of the next item
-// (claimed one item ahead), so a workgroup moves from tile to tile without
-// an exposed load.  Two slots in LDS alternate between items: the slot an
-// item reads was written before the previous item's last barrier, and the
-// other slot is rewritten only after every thread has passed a barrier since
-// reading it (each item passes at least one).  Thread 0 issues the claim
-// when an item starts and publishes it before the item's last barrier.
-template <bool BF16>
-__global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* __restrict__ probs,
-                                                           const VbItem* __restrict__ items, int nitems, int* ctr,
-                                                           float alpha, float beta, const float* const* A,
-                                                           const float* const* B, float* const* C) {
-    typedef VbLean<BF16> L;
-    constexpr int S = L::S;
+// Persistent walk over a work queue of T x T items with the k-loop pipelined
+// across items: the last k-step of an item loads the first k-tile of the
+// next item (claimed one item ahead), so a workgroup moves from tile to tile
+// without an exposed load.  Two slots in LDS alternate between items: the
+// slot an item reads was written before the previous item's last barrier,
+// and the other slot is rewritten only after every thread has passed a
+// barrier since reading it (each item passes at least one).  Thread 0 issues
+// the claim when an item starts and publishes it before the item's last
+// barrier.
+template <bool BF16, int T>
+__global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_kernel(
+    const VbProblem* __restrict__ probs, const VbItem* __restrict__ items, int nitems, int* ctr, float alpha,
+    float beta, const float* const* A, const float* const* B, float* const* C) {
+    typedef VbLean<BF16, T> L;
+    constexpr int S = L::S, NQ = L::NQ, TW = L::TW, KS = L::KS;
     __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
     __shared__ int slot[2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ga = tid & 15, ka = tid >> 4;  // A: row group, k
-    const int gb = tid & 7, cb = tid >> 3;   // B: k group, column
-    const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
+    const int ga = tid % L::AG, ka = tid / L::AG;  // A: row group, k
+    const int gb = tid & 7, cb = tid >> 3;         // B: k group, column
+    const int wm0 = (wave & 1) * (T / 2), wn0 = (wave >> 1) * (T / 2);
     // Staged k-tile: va[q] holds the four elements of A from linear index
-    // r + k lda - sa[q] (r = m0 + 4ga, k = k0 + ka + 16q), vb[q] those of B
+    // r + k lda - sa[q] (r = m0 + 4ga, k = k0 + ka + KS q), vb[q] those of B
     // from kk + col ldb - sb[q] (kk = k0 + 4gb, col = n0 + cb + 32q); the
-    // store shifts them back by sa / sb and zeroes k >= K.  A group of four
-    // that would run past the operand's last element is loaded from its last
-    // four instead, so every load stays inside the operand.  The loads only
+    // store shifts them back by sa / sb and zeroes k >= K.  The loads only
     // issue here and are consumed by the store after the k-tile's MFMAs.
-    float4 va[2], vb[2];
-    int sa[2], sb[2];
+    float4 va[NQ], vb[NQ];
+    int sa[NQ], sb[NQ];
     auto load = [&](const VbLeanItem& d, int k0) {
         const int r = d.m0 + 4 * ga;
         const int kk = k0 + 4 * gb;
@@ -196,8 +200,8 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
         const int alast = (d.K - 1) * d.lda + d.m - 1, blast = (d.n - 1) * d.ldb + d.K - 1;
         if (alast >= 3 && blast >= 3) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int ia = r + (k0 + ka + 16 * q) * d.lda, ja = min(ia, alast - 3);
+            for (int q = 0; q < NQ; ++q) {
+                const int ia = r + (k0 + ka + KS * q) * d.lda, ja = min(ia, alast - 3);
                 sa[q] = ia - ja;
                 va[q] = vb_load4(d.A, 4u * (unsigned)ja);
                 const int ib = kk + min(d.n0 + cb + 32 * q, d.n - 1) * d.ldb, jb = min(ib, blast - 3);
@@ -208,10 +212,10 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
             // Operands of fewer than four elements: single elements, relaxed
             // atomic loads so that they stay apart from the 16-byte loads.
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 sa[q] = 0;
                 sb[q] = 0;
-                const int k = k0 + ka + 16 * q, col = d.n0 + cb + 32 * q;
+                const int k = k0 + ka + KS * q, col = d.n0 + cb + 32 * q;
                 float a4[4], b4[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -229,20 +233,20 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
     // neither operand's last element, so its groups need no shift and no
     // k mask.
     auto store = [&](MBH_LDS unsigned char* buf, int krem) {
-        float ra[8], rb[8];
+        float ra[4 * NQ], rb[4 * NQ];
         if (krem > 32) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 ra[4 * q] = va[q].x, ra[4 * q + 1] = va[q].y, ra[4 * q + 2] = va[q].z, ra[4 * q + 3] = va[q].w;
                 rb[4 * q] = vb[q].x, rb[4 * q + 1] = vb[q].y, rb[4 * q + 2] = vb[q].z, rb[4 * q + 3] = vb[q].w;
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 float a4[4], b4[4];
                 vb_shift4(va[q], sa[q], a4);
                 vb_shift4(vb[q], sb[q], b4);
-                const bool kva = ka + 16 * q < krem;
+                const bool kva = ka + KS * q < krem;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     ra[4 * q + e] = kva ? a4[e] : 0.0f;
@@ -252,11 +256,11 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
         }
         if constexpr (BF16) {
             MBH_LDS uint16_t* a16 = (MBH_LDS uint16_t*)buf;
-            MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + 64 * 64);
+            MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + T * 64);
             typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int k = ka + 16 * q;
+            for (int q = 0; q < NQ; ++q) {
+                const int k = ka + KS * q;
 #pragma unroll
                 for (int e = 0; e < 4; e += 2) {
                     const uint32_t w = bf16x2_rn_bits(ra[4 * q + e], ra[4 * q + e + 1]);
@@ -272,51 +276,72 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
             MBH_LDS float* as = (MBH_LDS float*)buf;
             MBH_LDS float* bs = as + 32 * S;
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
+            for (int q = 0; q < NQ; ++q)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    as[(ka + 16 * q) * S + 4 * ga + e] = ra[4 * q + e];
+                    as[(ka + KS * q) * S + 4 * ga + e] = ra[4 * q + e];
                     bs[(4 * gb + e) * S + cb + 32 * q] = rb[4 * q + e];
                 }
         }
     };
-    f32x16 acc[1][1];
+    f32x16 acc[TW][TW];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[0][0][r] = 0.0f;
+    for (int i = 0; i < TW; ++i)
+#pragma unroll
+        for (int j = 0; j < TW; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
     // The scheduling barrier keeps the staging of the next k-tile (which
     // waits for its loads) behind this k-tile's MFMAs.
     auto compute = [&](const MBH_LDS unsigned char* buf) {
         if constexpr (BF16) {
             const MBH_LDS u32x4* a4 = (const MBH_LDS u32x4*)buf;
-            const MBH_LDS u32x4* b4 = (const MBH_LDS u32x4*)(buf + 64 * 64);
-            packed_ktile_mfma<1, 1, 1>(a4, a4, b4, b4, wm0, wn0, lane, 0, 0, acc);
+            const MBH_LDS u32x4* b4 = (const MBH_LDS u32x4*)(buf + T * 64);
+            packed_ktile_mfma<1, TW, TW>(a4, a4, b4, b4, wm0, wn0, lane, 0, 0, acc);
         } else {
             const MBH_LDS float* as = (const MBH_LDS float*)buf + (lane >> 5) * S + wm0 + (lane & 31);
             const MBH_LDS float* bs = (const MBH_LDS float*)buf + 32 * S + (lane >> 5) * S + wn0 + (lane & 31);
 #pragma unroll
-            for (int kk = 0; kk < 32; kk += 2)
-                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(bs[kk * S], as[kk * S], acc[0][0], 0, 0, 0);
+            for (int kk = 0; kk < 32; kk += 2) {
+                float af[TW], bf[TW];
+#pragma unroll
+                for (int i = 0; i < TW; ++i) af[i] = as[kk * S + 32 * i];
+#pragma unroll
+                for (int j = 0; j < TW; ++j) bf[j] = bs[kk * S + 32 * j];
+#pragma unroll
+                for (int i = 0; i < TW; ++i)
+#pragma unroll
+                    for (int j = 0; j < TW; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
     };
-    // C^T accumulator: register r of lane l is C(row0 + l%32, col0 + acc_col(r, l)).
+    // C^T accumulators: register r of block (i, j), lane l is
+    // C(row0 + 32i + l%32, col0 + 32j + acc_col(r, l)).
     auto epilogue = [&](const VbLeanItem& d) {
-        const int row = d.m0 + wm0 + (lane & 31);
-        const bool rok = row < d.m;
-        const int col0 = d.n0 + wn0;
-        if (beta == 0.0f) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int col = col0 + acc_col<32>(r, lane);
-                if (rok && col < d.n) d.C[row + col * d.ldc] = alpha * acc[0][0][r];
-            }
-        } else {
+        for (int i = 0; i < TW; ++i) {
+            const int row = d.m0 + wm0 + 32 * i + (lane & 31);
+            const bool rok = row < d.m;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int col = col0 + acc_col<32>(r, lane);
-                if (rok && col < d.n) {
-                    MBH_GLOBAL float* cp = d.C + (row + col * d.ldc);
-                    *cp = alpha * acc[0][0][r] + beta * *cp;
+            for (int j = 0; j < TW; ++j) {
+                const int col0 = d.n0 + wn0 + 32 * j;
+                if (beta == 0.0f) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int col = col0 + acc_col<32>(r, lane);
+                        if (rok && col < d.n) d.C[row + col * d.ldc] = alpha * acc[i][j][r];
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int col = col0 + acc_col<32>(r, lane);
+                        if (rok && col < d.n) {
+                            MBH_GLOBAL float* cp = d.C + (row + col * d.ldc);
+                            *cp = alpha * acc[i][j][r] + beta * *cp;
+                        }
+                    }
                 }
             }
         }
@@ -331,7 +356,7 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
     __syncthreads();
     const int it0 = __builtin_amdgcn_readfirstlane(slot[0]);
     if (it0 >= nitems) return;
-    VbLeanItem cur = vb_lean_item(probs, items, it0, A, B, C);
+    VbLeanItem cur = vb_lean_item<T>(probs, items, it0, A, B, C);
     int sel = 1, buf = 0;
     if (cur.nkt > 0) {
         load(cur, 0);
@@ -345,7 +370,7 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
         int claim = 0;
         if (tid == 0 && has_next) claim = atomicAdd(ctr, 1);
         VbLeanItem nx = cur;
-        if (has_next) nx = vb_lean_item(probs, items, itn, A, B, C);
+        if (has_next) nx = vb_lean_item<T>(probs, items, itn, A, B, C);
         const bool pre = has_next && nx.nkt > 0;
         // k-steps but the last, then the last one, which loads the next
         // item's first k-tile instead (k = 0: C := beta C, no k-steps).
@@ -365,7 +390,11 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
         buf = nbuf;
         epilogue(cur);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[0][0][r] = 0.0f;
+        for (int i = 0; i < TW; ++i)
+#pragma unroll
+            for (int j = 0; j < TW; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
         if (!has_next) break;
         cur = nx;
         sel ^= 1;
@@ -377,8 +406,8 @@ __global__ __launch_bounds__(256, 4) void vb_gemm_lean_kernel(const VbProblem* _
 // and the byte offsets of the elements it loads fit in 32 bits.
 bool vb_lean_ok(const VbProblem& q) {
     const int64_t l31 = (int64_t)1 << 31, l30 = (int64_t)1 << 30;
-    return q.m < l30 && q.n < l30 && q.k < l30 && q.lda * (q.k + 64) + q.m + 64 < l31 &&
-           q.ldb * (q.n + 64) + q.k + 64 < l31 && q.ldc * (q.n + 64) + q.m + 64 < l31 &&
+    return q.m < l30 && q.n < l30 && q.k < l30 && q.lda * (q.k + 128) + q.m + 128 < l31 &&
+           q.ldb * (q.n + 128) + q.k + 128 < l31 && q.ldc * (q.n + 128) + q.m + 128 < l31 &&
            q.lda * q.k < l30 && q.ldb * q.n < l30;  // byte offsets of loaded elements fit 32 bits
 }
 
@@ -570,7 +599,10 @@ struct VbPlan {
     VbItem* items = nullptr;
     int* ctr = nullptr;
     int64_t class_count[2] = {0, 0};
-    bool lean = true;  // every problem fits the lean 64x64 tile's int32 offsets
+    bool lean = true;  // every problem fits the lean tiles' int32 offsets
+    // Lean path: items [0, nitems1) are 128x128 tiles (queue ctr[0], grid
+    // grid1), the rest 64x64 tiles (queue ctr[1], grid).
+    int nitems1 = 0, grid1 = 0;
 };
 
 }  // namespace mbh
@@ -671,7 +703,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         return MBH_E_ARG;
     }
     const bool mixed = plan->class_count[1] > 0;
-    const bool use_lean = !mixed && plan->lean;
+    const bool use_lean = plan->lean;
     // Cost of one tile of problem p (a k == 0 tile still scales C).
     auto tile_cost = [&](int64_t p) {
         const int T = vb_tile(probs[(size_t)p].cls);
@@ -682,17 +714,23 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         const int T = vb_tile(q.cls);
         return ((q.m + T - 1) / T) * ((q.n + T - 1) / T);
     };
-    std::vector<Item> items;
-    items.reserve((size_t)ntiles);
+    // Lean path: one queue per tile class (128x128 first); otherwise one
+    // queue for the two-class kernels.
+    std::vector<Item> queues[2];
     for (int64_t p = 0; p < count; ++p)
         if (probs[(size_t)p].m > 0 && probs[(size_t)p].n > 0)
-            for (int64_t t = 0; t < tiles_of(p); ++t) items.push_back({(int)p, (int)t, tile_cost(p)});
+            for (int64_t t = 0; t < tiles_of(p); ++t)
+                queues[use_lean && probs[(size_t)p].cls == 0 ? 1 : 0].push_back({(int)p, (int)t, tile_cost(p)});
     // Longest-first (the tiles of a problem stay adjacent): the dynamic
     // queue then finishes with the short items.
-    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-    std::vector<VbItem> ditems(items.size());
-    for (size_t i = 0; i < items.size(); ++i) ditems[i] = {items[i].prob, items[i].tile};
+    std::vector<VbItem> ditems;
+    ditems.reserve((size_t)ntiles);
+    for (int q = 0; q < 2; ++q) {
+        std::stable_sort(queues[q].begin(), queues[q].end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+        for (const Item& it : queues[q]) ditems.push_back({it.prob, it.tile});
+    }
     plan->nitems = (int)ditems.size();
+    plan->nitems1 = use_lean ? (int)queues[0].size() : 0;
     int dev = 0, ncu = 256;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) {
@@ -702,22 +740,30 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
     // A persistent grid never asks for more workgroups per CU than the kernel
     // it launches can keep resident (late workgroups would only find the
     // queue drained).
-    const void* kfn = nullptr;
-    if (use_lean)
-        kfn = precision == MBH_PREC_BF16 ? (const void*)vb_gemm_lean_kernel<true> : (const void*)vb_gemm_lean_kernel<false>;
-    else if (precision == MBH_PREC_BF16)
-        kfn = mixed ? (const void*)vb_gemm_bf16_kernel<1> : (const void*)vb_gemm_bf16_kernel<0>;
-    else
-        kfn = mixed ? (const void*)vb_gemm_f32_kernel<1> : (const void*)vb_gemm_f32_kernel<0>;
-    int resident = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kfn, 256, 0) == hipSuccess && resident > 0)
-        wgs = std::min(wgs, resident);
-    plan->grid = std::max(1, std::min(plan->nitems, ncu * wgs));
+    auto grid_for = [&](const void* kfn, int nit) {
+        int resident = 0, w = wgs;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kfn, 256, 0) == hipSuccess && resident > 0)
+            w = std::min(w, resident);
+        return std::min(nit, ncu * w);  // 0 for an empty queue (no launch)
+    };
+    const bool bf = precision == MBH_PREC_BF16;
+    if (use_lean) {
+        plan->grid1 = grid_for(bf ? (const void*)vb_gemm_lean_kernel<true, 128> : (const void*)vb_gemm_lean_kernel<false, 128>,
+                               plan->nitems1);
+        plan->grid = grid_for(bf ? (const void*)vb_gemm_lean_kernel<true, 64> : (const void*)vb_gemm_lean_kernel<false, 64>,
+                              plan->nitems - plan->nitems1);
+    } else if (bf) {
+        plan->grid = grid_for(mixed ? (const void*)vb_gemm_bf16_kernel<1> : (const void*)vb_gemm_bf16_kernel<0>,
+                              plan->nitems);
+    } else {
+        plan->grid = grid_for(mixed ? (const void*)vb_gemm_f32_kernel<1> : (const void*)vb_gemm_f32_kernel<0>,
+                              plan->nitems);
+    }
     auto fail = [&](hipError_t err, const char* what) {
         mbh_vbatch_plan_destroy((mbh_vbatch_plan)plan);
         return hip_fail(err, what);
     };
-    if ((e = hipMalloc(&plan->ctr, sizeof(int))) != hipSuccess) return fail(e, "vbatch plan: counter");
+    if ((e = hipMalloc(&plan->ctr, 2 * sizeof(int))) != hipSuccess) return fail(e, "vbatch plan: counters");
     if (count > 0) {
         if ((e = hipMalloc(&plan->probs, sizeof(VbProblem) * probs.size())) != hipSuccess)
             return fail(e, "vbatch plan: problems");
@@ -744,7 +790,7 @@ int mbh_vbatch_plan_info(mbh_vbatch_plan handle, double* info) {
     }
     info[0] = (double)plan->count;
     info[1] = (double)plan->nitems;
-    info[2] = (double)plan->grid;
+    info[2] = (double)(plan->grid + plan->grid1);  // workgroups launched per exec
     info[3] = plan->flops;
     info[4] = (double)plan->class_count[0];
     info[5] = (double)plan->class_count[1];
@@ -770,16 +816,29 @@ int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* co
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : hip_fail(e, "vbatched scale");
     }
-    vb_reset_kernel<<<1, 1, 0, s>>>(plan->ctr);
+    vb_reset_kernel<<<1, 64, 0, s>>>(plan->ctr);
     const dim3 grid((unsigned)plan->grid), block(256);
     const bool mixed = plan->class_count[1] > 0;
-    if (!mixed && plan->lean) {
-        if (plan->prec == MBH_PREC_BF16)
-            vb_gemm_lean_kernel<true><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
-                                                             beta, a, b, c);
-        else
-            vb_gemm_lean_kernel<false><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,
-                                                              beta, a, b, c);
+    if (plan->lean) {
+        // 128x128 items first (the longer ones), then the 64x64 queue.
+        const bool bf = plan->prec == MBH_PREC_BF16;
+        const int n1 = plan->nitems1, n0 = plan->nitems - plan->nitems1;
+        if (n1 > 0) {
+            if (bf)
+                vb_gemm_lean_kernel<true, 128><<<dim3((unsigned)plan->grid1), block, 0, s>>>(
+                    plan->probs, plan->items, n1, plan->ctr, alpha, beta, a, b, c);
+            else
+                vb_gemm_lean_kernel<false, 128><<<dim3((unsigned)plan->grid1), block, 0, s>>>(
+                    plan->probs, plan->items, n1, plan->ctr, alpha, beta, a, b, c);
+        }
+        if (n0 > 0) {
+            if (bf)
+                vb_gemm_lean_kernel<true, 64><<<grid, block, 0, s>>>(plan->probs, plan->items + n1, n0, plan->ctr + 1,
+                                                                     alpha, beta, a, b, c);
+            else
+                vb_gemm_lean_kernel<false, 64><<<grid, block, 0, s>>>(plan->probs, plan->items + n1, n0, plan->ctr + 1,
+                                                                      alpha, beta, a, b, c);
+        }
     } else if (plan->prec == MBH_PREC_BF16) {
         if (mixed)
             vb_gemm_bf16_kernel<1><<<grid, block, 0, s>>>(plan->probs, plan->items, plan->nitems, plan->ctr, alpha,

@@ -8,13 +8,13 @@
    [16, 512]) and timed with every problem on 64x64 tiles, then on 128x128
    tiles; the faster class wins the bucket.
 2. Wave schedule: the 4096-problem config batch (m, n, k log-uniform in
-   [16, 512]) is timed with the winning table, with an all-64x64 table
-   (which selects the 64x64-only persistent kernel, lighter on registers and
-   LDS, so more workgroups stay resident) and with threshold tables
-   (128x128 tiles once both m and n exceed 64, or 128), each with 1..8
-   persistent workgroups per CU; the fastest (table, count) pair wins.  The
-   per-bucket step times each bucket alone, so it cannot see how classes
-   share the GPU inside one mixed batch; the threshold tables cover that.
+   [16, 512]) is timed with the winning table, with all-64x64 and
+   all-128x128 tables and with threshold tables (128x128 tiles once both m
+   and n exceed 64, or 128), each with 1..8 persistent workgroups per CU
+   (capped per tile class at what the kernel keeps resident); the fastest
+   (table, count) pair wins.  The per-bucket step times each bucket alone,
+   so it cannot see how classes share the GPU inside one mixed batch; the
+   whole-batch tables cover that.
 Both steps run for fp32 and for bf16-input mode.  Every measurement is
 appended to DIR/vbatched_tuning.jsonl (one JSON record per line: precision,
 bucket, class or workgroups, median ms) and the winners are written as
@@ -112,7 +112,7 @@ def main():
         gt128 = np.zeros((6, 6), np.int32)
         gt128[4:, 4:] = 1
         for name, cand in (("tuned", table), ("all64", np.zeros((6, 6), np.int32)), ("mn>64", gt64),
-                           ("mn>128", gt128)):
+                           ("mn>128", gt128), ("all128", np.ones((6, 6), np.int32))):
             for w in (1, 2, 3, 4, 5, 6, 8):
                 batched.set_tuning(cand, w, prec)
                 ms, flops = run_batch(torch, batched, m, n, k, prec, args.reps, gen)

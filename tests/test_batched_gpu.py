@@ -53,9 +53,13 @@ def _check_problems(prec, m, n, k, bufs, offs, alpha, beta, c0, which, sample=No
         assert (err <= bound).all(), (prec, p, mp, np_, kp, float((err / bound).max()))
 
 
-# Tile-class tables: "all64" selects the lean 64x64-only persistent kernel,
-# "all128" the two-class kernel; "default" is the compiled (tuned) table.
-TABLES = {"default": None, "all64": np.zeros((6, 6), np.int32), "all128": np.ones((6, 6), np.int32)}
+# Tile-class tables: "all64" puts every problem on 64x64 tiles, "all128" on
+# 128x128 tiles, "mixed" 128x128 once both m and n exceed 64 (two queues);
+# "default" is the compiled (tuned) table.
+_MIXED = np.zeros((6, 6), np.int32)
+_MIXED[3:, 3:] = 1
+TABLES = {"default": None, "all64": np.zeros((6, 6), np.int32), "all128": np.ones((6, 6), np.int32),
+          "mixed": _MIXED}
 
 
 @pytest.fixture
@@ -92,7 +96,7 @@ def test_ragged_batch_vs_oracle(prec, table):
     plan.close()
 
 
-@pytest.mark.parametrize("prec,table", _cases(("default", "all64")), indirect=["table"])
+@pytest.mark.parametrize("prec,table", _cases(("default", "all64", "mixed")), indirect=["table"])
 def test_config4_4096_problems(prec, table):
     """Config size: 4096 N,N problems, m, n, k log-uniform in [16, 512],
     U[-1,1], alpha=1, beta=0, ld = rows.  Every problem is priced on 8 sampled
@@ -140,7 +144,7 @@ def test_graph_capture_replays_match_eager(prec):
     plan.close()
 
 
-@pytest.mark.parametrize("prec,table", _cases(("all64", "all128")), indirect=["table"])
+@pytest.mark.parametrize("prec,table", _cases(("all64", "all128", "mixed")), indirect=["table"])
 def test_padded_leading_dimensions(prec, table):
     """lda > m, ldb > k, ldc > m with odd paddings (columns off 16-byte
     alignment); the padding rows of C must stay untouched."""
