@@ -277,12 +277,13 @@ hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double
     p.a_vec = vec_ok(A, lda);
     p.b_vec = vec_ok(B, ldb);
     // 1 / 2: one 128-row tile row / 128-column tile column (in-place leaf
-    // products of the triangular solves), 32-deep k-tiles.  0: 64 x 64, which
-    // measured faster than 128 x 64 and 128 x 128 for the 4096^2
-    // refinement residual.
+    // products of the triangular solves), 32-deep k-tiles.  0: 64 x 64 with
+    // 32-deep k-tiles, which measured faster than 64 x 64 x 16, 128 x 64 and
+    // 128 x 128 for the 4096^2 refinement residual (the fp64 dgemm keeps
+    // 64 x 64 x 16, faster there).
     if (shape == 1) return launch_fp<double, TA, TB, TC, 128, 32, 32>(at, bt, p, 0, s);
     if (shape == 2) return launch_fp<double, TA, TB, TC, 32, 128, 32>(at, bt, p, 0, s);
-    return launch_fp<double, TA, TB, TC, 64, 64, 16>(at, bt, p, 0, s);
+    return launch_fp<double, TA, TB, TC, 64, 64, 32>(at, bt, p, 0, s);
 }
 
 #define MBH_F64ACC(TA, TB, TC)                                                                                     \
