@@ -132,6 +132,76 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, const
     }
 }
 
+// 'T' with 16-byte loads (A 16-byte aligned, lda a multiple of V): each
+// lane reads V consecutive rows of its wave's four columns per load, four
+// such groups in flight; x is read with 16-byte loads too when it is
+// contiguous and aligned (xvec).  Rows past the last whole 256 V block take
+// single loads.
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_tv_kernel(int64_t m, int64_t n, const T* __restrict__ A, int64_t lda,
+                                                      const T* __restrict__ x, int64_t incx, int64_t chunk, T alpha,
+                                                      T beta, T* y, int64_t incy, T* part, int xvec) {
+    constexpr int V = 16 / sizeof(T);
+    typedef T vT __attribute__((ext_vector_type(V)));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t jb = ((int64_t)blockIdx.x * 4 + wave) * GT_COLS;
+    if (jb >= n) return;  // whole wave
+    const int nc = (n - jb < GT_COLS) ? (int)(n - jb) : GT_COLS;
+    const int64_t i0 = (int64_t)blockIdx.y * chunk, i1 = (i0 + chunk < m) ? i0 + chunk : m;
+    const MBH_GLOBAL T* a = (const MBH_GLOBAL T*)A + jb * lda;
+    const MBH_GLOBAL T* xg = (const MBH_GLOBAL T*)x;
+    // Column c of the wave reads column min(c, nc-1) (in range) and drops the
+    // result when c >= nc, so the loads carry no branches.
+    int64_t coff[GT_COLS];
+#pragma unroll
+    for (int c = 0; c < GT_COLS; ++c) coff[c] = (int64_t)(c < nc ? c : nc - 1) * lda;
+    vT accv[GT_COLS];
+#pragma unroll
+    for (int c = 0; c < GT_COLS; ++c) accv[c] = vT(T(0));
+    int64_t base = i0;
+    for (; base + 256 * V <= i1; base += 256 * V) {  // four groups of V rows per lane in flight
+        vT xv[4], av[4][GT_COLS];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t i = base + V * lane + 64 * V * u;
+            if (xvec) {
+                xv[u] = *(const MBH_GLOBAL vT*)(xg + i);
+            } else {
+#pragma unroll
+                for (int e = 0; e < V; ++e) xv[u][e] = xg[vidx(i + e, m, incx)];
+            }
+#pragma unroll
+            for (int c = 0; c < GT_COLS; ++c) av[u][c] = *(const MBH_GLOBAL vT*)(a + coff[c] + i);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < GT_COLS; ++c) accv[c] += av[u][c] * xv[u];
+    }
+    T acc[GT_COLS];
+#pragma unroll
+    for (int c = 0; c < GT_COLS; ++c) {
+        acc[c] = T(0);
+#pragma unroll
+        for (int e = 0; e < V; ++e) acc[c] += accv[c][e];
+    }
+    for (int64_t i = base + lane; i < i1; i += 64) {
+        const T xi = xg[vidx(i, m, incx)];
+#pragma unroll
+        for (int c = 0; c < GT_COLS; ++c) acc[c] += a[coff[c] + i] * xi;
+    }
+#pragma unroll
+    for (int c = 0; c < GT_COLS; ++c) acc[c] = wave_sum(acc[c]);
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < GT_COLS; ++c) {
+            if (c >= nc) break;
+            if (part) part[(int64_t)blockIdx.y * n + jb + c] = acc[c];
+            else gv_store(y, jb + c, n, incy, alpha, beta, acc[c]);
+        }
+    }
+}
+
 template <typename T>
 __global__ void gemv_reduce_kernel(int64_t len, int splits, const T* __restrict__ part, T alpha, T beta, T* y,
                                    int64_t incy) {
@@ -286,6 +356,11 @@ hipError_t gemv(bool trans, int64_t m, int64_t n, T alpha, const T* A, int64_t l
         vec_scale_kernel<T><<<grid1(leny), 256, 0, s>>>(leny, beta, y, incy);
         return hipGetLastError();
     }
+    // 'T': 16-byte loads of A when its columns start 16-byte aligned ('N'
+    // keeps single loads along its rows: measured faster there).
+    constexpr int V = 16 / sizeof(T);
+    const bool avec = trans && ((uintptr_t)A % 16 == 0) && (lda % V == 0);
+    const int rv = avec ? V : 1;  // rows per load
     int64_t splits, chunk, groups;
     if (!trans) {
         groups = (m + GV_NT - 1) / GV_NT;
@@ -296,7 +371,7 @@ hipError_t gemv(bool trans, int64_t m, int64_t n, T alpha, const T* A, int64_t l
         groups = (n + 4 * GT_COLS - 1) / (4 * GT_COLS);
         splits = std::max<int64_t>(1, std::min((GV_TARGET_WGS + groups - 1) / groups, (m + 255) / 256));
         chunk = (m + splits - 1) / splits;
-        chunk = (chunk + 255) / 256 * 256;
+        chunk = (chunk + 256 * rv - 1) / (256 * rv) * (256 * rv);
     }
     splits = (((trans ? m : n) + chunk - 1) / chunk);
     if (groups > INT_MAX || splits > 65535) return hipErrorInvalidValue;
@@ -308,8 +383,15 @@ hipError_t gemv(bool trans, int64_t m, int64_t n, T alpha, const T* A, int64_t l
         part = (T*)ws;
     }
     const dim3 grid((unsigned)groups, (unsigned)splits);
-    if (!trans) gemv_n_kernel<T><<<grid, GV_NT, 0, s>>>(m, n, A, lda, x, incx, chunk, alpha, beta, y, incy, part);
-    else gemv_t_kernel<T><<<grid, 256, 0, s>>>(m, n, A, lda, x, incx, chunk, alpha, beta, y, incy, part);
+    if (!trans) {
+        gemv_n_kernel<T><<<grid, GV_NT, 0, s>>>(m, n, A, lda, x, incx, chunk, alpha, beta, y, incy, part);
+    } else {
+        const int xvec = incx == 1 && (uintptr_t)x % 16 == 0;
+        if (avec)
+            gemv_tv_kernel<T><<<grid, 256, 0, s>>>(m, n, A, lda, x, incx, chunk, alpha, beta, y, incy, part, xvec);
+        else
+            gemv_t_kernel<T><<<grid, 256, 0, s>>>(m, n, A, lda, x, incx, chunk, alpha, beta, y, incy, part);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || splits == 1) return e;
     gemv_reduce_kernel<T><<<grid1(leny), 256, 0, s>>>(leny, (int)splits, part, alpha, beta, y, incy);
