@@ -147,10 +147,11 @@ namespace mbh {
 // k, one byte per element, k fastest, zero padding), scales as E8M0 bytes
 // (exponent + 127) in a [R/128][K/128] int array: x ~= 2^(e) * q.
 // One workgroup holds its whole 128x128 fp32 tile in registers (64 values
-// per thread, each thread's share being 16 consecutive k of one or four
-// rows), reduces the tile's amax (wave shuffles, then four partials in LDS),
-// picks e = ceil(log2(amax / 448)) and writes its rows' 16-byte runs of
-// quantized k straight from registers.
+// per thread: 4 consecutive k of 16 rows when k is contiguous in HBM, 16
+// consecutive k of 4 rows otherwise, so that every 16-byte load of a wave
+// covers whole 512-byte runs), reduces the tile's amax (wave shuffles, then
+// four partials in LDS), picks e = ceil(log2(amax / 448)) and writes the
+// quantized k-runs (4 or 16 bytes per row) straight from registers.
 struct PackFp8Params {
     const float* X;
     int64_t ld;
@@ -175,25 +176,26 @@ __global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params p) {
     const int tid = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.x * T, k0 = (int64_t)blockIdx.y * T;
     const bool full = p.vec && r0 + T <= p.R && k0 + T <= p.K;  // no edge: unguarded 16-byte loads
-    // KCONTIG ((r, k) at X[k + r ld]): v[j] = X(gr, gk0 + 4j .. 4j + 3), one
-    //   row gr = r0 + tid/2 and 64 consecutive k from gk0 = k0 + 64 (tid%2);
+    // KCONTIG ((r, k) at X[k + r ld]): v[j] = X(gr0 + 8j, gk0 .. gk0 + 3),
+    //   16 rows from gr0 = r0 + tid/32 (stride 8) at the 4 consecutive k from
+    //   gk0 = k0 + 4 (tid%32): a wave reads two whole 128-k rows per load;
     // else ((r, k) at X[r + k ld]): v[j] = X(gr0 .. gr0 + 3, gk0 + j), four
     //   rows from gr0 = r0 + 4 (tid%32) and 16 consecutive k from
     //   gk0 = k0 + 16 (tid/32).
     f32x4 v[16];
-    const int64_t gr0 = KCONTIG ? r0 + tid / 2 : r0 + 4 * (tid % 32);
-    const int64_t gk0 = KCONTIG ? k0 + 64 * (tid % 2) : k0 + 16 * (tid / 32);
+    const int64_t gr0 = KCONTIG ? r0 + tid / 32 : r0 + 4 * (tid % 32);
+    const int64_t gk0 = KCONTIG ? k0 + 4 * (tid % 32) : k0 + 16 * (tid / 32);
     if (full) {
 #pragma unroll
         for (int j = 0; j < 16; ++j)
-            v[j] = KCONTIG ? *reinterpret_cast<const f32x4*>(p.X + gk0 + 4 * j + gr0 * p.ld)
+            v[j] = KCONTIG ? *reinterpret_cast<const f32x4*>(p.X + gk0 + (gr0 + 8 * j) * p.ld)
                            : *reinterpret_cast<const f32x4*>(p.X + gr0 + (gk0 + j) * p.ld);
     } else {
 #pragma unroll
         for (int j = 0; j < 16; ++j)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t gr = KCONTIG ? gr0 : gr0 + e, gk = KCONTIG ? gk0 + 4 * j + e : gk0 + j;
+                const int64_t gr = KCONTIG ? gr0 + 8 * j : gr0 + e, gk = KCONTIG ? gk0 + e : gk0 + j;
                 v[j][e] = (gr < p.R && gk < p.K) ? p.X[KCONTIG ? gk + gr * p.ld : gr + gk * p.ld] : 0.0f;
             }
     }
@@ -217,20 +219,15 @@ __global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params p) {
     }
     const float inv = ldexpf(1.0f, -e);
     if (tid == 0) p.scale[(r0 / T) * p.kblocks + k0 / T] = e + 127;
-    // Packed byte offset of element (gr, gk), gk a multiple of 16.
+    // Packed byte offset of element (gr, gk), gk a multiple of 4.
     auto dst = [&](int64_t gr, int64_t gk) {
-        return reinterpret_cast<uint4*>(p.q + (((gr / PR) * p.ktiles + gk / PK) * PR + gr % PR) * PK + gk % PK);
+        return p.q + (((gr / PR) * p.ktiles + gk / PK) * PR + gr % PR) * PK + gk % PK;
     };
-    if (KCONTIG) {
+    if (KCONTIG) {  // 4 quantized k per row: one dword, a wave writing whole 64-byte runs
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            uint4 w;
-            w.x = fp8_pack4(v[4 * c][0] * inv, v[4 * c][1] * inv, v[4 * c][2] * inv, v[4 * c][3] * inv);
-            w.y = fp8_pack4(v[4 * c + 1][0] * inv, v[4 * c + 1][1] * inv, v[4 * c + 1][2] * inv, v[4 * c + 1][3] * inv);
-            w.z = fp8_pack4(v[4 * c + 2][0] * inv, v[4 * c + 2][1] * inv, v[4 * c + 2][2] * inv, v[4 * c + 2][3] * inv);
-            w.w = fp8_pack4(v[4 * c + 3][0] * inv, v[4 * c + 3][1] * inv, v[4 * c + 3][2] * inv, v[4 * c + 3][3] * inv);
-            *dst(gr0, gk0 + 16 * c) = w;
-        }
+        for (int j = 0; j < 16; ++j)
+            *reinterpret_cast<uint32_t*>(dst(gr0 + 8 * j, gk0)) =
+                fp8_pack4(v[j][0] * inv, v[j][1] * inv, v[j][2] * inv, v[j][3] * inv);
     } else {
 #pragma unroll
         for (int e4 = 0; e4 < 4; ++e4) {
@@ -239,7 +236,7 @@ __global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params p) {
             w.y = fp8_pack4(v[4][e4] * inv, v[5][e4] * inv, v[6][e4] * inv, v[7][e4] * inv);
             w.z = fp8_pack4(v[8][e4] * inv, v[9][e4] * inv, v[10][e4] * inv, v[11][e4] * inv);
             w.w = fp8_pack4(v[12][e4] * inv, v[13][e4] * inv, v[14][e4] * inv, v[15][e4] * inv);
-            *dst(gr0 + e4, gk0) = w;
+            *reinterpret_cast<uint4*>(dst(gr0 + e4, gk0)) = w;
         }
     }
 }
