@@ -291,9 +291,12 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
         for (int j = 0; j < TW; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    // nk: k values of the tile below K.  fp32 mode runs only the 2-deep
+    // MFMA steps that hold such k (the rows past K are zero in both
+    // operands, so the sum is unchanged); bf16 mode runs the whole tile.
     // The scheduling barrier keeps the staging of the next k-tile (which
     // waits for its loads) behind this k-tile's MFMAs.
-    auto compute = [&](const MBH_LDS unsigned char* buf) {
+    auto compute = [&](const MBH_LDS unsigned char* buf, int nk) {
         if constexpr (BF16) {
             const MBH_LDS u32x4* a4 = (const MBH_LDS u32x4*)buf;
             const MBH_LDS u32x4* b4 = (const MBH_LDS u32x4*)(buf + T * 64);
@@ -301,8 +304,7 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
         } else {
             const MBH_LDS float* as = (const MBH_LDS float*)buf + (lane >> 5) * S + wm0 + (lane & 31);
             const MBH_LDS float* bs = (const MBH_LDS float*)buf + 32 * S + (lane >> 5) * S + wn0 + (lane & 31);
-#pragma unroll
-            for (int kk = 0; kk < 32; kk += 2) {
+            auto kstep = [&](int kk) {
                 float af[TW], bf[TW];
 #pragma unroll
                 for (int i = 0; i < TW; ++i) af[i] = as[kk * S + 32 * i];
@@ -313,6 +315,12 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
 #pragma unroll
                     for (int j = 0; j < TW; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+            };
+            if (nk >= 32) {
+#pragma unroll
+                for (int kk = 0; kk < 32; kk += 2) kstep(kk);
+            } else {
+                for (int kk = 0; kk < nk; kk += 2) kstep(kk);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -376,13 +384,13 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
         // item's first k-tile instead (k = 0: C := beta C, no k-steps).
         for (int kt = 0; kt + 1 < cur.nkt; ++kt) {
             load(cur, (kt + 1) * 32);
-            compute(lbuf(buf));
+            compute(lbuf(buf), 32);
             store(lbuf(buf ^ 1), cur.K - (kt + 1) * 32);
             __syncthreads();
             buf ^= 1;
         }
         if (pre) load(nx, 0);
-        if (cur.nkt > 0) compute(lbuf(buf));
+        if (cur.nkt > 0) compute(lbuf(buf), cur.K - (cur.nkt - 1) * 32);
         const int nbuf = cur.nkt > 0 ? buf ^ 1 : buf;
         if (pre) store(lbuf(nbuf), nx.K);
         if (tid == 0 && has_next) slot[sel ^ 1] = claim;
