@@ -29,6 +29,7 @@
 #ifndef MOJOBLAS_HIP_H
 #define MOJOBLAS_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -250,6 +251,36 @@ int mbh_strsm_backward(char side, char uplo, char transa, char diag, int64_t m, 
                        float alpha, const float* a, int64_t lda, const float* x, int64_t ldx,
                        const float* dx, int64_t lddx, float* db, int64_t lddb, float* da,
                        int64_t ldda, int precision, void* stream);
+
+/* ---- device-resident buffers (no counterpart in the reference) ----------
+ * The reference's functions take host pointers (docs/reference.md:264-327),
+ * so every _host call moves its operands over PCIe.  These entries let a
+ * caller without its own HIP bindings (a Mojo program) keep operands in HBM
+ * across calls and pass them to the _dev entries.
+ *  dev_alloc / dev_free   : device memory (hipMalloc / hipFree); bytes == 0
+ *                           gives NULL; freeing NULL is a no-op.
+ *  host_alloc / host_free : pinned host memory for fast copies.
+ *  set_matrix / get_matrix: copy a rows x cols column-major matrix of
+ *                           elem_size-byte elements between host (ldh) and
+ *                           device (ldd), ordered on `stream` (NULL = the
+ *                           default stream); the destination is complete
+ *                           after stream_sync(stream).  MBH_E_ARG for
+ *                           rows/cols < 0, elem_size outside [1, 16], an ld
+ *                           below max(1, rows) or a NULL buffer.
+ *  stream_create / _destroy / _sync: a non-blocking HIP stream at the
+ *                           default priority (each stream used with the
+ *                           _dev entries keeps its own scratch workspace). */
+int mbh_dev_alloc(size_t bytes, void** ptr);
+int mbh_dev_free(void* ptr);
+int mbh_host_alloc(size_t bytes, void** ptr);
+int mbh_host_free(void* ptr);
+int mbh_set_matrix(int64_t rows, int64_t cols, int64_t elem_size, const void* host, int64_t ldh, void* dev,
+                   int64_t ldd, void* stream);
+int mbh_get_matrix(int64_t rows, int64_t cols, int64_t elem_size, const void* dev, int64_t ldd, void* host,
+                   int64_t ldh, void* stream);
+int mbh_stream_create(void** stream);
+int mbh_stream_destroy(void* stream);
+int mbh_stream_sync(void* stream);
 
 #ifdef __cplusplus
 }

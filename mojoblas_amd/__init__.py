@@ -19,6 +19,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from .device import DeviceMatrix, Stream
 
 PREC_FP32 = 0
 PREC_BF16X3 = 1
@@ -48,6 +49,8 @@ def _ptr_and_kind(x):
     """(address, 'dev'|'host', dtype-name) of an operand buffer."""
     if x is None:
         return None, None, None
+    if isinstance(x, DeviceMatrix):
+        return x.ptr, "dev", str(x.dtype)
     if _is_torch(x):
         if not x.is_cuda:
             raise TypeError("torch operands must be CUDA tensors; pass numpy arrays for host memory")

@@ -71,6 +71,15 @@ SIGNATURES = {
     "mbh_sgemm_backward": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, i32, vp], i32),
     "mbh_strsm_backward": ([ch, ch, ch, ch, i64, i64, f32, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, i32, vp],
                            i32),
+    "mbh_dev_alloc": ([ctypes.c_size_t, ctypes.POINTER(vp)], i32),
+    "mbh_dev_free": ([vp], i32),
+    "mbh_host_alloc": ([ctypes.c_size_t, ctypes.POINTER(vp)], i32),
+    "mbh_host_free": ([vp], i32),
+    "mbh_set_matrix": ([i64, i64, i64, vp, i64, vp, i64, vp], i32),
+    "mbh_get_matrix": ([i64, i64, i64, vp, i64, vp, i64, vp], i32),
+    "mbh_stream_create": ([ctypes.POINTER(vp)], i32),
+    "mbh_stream_destroy": ([vp], i32),
+    "mbh_stream_sync": ([vp], i32),
 }
 
 

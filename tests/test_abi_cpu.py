@@ -169,3 +169,39 @@ def test_level2_python_mirror_prints_info(capsys):
     assert "gemv: Info 1" in capsys.readouterr().out
     assert mb.trsv("U", "N", "N", 2, a, 1, a.copy(), 1) == 6
     assert "trsv: Info 6" in capsys.readouterr().out
+
+
+# ---- device-resident buffer entries: argument checks run before any HIP call
+
+E_ARG = -4
+
+
+def test_buffer_entries_validate_without_gpu():
+    L = _lib.lib()
+    p = ctypes.c_void_p(123)
+    assert L.mbh_dev_alloc(0, ctypes.byref(p)) == 0 and p.value is None  # zero bytes: NULL, no allocation
+    assert L.mbh_host_alloc(0, ctypes.byref(p)) == 0 and p.value is None
+    assert L.mbh_dev_alloc(16, None) == E_ARG
+    assert L.mbh_host_alloc(16, None) == E_ARG
+    assert L.mbh_stream_create(None) == E_ARG
+    assert L.mbh_dev_free(None) == 0 and L.mbh_host_free(None) == 0 and L.mbh_stream_destroy(None) == 0
+    host = np.zeros(16, np.float32)
+    hp, dp = host.ctypes.data, ctypes.c_void_p(0x1000)  # dp is never dereferenced: every case fails validation
+    for fn in (L.mbh_set_matrix, L.mbh_get_matrix):
+        assert fn(-1, 2, 4, hp, 4, dp, 4, None) == E_ARG          # negative rows
+        assert fn(2, -1, 4, hp, 4, dp, 4, None) == E_ARG          # negative cols
+        assert fn(2, 2, 0, hp, 4, dp, 4, None) == E_ARG           # element size
+        assert fn(2, 2, 17, hp, 4, dp, 4, None) == E_ARG
+        assert fn(4, 2, 4, hp, 3, dp, 4, None) == E_ARG           # source ld below rows
+        assert fn(4, 2, 4, hp, 4, dp, 3, None) == E_ARG           # destination ld below rows
+        assert fn(4, 2, 4, None, 4, dp, 4, None) == E_ARG         # null buffer
+        assert fn(0, 2, 4, None, 1, None, 1, None) == 0           # empty: quick return
+    assert L.mbh_set_matrix(4, 2, 4, hp, 3, dp, 4, None) == E_ARG
+    assert "leading dimensions" in _lib.last_error()
+
+
+def test_device_matrix_rejects_bad_shapes_without_gpu():
+    with pytest.raises(TypeError):
+        mb.DeviceMatrix(2, 2, np.int32)
+    with pytest.raises(ValueError):
+        mb.DeviceMatrix(4, 2, np.float32, ld=3)
