@@ -87,6 +87,29 @@ int mbh_dgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, dou
                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                   double* c, int64_t ldc, void* stream);
 
+/* ---- 16-bit gemm: C := alpha*op(A)*op(B) + beta*C on half (h) or bfloat16
+ * (b) data.  Replaces gemm[DType.float16] / gemm[DType.bfloat16]
+ * (src/level3/gemm.mojo:56-124, whose non-f32 instantiations run the
+ * generic v3 / v7 / naive kernels); info codes as for sgemm.  Elements are
+ * passed as their 16-bit patterns (the Mojo shim bitcasts its
+ * UnsafePointer[Float16] / [BFloat16]); alpha and beta as float (the shim
+ * passes Float32(alpha), exact for every 16-bit value).  Products run on the
+ * 16-bit MFMA (v_mfma_f32_32x32x16_f16 / _bf16) with fp32 accumulation;
+ * alpha*acc + beta*C is formed in fp32 and rounded once, to nearest even,
+ * into C's format. */
+int mbh_hgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                   const uint16_t* a, int64_t lda, const uint16_t* b, int64_t ldb, float beta,
+                   uint16_t* c, int64_t ldc);
+int mbh_bgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                   const uint16_t* a, int64_t lda, const uint16_t* b, int64_t ldb, float beta,
+                   uint16_t* c, int64_t ldc);
+int mbh_hgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                  const uint16_t* a, int64_t lda, const uint16_t* b, int64_t ldb, float beta,
+                  uint16_t* c, int64_t ldc, void* stream);
+int mbh_bgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                  const uint16_t* a, int64_t lda, const uint16_t* b, int64_t ldb, float beta,
+                  uint16_t* c, int64_t ldc, void* stream);
+
 /* ---- trsm: B := alpha*inv(op(A))*B (side L) or alpha*B*inv(op(A)) (side R)
  * Replaces trsm[dtype] (src/level3/trsm.mojo:20-275; info codes :62-89:
  * 1,2,3,4,5,6,9,11).  BLAS semantics, including side R with trans T (the

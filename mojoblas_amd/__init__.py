@@ -88,7 +88,7 @@ def _finish(name, rc):
     return rc
 
 
-def _common(*xs):
+def _common(*xs, allow16=False):
     kinds = set()
     dtypes = set()
     for x in xs:
@@ -102,8 +102,10 @@ def _common(*xs):
         raise TypeError("operands mix dtypes %s" % sorted(dtypes))
     kind = kinds.pop() if kinds else "host"
     dt = dtypes.pop() if dtypes else "float32"
+    if dt in ("float16", "bfloat16") and allow16:
+        return kind, dt
     if dt not in ("float32", "float64"):
-        raise TypeError("dtype %s not supported (float32, float64)" % dt)
+        raise TypeError("dtype %s not supported (float32, float64%s)" % (dt, ", float16, bfloat16" if allow16 else ""))
     return kind, dt
 
 
@@ -115,12 +117,21 @@ def gemm(trans_a, trans_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, precisi
     """C := alpha*op(A)*op(B) + beta*C (src/level3/gemm.mojo:56-124).
 
     precision: "fp32" (default), "bf16x3" or "fp8" for float32 data; float64
-    data always runs the fp64 matrix-core kernel.
+    data always runs the fp64 matrix-core kernel.  float16 / bfloat16 data
+    (torch CUDA tensors, or numpy float16 host arrays) runs the native 16-bit
+    matrix-core kernel: fp32 accumulation, C rounded once to its format;
+    `precision` must be None there.
     """
     L = _lib.lib()
-    kind, dt = _common(a, b, c)
+    kind, dt = _common(a, b, c, allow16=True)
     ta, tb = _flag(trans_a), _flag(trans_b)
-    if dt == "float32":
+    if dt in ("float16", "bfloat16"):
+        if precision is not None:
+            raise ValueError("precision applies to float32 data only")
+        fn = getattr(L, "mbh_%sgemm_%s" % ("h" if dt == "float16" else "b", kind))
+        args = [ta, tb, m, n, k, float(alpha), _p(a), lda, _p(b), ldb, float(beta), _p(c), ldc]
+        rc = fn(*(args + [_stream(stream)] if kind == "dev" else args))
+    elif dt == "float32":
         if kind == "dev":
             rc = L.mbh_sgemm_dev(ta, tb, m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc, _prec(precision),
                                  _stream(stream))

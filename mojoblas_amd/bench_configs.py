@@ -16,6 +16,10 @@
   l3     the level-3 routines without a sized workload of their own: ssymm
          L,L m=n=8192, strmm L,L,N,N m=n=4096, ssyr2k L,N n=8192 k=2048
   dgemm  DGEMM N,N 8192^3 on the fp64 MFMA (alpha=1, beta=0)
+  gemm16 native 16-bit gemm 8192^3 (hgemm f16, bgemm bf16; N,N and T,N):
+         step (pack + main kernel) and main kernel alone (library event
+         timing), error against the fp64 product of the same 16-bit
+         inputs, and torch.mm on the same tensors (hipBLASLt) as a yardstick
   l2     level-2: sgemv N and T at 16384 x 16384 (1 GiB matrix) and dgemv N at
          8192 x 8192 (HBM-bound: algorithmic bytes / time against 8 TB/s);
          strsv L,N,N at n = 4096 and 16384, dtrsv at 4096
@@ -325,6 +329,47 @@ def dgemm(torch, mb, reps):
           "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK["fp64"], "diff_vs_torch_fp64": e})
 
 
+def gemm16(torch, mb, reps):
+    import ctypes
+
+    from . import _lib
+
+    L = _lib.lib()
+    n = 8192
+    g = torch.Generator(device="cuda").manual_seed(12)
+    fl = 2.0 * n ** 3
+    for name, dt in (("hgemm", torch.float16), ("bgemm", torch.bfloat16)):
+        A = (torch.rand(n * n, device="cuda", generator=g) * 2 - 1).to(dt)
+        B = (torch.rand(n * n, device="cuda", generator=g) * 2 - 1).to(dt)
+        C = torch.empty(n * n, device="cuda", dtype=dt)
+        for ta in ("N", "T"):
+            run = lambda: mb.gemm(ta, "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n)
+            tm = np.zeros(4)
+            for _ in range(2):
+                run()
+            torch.cuda.synchronize()
+            L.mbh_timing_enable(1)
+            L.mbh_timing_read(ctypes.c_void_p(tm.ctypes.data), 1)
+            ms = timed(torch, run, reps, warmup=0)
+            L.mbh_timing_read(ctypes.c_void_p(tm.ctypes.data), 1)
+            L.mbh_timing_enable(0)
+            kms, pms = tm[0] / max(tm[1], 1), tm[2] / max(tm[3], 1)
+            # Column-major X is the row-major view of X^T: C^T = B^T op(A)^T.
+            a_rm, b_rm = A.view(n, n), B.view(n, n)
+            opa_t = a_rm if ta == "N" else a_rm.t().contiguous()
+            ref = (b_rm.double() @ opa_t.double()).t()
+            e = err_stats(torch, C.view(n, n).t().double(), ref)
+            out = torch.empty(n, n, device="cuda", dtype=dt)
+            tms = timed(torch, lambda: torch.mm(b_rm, opa_t, out=out), reps)
+            emit({"config": "gemm16", "routine": "%s %sN 8192^3" % (name, ta), "precision": str(dt)[6:],
+                  "ms": ms, "tflops": fl / ms / 1e9, "kernel_ms": kms, "pack_ms": pms,
+                  "kernel_tflops": fl / kms / 1e9 if kms else None,
+                  "roofline_frac": fl / kms / 1e9 / PEAK["bf16"] if kms else None,
+                  "error_vs_fp64": e, "torch_mm_ms": tms, "torch_mm_tflops": fl / tms / 1e9})
+            del ref, out
+        del A, B, C
+
+
 def l2(torch, mb, reps):
     g = torch.Generator(device="cuda").manual_seed(10)
     for dt, n in ((torch.float32, 16384), (torch.float64, 8192)):
@@ -365,7 +410,7 @@ def l2(torch, mb, reps):
 
 
 CONFIGS = {"cfg1": cfg1, "cfg2b": cfg2b, "cfg2h": cfg2h, "cfg3a": cfg3a, "cfg3b": cfg3b, "cfg4": cfg4, "cfg5": cfg5,
-           "l3": l3, "dgemm": dgemm, "l2": l2}
+           "l3": l3, "dgemm": dgemm, "gemm16": gemm16, "l2": l2}
 
 
 def main():

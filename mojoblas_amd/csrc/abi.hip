@@ -190,6 +190,46 @@ static int gemm_host(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha,
     return 0;
 }
 
+// 16-bit (f16 / bf16) gemm on host buffers: same staging as gemm_host.
+static int gemm16_host(bool f16, char ta, char tb, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* a,
+                       int64_t lda, const uint16_t* b, int64_t ldb, float beta, uint16_t* c, int64_t ldc) {
+    int info = check_gemm(ta, tb, m, n, k, lda, ldb, ldc);
+    if (info) return info;
+    if (m == 0 || n == 0) return 0;
+    const bool at = !flag_n(ta), bt = !flag_n(tb);
+    const bool need_ab = !(alpha == 0.0f || k == 0);
+    const int64_t ar = at ? k : m, ac = at ? m : k, br = bt ? n : k, bc = bt ? k : n;
+    auto lk = stage_lock();
+    void *dA = nullptr, *dB = nullptr, *dC = nullptr;
+    hipError_t e;
+    const size_t es = sizeof(uint16_t);
+    if ((e = slot_get(2, (size_t)m * n * es, &dC)) != hipSuccess) return hip_fail(e, "hipMalloc(C)");
+    if (need_ab) {
+        if ((e = slot_get(0, (size_t)ar * ac * es, &dA)) != hipSuccess) return hip_fail(e, "hipMalloc(A)");
+        if ((e = slot_get(1, (size_t)br * bc * es, &dB)) != hipSuccess) return hip_fail(e, "hipMalloc(B)");
+        if ((e = put_matrix(dA, a, ar, ac, lda, es)) != hipSuccess) return hip_fail(e, "copy A");
+        if ((e = put_matrix(dB, b, br, bc, ldb, es)) != hipSuccess) return hip_fail(e, "copy B");
+    }
+    if (beta != 0.0f)
+        if ((e = put_matrix(dC, c, m, n, ldc, es)) != hipSuccess) return hip_fail(e, "copy C");
+    e = gemm16(f16, at, bt, m, n, k, alpha, (const uint16_t*)dA, max1(ar), (const uint16_t*)dB, max1(br), beta,
+               (uint16_t*)dC, m, nullptr);
+    if (e != hipSuccess) return hip_fail(e, "gemm launch");
+    if ((e = get_matrix(c, dC, m, n, ldc, es)) != hipSuccess) return hip_fail(e, "copy C back");
+    return 0;
+}
+
+static int gemm16_dev(bool f16, const char* name, char ta, char tb, int64_t m, int64_t n, int64_t k, float alpha,
+                      const uint16_t* a, int64_t lda, const uint16_t* b, int64_t ldb, float beta, uint16_t* c,
+                      int64_t ldc, void* stream) {
+    int info = check_gemm(ta, tb, m, n, k, lda, ldb, ldc);
+    if (info) return info;
+    if (m == 0 || n == 0) return 0;
+    hipError_t e = gemm16(f16, !flag_n(ta), !flag_n(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                          (hipStream_t)stream);
+    return e == hipSuccess ? 0 : hip_fail(e, name);
+}
+
 }  // namespace mbh
 
 using namespace mbh;
@@ -249,6 +289,26 @@ int mbh_dgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, dou
     hipError_t e = dgemm(!flag_n(transa), !flag_n(transb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
                          (hipStream_t)stream);
     return e == hipSuccess ? 0 : hip_fail(e, "mbh_dgemm_dev");
+}
+
+int mbh_hgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* a,
+                   int64_t lda, const uint16_t* b, int64_t ldb, float beta, uint16_t* c, int64_t ldc) {
+    return gemm16_host(true, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+int mbh_bgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* a,
+                   int64_t lda, const uint16_t* b, int64_t ldb, float beta, uint16_t* c, int64_t ldc) {
+    return gemm16_host(false, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+int mbh_hgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* a,
+                  int64_t lda, const uint16_t* b, int64_t ldb, float beta, uint16_t* c, int64_t ldc, void* stream) {
+    return gemm16_dev(true, "mbh_hgemm_dev", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+
+int mbh_bgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* a,
+                  int64_t lda, const uint16_t* b, int64_t ldb, float beta, uint16_t* c, int64_t ldc, void* stream) {
+    return gemm16_dev(false, "mbh_bgemm_dev", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
 }
 
 }  // extern "C"

@@ -20,6 +20,12 @@ hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha
                  const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s, int tri = 0,
                  int tri_grid = 0, int krange = 0);
 
+// Native 16-bit gemm: A, B and C hold f16 (f16 = true) or bf16 bit patterns;
+// products on the 16-bit MFMA with fp32 accumulation, alpha*acc + beta*C
+// formed in fp32 and rounded once to the 16-bit format.
+hipError_t gemm16(bool f16, bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* A,
+                  int64_t lda, const uint16_t* B, int64_t ldb, float beta, uint16_t* C, int64_t ldc, hipStream_t s);
+
 // fp64-MFMA gemm over mixed storage types (fp32/fp64 operands, fp32/fp64 C),
 // all arithmetic in fp64.  shape 0: 64x64 tiles; 1: 128x64 (a <=128-row C is
 // one tile row, so C may alias B); 2: 64x128 (C may alias A when n <= 128).

@@ -177,6 +177,82 @@ hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_g
     return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s);
 }
 
+// Native 16-bit path.  op(A) (rows m) is k-contiguous in memory iff A is
+// transposed, op(B)^T (rows n) iff B is not; such an operand with K a
+// multiple of 32, a 16-byte aligned base and ld % 8 == 0 is read in place by
+// the main kernel.  Any other operand is first re-laid into the packed
+// single plane (bits unchanged; one launch packs whichever operands need it).
+template <bool F16, int BT, int WM_, int WN_>
+hipError_t run_packed16(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* A,
+                        int64_t lda, const uint16_t* B, int64_t ldb, float beta, uint16_t* C, int64_t ldc,
+                        hipStream_t s) {
+    constexpr int PK = 32;
+    const int64_t tm = (m + BT - 1) / BT, tn = (n + BT - 1) / BT;
+    const int64_t ktiles = (k + PK - 1) / PK;
+    const int64_t ntile = tm * tn;
+    if (ntile <= 0 || ntile > INT_MAX || ktiles > INT_MAX) return hipErrorInvalidValue;
+    const bool direct_a = at && k % PK == 0 && vec_ok(A, lda);
+    const bool direct_b = !bt && k % PK == 0 && vec_ok(B, ldb);
+    const size_t a_el = direct_a ? 0 : (size_t)tm * BT * ktiles * PK;
+    const size_t b_el = direct_b ? 0 : (size_t)tn * BT * ktiles * PK;
+    uint16_t* base = nullptr;
+    hipError_t e;
+    if (a_el + b_el) {
+        void* ws = nullptr;
+        if ((e = workspace(s, 0, (a_el + b_el) * sizeof(uint16_t), &ws)) != hipSuccess) return e;
+        base = (uint16_t*)ws;
+    }
+    Pack16Params pa{A, lda, m, k, tm * BT, (int)ktiles, base, (int)vec_ok(A, lda)};
+    Pack16Params pb{B, ldb, n, k, tn * BT, (int)ktiles, base + a_el, (int)vec_ok(B, ldb)};
+    TimingMark tmk;
+    tmk.active = timing_on();
+    timing_mark(tmk, 0, s);
+    // 128-row x 128-k pack blocks; an operand read in place gets none.
+    const int64_t kblk = (ktiles * PK + 127) / 128;
+    const int64_t gx_a = (tm * BT + 127) / 128, gx_b = (tn * BT + 127) / 128;
+    const int64_t nblk_a = direct_a ? 0 : gx_a * kblk, nblk = nblk_a + (direct_b ? 0 : gx_b * kblk);
+    if (nblk > INT_MAX) return hipErrorInvalidValue;
+    if (nblk > 0) {
+        const dim3 pgrid((unsigned)nblk), pblock(256);
+        const int ga = (int)gx_a, na = (int)nblk_a, gb = (int)gx_b;
+        if (at && !bt) pack16_kernel<true, true, BT><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+        else if (at && bt) pack16_kernel<true, false, BT><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+        else if (!at && !bt) pack16_kernel<false, true, BT><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+        else pack16_kernel<false, false, BT><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    PackedGemmParams p{};
+    p.m = m;
+    p.n = n;
+    p.ktiles = (int)ktiles;
+    p.Ah = direct_a ? A : pa.out;
+    p.Bh = direct_b ? B : pb.out;
+    p.a_ld = direct_a ? lda : 0;
+    p.b_ld = direct_b ? ldb : 0;
+    p.C = C;
+    p.ldc = ldc;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.tiles_m = (int)tm;
+    p.tiles_n = (int)tn;
+    timing_mark(tmk, 1, s);
+    gemm_packed16_kernel<F16, BT, BT, WM_, WN_><<<dim3((unsigned)ntile), dim3(64 * WM_ * WN_), 0, s>>>(p);
+    timing_mark(tmk, 2, s);
+    return hipGetLastError();
+}
+
+// C := beta*C on 16-bit storage (beta == 0 writes zeros without reading).
+template <bool F16>
+__global__ void scale16_kernel(int64_t m, int64_t n, float beta, uint16_t* C, int64_t ldc) {
+    typedef CElem<F16 ? 1 : 2> E;
+    const int64_t total = m * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        uint16_t* c = C + idx % m + (idx / m) * ldc;
+        *c = (beta == 0.0f) ? (uint16_t)0 : E::store(beta * E::load(*c));
+    }
+}
+
 template <typename T>
 __global__ void scale_kernel(int64_t m, int64_t n, T beta, T* C, int64_t ldc, int tri) {
     const int64_t total = m * n;
@@ -284,6 +360,32 @@ hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double
     if (shape == 1) return launch_fp<double, TA, TB, TC, 128, 32, 32>(at, bt, p, 0, s);
     if (shape == 2) return launch_fp<double, TA, TB, TC, 32, 128, 32>(at, bt, p, 0, s);
     return launch_fp<double, TA, TB, TC, 64, 64, 32>(at, bt, p, 0, s);
+}
+
+template <bool F16>
+static hipError_t gemm16_t(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* A,
+                           int64_t lda, const uint16_t* B, int64_t ldb, float beta, uint16_t* C, int64_t ldc,
+                           hipStream_t s) {
+    if (alpha == 0.0f || k == 0) {
+        if (beta == 1.0f) return hipSuccess;
+        int64_t blocks = (m * n + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        scale16_kernel<F16><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(m, n, beta, C, ldc);
+        return hipGetLastError();
+    }
+    // Tile size by grid fill, as for the split-precision path.
+    if (tile_count(m, n, 256, 256, 0) >= 256)
+        return run_packed16<F16, 256, 2, 4>(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    if (tile_count(m, n, 128, 128, 0) >= 128)
+        return run_packed16<F16, 128, 2, 2>(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return run_packed16<F16, 64, 2, 2>(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+hipError_t gemm16(bool f16, bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* A,
+                  int64_t lda, const uint16_t* B, int64_t ldb, float beta, uint16_t* C, int64_t ldc, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (f16) return gemm16_t<true>(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return gemm16_t<false>(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 #define MBH_F64ACC(TA, TB, TC)                                                                                     \

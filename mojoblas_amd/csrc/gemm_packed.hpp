@@ -2,7 +2,8 @@
 // accumulate.
 //
 //   NPROD = 3: BF16x3, acc += Bh*Ah + Bl*Ah + Bh*Al   (x = hi + lo per operand)
-//   NPROD = 1: plain bf16 operands, acc += Bh*Ah
+//   NPROD = 1: plain bf16 operands, acc += Bh*Ah (also the native 16-bit
+//              gemm: bf16 or f16 operands, C in that format, CK / F16 below)
 //   NPROD = 8: e4m3 operands with one power-of-two scale per 128x128 tile of
 //              A and of B, applied by v_mfma_scale_f32_32x32x64_f8f6f4 (E8M0
 //              scale operands), so no per-slab partial sums are needed
@@ -43,7 +44,7 @@ struct PackedGemmParams {
     const uint16_t* Al;
     const uint16_t* Bh;
     const uint16_t* Bl;
-    float* C;
+    void* C;        // fp32, or the 16-bit format of the native 16-bit gemm
     int64_t ldc;
     float alpha, beta;
     int tiles_m, tiles_n;
@@ -53,23 +54,50 @@ struct PackedGemmParams {
     const int* sA;  // fp8: E8M0 scale per 128x128 tile, [rows/128][kblocks]
     const int* sB;
     int kblocks;
+    // Native 16-bit gemm only: a_ld / b_ld != 0 means Ah / Bh is the caller's
+    // k-contiguous operand itself (row r at Ah + r*a_ld, K a multiple of 32),
+    // read in place instead of from a packed plane.
+    int64_t a_ld, b_ld;
 };
 
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 __device__ inline int lds_swz(int row) { return ((row >> 1) ^ (row >> 2)) & 3; }
 
+// Storage format of C: CK = 0 fp32, 1 f16, 2 bf16.  The 16-bit formats are
+// read into fp32, and alpha*acc + beta*c is rounded once (nearest even).
+template <int CK>
+struct CElem {
+    typedef float T;
+    static __device__ float load(float x) { return x; }
+    static __device__ float store(float v) { return v; }
+};
+template <>
+struct CElem<1> {
+    typedef uint16_t T;
+    static __device__ float load(uint16_t x) { return (float)__builtin_bit_cast(_Float16, x); }
+    static __device__ uint16_t store(float v) { return __builtin_bit_cast(uint16_t, (_Float16)v); }
+};
+template <>
+struct CElem<2> {
+    typedef uint16_t T;
+    static __device__ float load(uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
+    static __device__ uint16_t store(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
+};
+
 // Epilogue of a wave's TM x TN block of 32x32 accumulators holding C^T
 // (register r of lane l is C(row0 + 32i + l%32, col0 + 32j + acc_col(r, l))).
 // Interior blocks without a triangle mask take a branch-free path.
-template <int TM, int TN>
-__device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* Cgen, int64_t ldc, int64_t m, int64_t n,
+template <int TM, int TN, int CK = 0>
+__device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], void* Cgen, int64_t ldc, int64_t m, int64_t n,
                                       int64_t row0, int64_t col0, int lane, float alpha, float beta, int tri) {
-    MBH_GLOBAL float* const C = (MBH_GLOBAL float*)Cgen;
+    typedef CElem<CK> E;
+    typedef typename E::T CT;
+    MBH_GLOBAL CT* const C = (MBH_GLOBAL CT*)Cgen;
     const int lrow = lane & 31;
     const bool interior = tri == TRI_NONE && row0 + 32 * TM <= m && col0 + 32 * TN <= n;
     if (interior) {
-        MBH_GLOBAL float* cb = C + (row0 + lrow) + col0 * ldc;
+        MBH_GLOBAL CT* cb = C + (row0 + lrow) + col0 * ldc;
         if (beta == 0.0f) {
 #pragma unroll
             for (int i = 0; i < TM; ++i)
@@ -77,7 +105,7 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* Cgen, 
                 for (int j = 0; j < TN; ++j)
 #pragma unroll
                     for (int r = 0; r < 16; ++r)
-                        cb[i * 32 + (int64_t)(j * 32 + acc_col<32>(r, lane)) * ldc] = alpha * acc[i][j][r];
+                        cb[i * 32 + (int64_t)(j * 32 + acc_col<32>(r, lane)) * ldc] = E::store(alpha * acc[i][j][r]);
         } else {
 #pragma unroll
             for (int i = 0; i < TM; ++i)
@@ -85,8 +113,8 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* Cgen, 
                 for (int j = 0; j < TN; ++j)
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        MBH_GLOBAL float* cp = cb + i * 32 + (int64_t)(j * 32 + acc_col<32>(r, lane)) * ldc;
-                        *cp = alpha * acc[i][j][r] + beta * *cp;
+                        MBH_GLOBAL CT* cp = cb + i * 32 + (int64_t)(j * 32 + acc_col<32>(r, lane)) * ldc;
+                        *cp = E::store(alpha * acc[i][j][r] + beta * E::load(*cp));
                     }
         }
         return;
@@ -101,10 +129,10 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], float* Cgen, 
             for (int r = 0; r < 16; ++r) {
                 const int64_t col = col0 + j * 32 + acc_col<32>(r, lane);
                 if (col < n && tri_keep(tri, row, col)) {
-                    MBH_GLOBAL float* cp = C + row + col * ldc;
+                    MBH_GLOBAL CT* cp = C + row + col * ldc;
                     float v = alpha * acc[i][j][r];
-                    if (beta != 0.0f) v += beta * *cp;
-                    *cp = v;
+                    if (beta != 0.0f) v += beta * E::load(*cp);
+                    *cp = E::store(v);
                 }
             }
         }
@@ -484,8 +512,18 @@ __device__ inline void packed_frag_read(PackedFrag<NPL, TM, TN>& f, const MBH_LD
 // MFMAs of one k-step.  Per accumulator the products come in the order
 // lo*hi, hi*lo, hi*hi (as in packed_ktile_mfma); between two products on the
 // same accumulator the other TM*TN - 1 accumulators are issued.
-template <int NPROD, int NPL, int TM, int TN>
+template <int NPROD, int NPL, int TM, int TN, bool F16 = false>
 __device__ inline void packed_frag_mfma(const PackedFrag<NPL, TM, TN>& f, f32x16 (&acc)[TM][TN]) {
+    if constexpr (F16) {  // native f16 operands (one plane): v_mfma_f32_32x32x16_f16
+        typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, f.bh[j]),
+                                                                   __builtin_bit_cast(f16x8, f.ah[i]), acc[i][j], 0, 0, 0);
+        return;
+    }
     if (NPROD == 3) {
 #pragma unroll
         for (int i = 0; i < TM; ++i)
@@ -526,9 +564,20 @@ __device__ inline void packed_interleave() {
 // the direct-to-LDS ones of gemm_packed_tile_dma; one barrier per k-tile,
 // after which the next k-tile's first step is read while the current tile's
 // second step (already in registers) is multiplied.
-template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
+//
+// CK: storage format of C (CElem); F16: the single operand plane holds f16
+// (native f16 gemm), multiplied on the f16 MFMA.  STRIDED: each lane keeps
+// one pointer per wave-load and advances it by a per-operand k-tile stride,
+// so an operand is either a packed plane (stride BM*4 chunks) or, when
+// p.a_ld / p.b_ld is set, the caller's k-contiguous matrix read in place
+// (lane chunk (row, c) at X + row*ld + 8c, stride 4 chunks; rows past the
+// operand's last are clamped to it: they only reach C entries that are never
+// stored).
+template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N, int CK = 0, bool F16 = false, bool STRIDED = false>
 __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, int tn, u32x4* lds0, u32x4* lds1) {
     static_assert(NPROD == 1 || NPROD == 3, "bf16 modes only");
+    static_assert(!F16 || NPROD == 1, "f16 operands use one plane");
+    static_assert(!STRIDED || NPROD == 1, "in-place operands hold one plane");
     constexpr int NW = WAVES_M * WAVES_N;
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
     constexpr int TM = WM / 32, TN = WN / 32;
@@ -557,32 +606,71 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     MBH_LDS u32x4* const L0 = (MBH_LDS u32x4*)lds0;
     MBH_LDS u32x4* const L1 = (MBH_LDS u32x4*)lds1;
     constexpr int OA1 = A_CH, OB0 = NPL * A_CH, OB1 = NPL * A_CH + B_CH;
+    // STRIDED: per wave-load pointers and k-tile strides (in 16-byte chunks).
+    gptr pa[A_WL], pb[B_WL];
+    size_t sa = A_CH, sb = B_CH;
+    if constexpr (STRIDED) {
+        const int c = (lane & 3) ^ lds_swz(lrow16);
+#pragma unroll
+        for (int v = 0; v < A_WL; ++v) {
+            const int blk = (wave + v * NW) * 64;
+            if (p.a_ld) {
+                int64_t row = (int64_t)tm * BM + (wave + v * NW) * 16 + lrow16;
+                row = row < p.m ? row : p.m - 1;
+                pa[v] = (gptr)(p.Ah + row * p.a_ld) + c;
+            } else {
+                pa[v] = gA0 + blk;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < B_WL; ++v) {
+            const int blk = (wave + v * NW) * 64;
+            if (p.b_ld) {
+                int64_t row = (int64_t)tn * BN + (wave + v * NW) * 16 + lrow16;
+                row = row < p.n ? row : p.n - 1;
+                pb[v] = (gptr)(p.Bh + row * p.b_ld) + c;
+            } else {
+                pb[v] = gB0 + blk;
+            }
+        }
+        if (p.a_ld) sa = 4;
+        if (p.b_ld) sb = 4;
+    }
 
 #define MBH_DMA(kt, LB)                                                                                 \
     do {                                                                                                \
-        const size_t ga = (size_t)(kt) * A_CH, gb = (size_t)(kt) * B_CH;                                \
-        _Pragma("unroll") for (int v = 0; v < A_WL; ++v) {                                              \
-            const int blk = (wave + v * NW) * 64;                                                       \
-            __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gA0 + ga + blk),                  \
-                                             (MBH_LDS void*)((LB) + blk), 16, 0, 0);                    \
-            if (NPL == 2)                                                                               \
-                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gA1 + ga + blk),              \
-                                                 (MBH_LDS void*)((LB) + OA1 + blk), 16, 0, 0);          \
-        }                                                                                               \
-        _Pragma("unroll") for (int v = 0; v < B_WL; ++v) {                                              \
-            const int blk = (wave + v * NW) * 64;                                                       \
-            __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gB0 + gb + blk),                  \
-                                             (MBH_LDS void*)((LB) + OB0 + blk), 16, 0, 0);              \
-            if (NPL == 2)                                                                               \
-                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gB1 + gb + blk),              \
-                                                 (MBH_LDS void*)((LB) + OB1 + blk), 16, 0, 0);          \
+        if constexpr (STRIDED) {                                                                        \
+            _Pragma("unroll") for (int v = 0; v < A_WL; ++v)                                            \
+                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(pa[v] + (size_t)(kt) * sa),   \
+                                                 (MBH_LDS void*)((LB) + (wave + v * NW) * 64), 16, 0, 0); \
+            _Pragma("unroll") for (int v = 0; v < B_WL; ++v)                                            \
+                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(pb[v] + (size_t)(kt) * sb),   \
+                                                 (MBH_LDS void*)((LB) + OB0 + (wave + v * NW) * 64), 16, 0, 0); \
+        } else {                                                                                        \
+            const size_t ga = (size_t)(kt) * A_CH, gb = (size_t)(kt) * B_CH;                            \
+            _Pragma("unroll") for (int v = 0; v < A_WL; ++v) {                                          \
+                const int blk = (wave + v * NW) * 64;                                                   \
+                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gA0 + ga + blk),              \
+                                                 (MBH_LDS void*)((LB) + blk), 16, 0, 0);                \
+                if (NPL == 2)                                                                           \
+                    __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gA1 + ga + blk),          \
+                                                     (MBH_LDS void*)((LB) + OA1 + blk), 16, 0, 0);      \
+            }                                                                                           \
+            _Pragma("unroll") for (int v = 0; v < B_WL; ++v) {                                          \
+                const int blk = (wave + v * NW) * 64;                                                   \
+                __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gB0 + gb + blk),              \
+                                                 (MBH_LDS void*)((LB) + OB0 + blk), 16, 0, 0);          \
+                if (NPL == 2)                                                                           \
+                    __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gB1 + gb + blk),          \
+                                                     (MBH_LDS void*)((LB) + OB1 + blk), 16, 0, 0);      \
+            }                                                                                           \
         }                                                                                               \
         __builtin_amdgcn_sched_barrier(0);                                                              \
     } while (0)
 #define MBH_STEP(FN, BUF, CH, FC)                                                                       \
     do {                                                                                                \
         packed_frag_read<NPL, TM, TN>(FN, BUF, OA1, OB0, OB1, wm0, wn0, lrow, CH);                      \
-        packed_frag_mfma<NPROD, NPL, TM, TN>(FC, acc);                                                  \
+        packed_frag_mfma<NPROD, NPL, TM, TN, F16>(FC, acc);                                             \
         packed_interleave<NREAD, NMFMA>();                                                              \
         __builtin_amdgcn_sched_barrier(0);                                                              \
     } while (0)
@@ -628,14 +716,14 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     }
     if (kt < kt1) {  // odd count: k-tile kt in L0, f0 = its step 0
         MBH_STEP(f1, L0, ch1, f0);
-        packed_frag_mfma<NPROD, NPL, TM, TN>(f1, acc);
+        packed_frag_mfma<NPROD, NPL, TM, TN, F16>(f1, acc);
     }
 #undef MBH_SYNC
 #undef MBH_STEP
 #undef MBH_DMA
 
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-    epilogue_32x32<TM, TN>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
+    epilogue_32x32<TM, TN, CK>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
 }
 
 
@@ -651,6 +739,21 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
         gemm_packed_tile_pipe<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
     else
         gemm_packed_tile_dma<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
+}
+
+// Native 16-bit gemm: one plane per operand holding f16 (F16) or bf16 values
+// (packed, or the caller's k-contiguous operand read in place), products on
+// the MFMA of that format with fp32 accumulation, C stored in the same 16-bit
+// format.
+template <bool F16, int BM, int BN, int WAVES_M, int WAVES_N>
+__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed16_kernel(PackedGemmParams p) {
+    typedef PackedTileLds<1, BM, BN> L;
+    __shared__ u32x4 lds0[L::TOTAL / 2];
+    __shared__ u32x4 lds1[L::TOTAL / 2];
+    int tm, tn;
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
+    else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
+    gemm_packed_tile_pipe<1, BM, BN, WAVES_M, WAVES_N, F16 ? 1 : 2, F16, true>(p, tm, tn, lds0, lds1);
 }
 
 }  // namespace mbh

@@ -138,6 +138,91 @@ __global__ __launch_bounds__(256) void pack_split_kernel(PackParams a, PackParam
     }
 }
 
+// 16-bit pack (native f16 / bf16 gemm): op(X) of 16-bit elements -> the same
+// tile-blocked single plane, bits copied unchanged (the products run on the
+// MFMA of that format).
+struct Pack16Params {
+    const uint16_t* X;
+    int64_t ld;
+    int64_t R, K;  // logical rows (m or n) and depth k of op(X)
+    int64_t Rp;    // rows of the packed plane (whole macro-tiles)
+    int ktiles;    // ceil(K / 32)
+    uint16_t* out;
+    int vec;       // X 16-byte aligned and ld % 8 == 0
+};
+
+// One 128-row x 128-k block per workgroup, held in registers: each thread
+// loads eight 16-byte runs and writes eight 16-byte packed runs (8 k of one
+// row), so no LDS is needed.
+//   KCONTIG ((r, k) at X[k + r ld]): v[j] = X(gr0 + 16j, gk0 .. gk0 + 7),
+//     gr0 = r0 + tid/16, gk0 = k0 + 8 (tid%16): 16 lanes read one 256-byte
+//     row segment;
+//   else ((r, k) at X[r + k ld]): v[j] = X(gr0 .. gr0 + 7, gk0 + j),
+//     gr0 = r0 + 8 (tid%16), gk0 = k0 + 8 (tid/16): 16 lanes read 128
+//     consecutive rows of one column, and the 8 x 8 block is transposed in
+//     registers before the store.
+// Rows >= R and k >= K are written as zeros; rows >= Rp and k-tiles >=
+// ktiles are not written.
+template <bool KCONTIG, int PR>
+__device__ inline void pack16_block(const Pack16Params& p, int64_t r0, int64_t k0) {
+    constexpr int PK = 32;
+    const int tid = threadIdx.x;
+    const MBH_GLOBAL uint16_t* const X = (const MBH_GLOBAL uint16_t*)p.X;
+    const int64_t gr0 = KCONTIG ? r0 + tid / 16 : r0 + 8 * (tid % 16);
+    const int64_t gk0 = KCONTIG ? k0 + 8 * (tid % 16) : k0 + 8 * (tid / 16);
+    u32x4 v[8];
+    if (p.vec && r0 + 128 <= p.R && k0 + 128 <= p.K) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            v[j] = *(const MBH_GLOBAL u32x4*)(KCONTIG ? X + gk0 + (gr0 + 16 * j) * p.ld : X + gr0 + (gk0 + j) * p.ld);
+    } else {
+        auto elem = [&](int j, int e) -> uint32_t {
+            const int64_t gr = KCONTIG ? gr0 + 16 * j : gr0 + e, gk = KCONTIG ? gk0 + e : gk0 + j;
+            return (gr < p.R && gk < p.K) ? (uint32_t)X[KCONTIG ? gk + gr * p.ld : gr + gk * p.ld] : 0u;
+        };
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int w = 0; w < 4; ++w) v[j][w] = elem(j, 2 * w) | (elem(j, 2 * w + 1) << 16);
+    }
+    // gk0 is a multiple of 8, so a thread's 8 k stay inside one 32-k tile.
+    if (gk0 / PK >= p.ktiles) return;
+    auto dst = [&](int64_t gr) {
+        return (MBH_GLOBAL u32x4*)(p.out + (((gr / PR) * p.ktiles + gk0 / PK) * PR + gr % PR) * PK + gk0 % PK);
+    };
+    if (KCONTIG) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (gr0 + 16 * j < p.Rp) *dst(gr0 + 16 * j) = v[j];
+    } else {
+        // Row gr0 + e: element e of v[0..7] (16-bit half e%2 of word e/2).
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (gr0 + e >= p.Rp) continue;
+            u32x4 w;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t lo = v[2 * q][e / 2], hi = v[2 * q + 1][e / 2];
+                w[q] = (e & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
+            }
+            *dst(gr0 + e) = w;
+        }
+    }
+}
+
+// Both operands in one launch: workgroups [0, nblk_a) pack op(A) (gx_a row
+// blocks of 128 x k blocks of 128), the rest op(B)^T.
+template <bool KA, bool KB, int PR>
+__global__ __launch_bounds__(256) void pack16_kernel(Pack16Params a, Pack16Params b, int gx_a, int nblk_a, int gx_b) {
+    int w = (int)blockIdx.x;
+    if (w < nblk_a) {
+        pack16_block<KA, PR>(a, (int64_t)(w % gx_a) * 128, (int64_t)(w / gx_a) * 128);
+    } else {
+        w -= nblk_a;
+        pack16_block<KB, PR>(b, (int64_t)(w % gx_b) * 128, (int64_t)(w / gx_b) * 128);
+    }
+}
+
 }  // namespace mbh
 
 namespace mbh {

@@ -60,3 +60,25 @@ def abs_product(ta, tb, amat, bmat):
     a = np.abs(amat if ta == "N" else amat.T)
     b = np.abs(bmat if tb == "N" else bmat.T)
     return a @ b
+
+
+def round_bf16(x):
+    """fp32 values -> bf16 bit patterns (uint16), round to nearest, ties to
+    even, on the fp32 bits: add 0x7fff plus the lowest kept bit, drop 16
+    bits.  Finite inputs only."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def bf16_to_f64(bits):
+    return (np.asarray(bits, dtype=np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def f16_to_f64(bits):
+    return np.asarray(bits, dtype=np.uint16).view(np.float16).astype(np.float64)
+
+
+def round_f16(x):
+    """fp32 values -> f16 bit patterns (numpy's conversion rounds to nearest
+    even)."""
+    return np.asarray(x, dtype=np.float32).astype(np.float16).view(np.uint16)

@@ -54,11 +54,11 @@ GEMM_INFO = [
 
 
 @pytest.mark.parametrize("args,info", GEMM_INFO)
-@pytest.mark.parametrize("entry", ["host_s", "host_d", "dev_s", "dev_d"])
+@pytest.mark.parametrize("entry", ["host_s", "host_d", "dev_s", "dev_d", "host_h", "host_b", "dev_h", "dev_b"])
 def test_gemm_info_codes_without_gpu(args, info, entry):
     L = _lib.lib()
     ta, tb, m, n, k, lda, ldb, ldc = args
-    dt = np.float32 if entry.endswith("s") else np.float64
+    dt = {"s": np.float32, "d": np.float64, "h": np.uint16, "b": np.uint16}[entry[-1]]
     a = np.full(16, 7, dt)
     b = np.full(16, 7, dt)
     c = np.full(16, 7, dt)
@@ -69,8 +69,14 @@ def test_gemm_info_codes_without_gpu(args, info, entry):
         rc = L.mbh_dgemm_host(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc)
     elif entry == "dev_s":
         rc = L.mbh_sgemm_dev(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc, 0, None)
-    else:
+    elif entry == "dev_d":
         rc = L.mbh_dgemm_dev(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc, None)
+    elif entry.startswith("host"):
+        fn = L.mbh_hgemm_host if entry == "host_h" else L.mbh_bgemm_host
+        rc = fn(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc)
+    else:
+        fn = L.mbh_hgemm_dev if entry == "dev_h" else L.mbh_bgemm_dev
+        rc = fn(ta, tb, m, n, k, 1.0, P(a), lda, P(b), ldb, 0.0, P(c), ldc, None)
     assert rc == info
     assert (c == 7).all() and (a == 7).all() and (b == 7).all()
 
@@ -81,6 +87,20 @@ def test_python_mirror_prints_info_like_the_reference(capsys):
     assert "gemm: Info 1" in capsys.readouterr().out
     assert mb.gemm("N", "N", 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a.copy(), 1) == 13
     assert "gemm: Info 13" in capsys.readouterr().out
+
+
+def test_python_gemm_16bit_dtype_routing_without_gpu(capsys):
+    # numpy float16 host arrays go to the hgemm entry: its info codes come
+    # back before any GPU call; precision= is refused for 16-bit data.
+    a = np.zeros(4, np.float16)
+    assert mb.gemm("X", "N", 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a.copy(), 2) == 1
+    assert "gemm: Info 1" in capsys.readouterr().out
+    with pytest.raises(ValueError):
+        mb.gemm("N", "N", 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a.copy(), 2, precision="bf16x3")
+    with pytest.raises(TypeError):
+        mb.gemm("N", "N", 2, 2, 2, 1.0, a, 2, np.zeros(4, np.float32), 2, 0.0, a.copy(), 2)
+    with pytest.raises(TypeError):  # the other level-3 routines stay fp32 / fp64
+        mb.syrk("L", "N", 2, 2, 1.0, a, 2, 0.0, a.copy(), 2)
 
 
 def test_zero_dimension_quick_return_touches_nothing():
