@@ -179,14 +179,14 @@ hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_g
 
 // Native 16-bit path.  op(A) (rows m) is k-contiguous in memory iff A is
 // transposed, op(B)^T (rows n) iff B is not; such an operand with K a
-// multiple of 32, a 16-byte aligned base and ld % 8 == 0 is read in place by
+// multiple of 64, a 16-byte aligned base and ld % 8 == 0 is read in place by
 // the main kernel.  Any other operand is first re-laid into the packed
 // single plane (bits unchanged; one launch packs whichever operands need it).
 template <bool F16, int BT, int WM_, int WN_>
 hipError_t run_packed16(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* A,
                         int64_t lda, const uint16_t* B, int64_t ldb, float beta, uint16_t* C, int64_t ldc,
                         hipStream_t s) {
-    constexpr int PK = 32;
+    constexpr int PK = GEMM16_BK;
     const int64_t tm = (m + BT - 1) / BT, tn = (n + BT - 1) / BT;
     const int64_t ktiles = (k + PK - 1) / PK;
     const int64_t ntile = tm * tn;
@@ -215,10 +215,10 @@ hipError_t run_packed16(bool at, bool bt, int64_t m, int64_t n, int64_t k, float
     if (nblk > 0) {
         const dim3 pgrid((unsigned)nblk), pblock(256);
         const int ga = (int)gx_a, na = (int)nblk_a, gb = (int)gx_b;
-        if (at && !bt) pack16_kernel<true, true, BT><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
-        else if (at && bt) pack16_kernel<true, false, BT><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
-        else if (!at && !bt) pack16_kernel<false, true, BT><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
-        else pack16_kernel<false, false, BT><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+        if (at && !bt) pack16_kernel<true, true, BT, PK><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+        else if (at && bt) pack16_kernel<true, false, BT, PK><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+        else if (!at && !bt) pack16_kernel<false, true, BT, PK><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
+        else pack16_kernel<false, false, BT, PK><<<pgrid, pblock, 0, s>>>(pa, pb, ga, na, gb);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     PackedGemmParams p{};

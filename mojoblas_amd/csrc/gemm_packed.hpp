@@ -141,10 +141,10 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], void* Cgen, i
 
 // LDS (in 16-byte chunks) of one double-buffered tile: 64-byte rows, one or
 // two planes per operand.
-template <int NPROD, int BM, int BN>
+template <int NPROD, int BM, int BN, int CPR = 4>
 struct PackedTileLds {
     static constexpr int NPL = (NPROD == 3) ? 2 : 1;
-    static constexpr int A = BM * 4, B = BN * 4;  // per plane per buffer
+    static constexpr int A = BM * CPR, B = BN * CPR;  // per plane per buffer (CPR chunks per row)
     static constexpr int TOTAL = 2 * NPL * (A + B);
 };
 
@@ -491,19 +491,20 @@ struct PackedFrag {
     bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
 };
 
-// LDS -> registers: chunk `ch` (swizzled) of the wave's rows in each plane.
-template <int NPL, int TM, int TN>
+// LDS -> registers: chunk `ch` (swizzled) of the wave's rows in each plane
+// (CPR chunks per LDS row).
+template <int NPL, int TM, int TN, int CPR = 4>
 __device__ inline void packed_frag_read(PackedFrag<NPL, TM, TN>& f, const MBH_LDS u32x4* buf, int oa1, int ob0,
                                         int ob1, int wm0, int wn0, int lrow, int ch) {
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
-        const int idx = (wm0 + i * 32 + lrow) * 4 + ch;
+        const int idx = (wm0 + i * 32 + lrow) * CPR + ch;
         f.ah[i] = __builtin_bit_cast(bf16x8, (u32x4)buf[idx]);
         if (NPL == 2) f.al[i] = __builtin_bit_cast(bf16x8, (u32x4)buf[oa1 + idx]);
     }
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-        const int idx = (wn0 + j * 32 + lrow) * 4 + ch;
+        const int idx = (wn0 + j * 32 + lrow) * CPR + ch;
         f.bh[j] = __builtin_bit_cast(bf16x8, (u32x4)buf[ob0 + idx]);
         if (NPL == 2) f.bl[j] = __builtin_bit_cast(bf16x8, (u32x4)buf[ob1 + idx]);
     }
@@ -568,21 +569,32 @@ __device__ inline void packed_interleave() {
 // CK: storage format of C (CElem); F16: the single operand plane holds f16
 // (native f16 gemm), multiplied on the f16 MFMA.  STRIDED: each lane keeps
 // one pointer per wave-load and advances it by a per-operand k-tile stride,
-// so an operand is either a packed plane (stride BM*4 chunks) or, when
+// so an operand is either a packed plane (stride BM*BKT/8 chunks) or, when
 // p.a_ld / p.b_ld is set, the caller's k-contiguous matrix read in place
-// (lane chunk (row, c) at X + row*ld + 8c, stride 4 chunks; rows past the
+// (lane chunk (row, c) at X + row*ld + 8c, stride BKT/8 chunks; rows past the
 // operand's last are clamped to it: they only reach C entries that are never
-// stored).
-template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N, int CK = 0, bool F16 = false, bool STRIDED = false>
+// stored).  BKT: k per LDS row, 32 (64-byte rows, two 16-deep MFMA steps per
+// k-tile) or, in the 16-bit kernel, 64 (128-byte rows, so an in-place row
+// read is one whole 128-byte line, and four steps per barrier); 64-deep rows
+// are swizzled over 8 chunks by swz8(row) = (row >> 1) & 7, which spreads
+// the 32-row fragment reads (even and odd rows on separate halves of the
+// banks) over distinct 16-byte bank slots.
+template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N, int CK = 0, bool F16 = false, bool STRIDED = false,
+          int BKT = 32>
 __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, int tn, u32x4* lds0, u32x4* lds1) {
     static_assert(NPROD == 1 || NPROD == 3, "bf16 modes only");
     static_assert(!F16 || NPROD == 1, "f16 operands use one plane");
     static_assert(!STRIDED || NPROD == 1, "in-place operands hold one plane");
+    static_assert(BKT == 32 || (BKT == 64 && STRIDED), "64-deep k-tiles: 16-bit kernel only");
     constexpr int NW = WAVES_M * WAVES_N;
+    static_assert(BKT == 32 || NW % 2 == 0, "swz8 of a wave-load's rows assumes an even wave count");
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
     constexpr int TM = WM / 32, TN = WN / 32;
     constexpr int NPL = (NPROD == 3) ? 2 : 1;
-    constexpr int A_CH = BM * 4, B_CH = BN * 4;
+    constexpr int CPR = BKT / 8;  // 16-byte chunks per LDS row
+    constexpr int KS = BKT / 16;  // 16-deep MFMA k-steps per k-tile
+    constexpr int RW = 64 / CPR;  // rows per wave-load
+    constexpr int A_CH = BM * CPR, B_CH = BN * CPR;
     constexpr int A_WL = A_CH / (64 * NW), B_WL = B_CH / (64 * NW);
     static_assert(A_WL * 64 * NW == A_CH && B_WL * 64 * NW == B_CH && A_WL >= 1 && B_WL >= 1,
                   "tile not divisible into wave-loads");
@@ -592,8 +604,14 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
-    const int lrow16 = lane >> 2;
-    const int lchunk = lrow16 * 4 + ((lane & 3) ^ lds_swz(lrow16));
+    auto swz = [](int row) { return BKT == 32 ? lds_swz(row) : (row >> 1) & 7; };
+    // Row of this lane inside its wave-load (RW x CPR chunks) and the
+    // swizzle of that row: for BKT = 32 the wave-load's base row (a multiple
+    // of 16) leaves the swizzle bits alone; for 64 its bit 3 is the parity
+    // of the wave-load index, which is the wave's parity (NW even).
+    const int lroww = lane / CPR;
+    const int lslot = (lane % CPR) ^ swz(BKT == 32 ? lroww : (wave & 1) * 8 + lroww);
+    const int lchunk = lroww * CPR + lslot;
     const size_t a_off = (size_t)tm * p.ktiles * A_CH + lchunk, b_off = (size_t)tn * p.ktiles * B_CH + lchunk;
     gptr gA0 = ((gptr)p.Ah) + a_off;
     gptr gB0 = ((gptr)p.Bh) + b_off;
@@ -610,12 +628,12 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     gptr pa[A_WL], pb[B_WL];
     size_t sa = A_CH, sb = B_CH;
     if constexpr (STRIDED) {
-        const int c = (lane & 3) ^ lds_swz(lrow16);
+        const int c = lslot;
 #pragma unroll
         for (int v = 0; v < A_WL; ++v) {
             const int blk = (wave + v * NW) * 64;
             if (p.a_ld) {
-                int64_t row = (int64_t)tm * BM + (wave + v * NW) * 16 + lrow16;
+                int64_t row = (int64_t)tm * BM + (wave + v * NW) * RW + lroww;
                 row = row < p.m ? row : p.m - 1;
                 pa[v] = (gptr)(p.Ah + row * p.a_ld) + c;
             } else {
@@ -626,15 +644,15 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
         for (int v = 0; v < B_WL; ++v) {
             const int blk = (wave + v * NW) * 64;
             if (p.b_ld) {
-                int64_t row = (int64_t)tn * BN + (wave + v * NW) * 16 + lrow16;
+                int64_t row = (int64_t)tn * BN + (wave + v * NW) * RW + lroww;
                 row = row < p.n ? row : p.n - 1;
                 pb[v] = (gptr)(p.Bh + row * p.b_ld) + c;
             } else {
                 pb[v] = gB0 + blk;
             }
         }
-        if (p.a_ld) sa = 4;
-        if (p.b_ld) sb = 4;
+        if (p.a_ld) sa = CPR;
+        if (p.b_ld) sb = CPR;
     }
 
 #define MBH_DMA(kt, LB)                                                                                 \
@@ -669,7 +687,7 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     } while (0)
 #define MBH_STEP(FN, BUF, CH, FC)                                                                       \
     do {                                                                                                \
-        packed_frag_read<NPL, TM, TN>(FN, BUF, OA1, OB0, OB1, wm0, wn0, lrow, CH);                      \
+        packed_frag_read<NPL, TM, TN, CPR>(FN, BUF, OA1, OB0, OB1, wm0, wn0, lrow, CH);                 \
         packed_frag_mfma<NPROD, NPL, TM, TN, F16>(FC, acc);                                             \
         packed_interleave<NREAD, NMFMA>();                                                              \
         __builtin_amdgcn_sched_barrier(0);                                                              \
@@ -689,33 +707,48 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
     int kt0, kt1;
-    kt_range(p.krange, tm, tn, BM, BN, 32, p.ktiles, kt0, kt1);
-    const int lrow = lane & 31, lgrp = lane >> 5, lsw = lds_swz(lrow);
-    const int ch0 = lgrp ^ lsw, ch1 = (2 + lgrp) ^ lsw;  // chunks of k-steps 0 and 1
+    kt_range(p.krange, tm, tn, BM, BN, BKT, p.ktiles, kt0, kt1);
+    const int lrow = lane & 31, lgrp = lane >> 5, lsw = swz(lrow);
+    // Chunks of the k-steps (16 k = two chunks; lanes 32-63 take the second).
+    const int ch0 = lgrp ^ lsw, ch1 = (2 + lgrp) ^ lsw, ch2 = (4 + lgrp) ^ lsw, ch3 = (6 + lgrp) ^ lsw;
     PackedFrag<NPL, TM, TN> f0, f1;
     if (kt0 < kt1) {
         MBH_DMA(kt0, L0);
         MBH_SYNC();
         if (kt0 + 1 < kt1) MBH_DMA(kt0 + 1, L1);
-        packed_frag_read<NPL, TM, TN>(f0, L0, OA1, OB0, OB1, wm0, wn0, lrow, ch0);
+        packed_frag_read<NPL, TM, TN, CPR>(f0, L0, OA1, OB0, OB1, wm0, wn0, lrow, ch0);
     }
     // Straight-line body, one exit: the accumulators stay in place.  The last
     // step's read of the next k-tile may read a stale buffer (never used).
+    // Inside a k-tile, step s is multiplied from f[s % 2] while step s+1 is
+    // read into the other set (KS is even, so every k-tile starts in f0).
     int kt = kt0;
     for (; kt + 1 < kt1; kt += 2) {
         // L0: k-tile kt (f0 = its step 0); L1: k-tile kt+1 in flight
         MBH_STEP(f1, L0, ch1, f0);
+        if constexpr (KS == 4) {
+            MBH_STEP(f0, L0, ch2, f1);
+            MBH_STEP(f1, L0, ch3, f0);
+        }
         MBH_SYNC();
         if (kt + 2 < kt1) MBH_DMA(kt + 2, L0);
         MBH_STEP(f0, L1, ch0, f1);
         // L1: k-tile kt+1 (f0 = its step 0); L0: k-tile kt+2 in flight
         MBH_STEP(f1, L1, ch1, f0);
+        if constexpr (KS == 4) {
+            MBH_STEP(f0, L1, ch2, f1);
+            MBH_STEP(f1, L1, ch3, f0);
+        }
         MBH_SYNC();
         if (kt + 3 < kt1) MBH_DMA(kt + 3, L1);
         MBH_STEP(f0, L0, ch0, f1);
     }
     if (kt < kt1) {  // odd count: k-tile kt in L0, f0 = its step 0
         MBH_STEP(f1, L0, ch1, f0);
+        if constexpr (KS == 4) {
+            MBH_STEP(f0, L0, ch2, f1);
+            MBH_STEP(f1, L0, ch3, f0);
+        }
         packed_frag_mfma<NPROD, NPL, TM, TN, F16>(f1, acc);
     }
 #undef MBH_SYNC
@@ -741,19 +774,22 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
         gemm_packed_tile_dma<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
 }
 
+// k per packed row of the native 16-bit gemm (its pack pass uses the same).
+constexpr int GEMM16_BK = 64;
+
 // Native 16-bit gemm: one plane per operand holding f16 (F16) or bf16 values
 // (packed, or the caller's k-contiguous operand read in place), products on
 // the MFMA of that format with fp32 accumulation, C stored in the same 16-bit
 // format.
 template <bool F16, int BM, int BN, int WAVES_M, int WAVES_N>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed16_kernel(PackedGemmParams p) {
-    typedef PackedTileLds<1, BM, BN> L;
+    typedef PackedTileLds<1, BM, BN, GEMM16_BK / 8> L;
     __shared__ u32x4 lds0[L::TOTAL / 2];
     __shared__ u32x4 lds1[L::TOTAL / 2];
     int tm, tn;
     if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
     else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
-    gemm_packed_tile_pipe<1, BM, BN, WAVES_M, WAVES_N, F16 ? 1 : 2, F16, true>(p, tm, tn, lds0, lds1);
+    gemm_packed_tile_pipe<1, BM, BN, WAVES_M, WAVES_N, F16 ? 1 : 2, F16, true, GEMM16_BK>(p, tm, tn, lds0, lds1);
 }
 
 }  // namespace mbh

@@ -146,7 +146,7 @@ struct Pack16Params {
     int64_t ld;
     int64_t R, K;  // logical rows (m or n) and depth k of op(X)
     int64_t Rp;    // rows of the packed plane (whole macro-tiles)
-    int ktiles;    // ceil(K / 32)
+    int ktiles;    // ceil(K / PK)
     uint16_t* out;
     int vec;       // X 16-byte aligned and ld % 8 == 0
 };
@@ -162,10 +162,9 @@ struct Pack16Params {
 //     consecutive rows of one column, and the 8 x 8 block is transposed in
 //     registers before the store.
 // Rows >= R and k >= K are written as zeros; rows >= Rp and k-tiles >=
-// ktiles are not written.
-template <bool KCONTIG, int PR>
+// ktiles are not written.  PK: k per packed row (32 or 64).
+template <bool KCONTIG, int PR, int PK>
 __device__ inline void pack16_block(const Pack16Params& p, int64_t r0, int64_t k0) {
-    constexpr int PK = 32;
     const int tid = threadIdx.x;
     const MBH_GLOBAL uint16_t* const X = (const MBH_GLOBAL uint16_t*)p.X;
     const int64_t gr0 = KCONTIG ? r0 + tid / 16 : r0 + 8 * (tid % 16);
@@ -185,7 +184,7 @@ __device__ inline void pack16_block(const Pack16Params& p, int64_t r0, int64_t k
 #pragma unroll
             for (int w = 0; w < 4; ++w) v[j][w] = elem(j, 2 * w) | (elem(j, 2 * w + 1) << 16);
     }
-    // gk0 is a multiple of 8, so a thread's 8 k stay inside one 32-k tile.
+    // gk0 is a multiple of 8, so a thread's 8 k stay inside one k-tile.
     if (gk0 / PK >= p.ktiles) return;
     auto dst = [&](int64_t gr) {
         return (MBH_GLOBAL u32x4*)(p.out + (((gr / PR) * p.ktiles + gk0 / PK) * PR + gr % PR) * PK + gk0 % PK);
@@ -212,14 +211,14 @@ __device__ inline void pack16_block(const Pack16Params& p, int64_t r0, int64_t k
 
 // Both operands in one launch: workgroups [0, nblk_a) pack op(A) (gx_a row
 // blocks of 128 x k blocks of 128), the rest op(B)^T.
-template <bool KA, bool KB, int PR>
+template <bool KA, bool KB, int PR, int PK>
 __global__ __launch_bounds__(256) void pack16_kernel(Pack16Params a, Pack16Params b, int gx_a, int nblk_a, int gx_b) {
     int w = (int)blockIdx.x;
     if (w < nblk_a) {
-        pack16_block<KA, PR>(a, (int64_t)(w % gx_a) * 128, (int64_t)(w / gx_a) * 128);
+        pack16_block<KA, PR, PK>(a, (int64_t)(w % gx_a) * 128, (int64_t)(w / gx_a) * 128);
     } else {
         w -= nblk_a;
-        pack16_block<KB, PR>(b, (int64_t)(w % gx_b) * 128, (int64_t)(w / gx_b) * 128);
+        pack16_block<KB, PR, PK>(b, (int64_t)(w % gx_b) * 128, (int64_t)(w / gx_b) * 128);
     }
 }
 

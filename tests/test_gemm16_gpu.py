@@ -133,12 +133,12 @@ def test_integer_data_bit_exact(ta, tb, fmt):
 @pytest.mark.parametrize("fmt", FMTS)
 @pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
 def test_in_place_operands_bit_exact(ta, tb, fmt):
-    # k % 32 == 0 and ld % 8 == 0: a k-contiguous operand (op(A) when A is
+    # k % 64 == 0 and ld % 8 == 0: a k-contiguous operand (op(A) when A is
     # transposed, op(B)^T when B is not) is read in place, with its rows past
     # the last clamped; the other one is packed.  A base one element off the
-    # 16-byte alignment falls back to packing.
+    # 16-byte alignment, or k = 96, falls back to packing.
     rng = np.random.default_rng(26)
-    for (m, n, k) in [(1, 1, 32), (129, 65, 64), (300, 257, 96), (64, 300, 256)]:
+    for (m, n, k) in [(1, 1, 64), (129, 65, 64), (300, 257, 128), (64, 300, 256), (65, 33, 96)]:
         for shift in (0, 1):
             av, bv, cv, lda, ldb, ldc, a, b, c = int_case(rng, fmt, ta, tb, m, n, k, pad=8)
             a = np.concatenate([np.zeros(shift, np.uint16), a])
@@ -154,13 +154,14 @@ def test_in_place_operands_bit_exact(ta, tb, fmt):
 
 
 @pytest.mark.parametrize("fmt", FMTS)
-@pytest.mark.parametrize("shape,ta,tb", [((2048, 1024, 96), "N", "N"), ((2048, 1024, 96), "T", "T"),
+@pytest.mark.parametrize("shape,ta,tb", [((2048, 1024, 128), "N", "N"), ((2048, 1024, 128), "T", "T"),
                                          ((2048, 1024, 96), "T", "N"), ((4096, 4096, 64), "N", "N"),
-                                         ((4096, 4096, 64), "T", "N"), ((4096, 4096, 72), "T", "N")])
+                                         ((4096, 4096, 192), "T", "N"), ((4096, 4096, 72), "T", "N")])
 def test_large_tiles_bit_exact(shape, ta, tb, fmt):
     # 2048 x 1024 takes the 128 x 128 tiles, 4096^2 the 256 x 256 ones; with
-    # k = 64 or 96 (ld padded by 8) the k-contiguous operands are read in
-    # place, with k = 72 both are packed.
+    # k a multiple of 64 (ld padded by 8) the k-contiguous operands are read
+    # in place, with k = 96 or 72 both are packed (a k-tail inside the last
+    # 64-deep k-tile).
     m, n, k = shape
     rng = np.random.default_rng(22)
     av, bv, cv, lda, ldb, ldc, a, b, c = int_case(rng, fmt, ta, tb, m, n, k, pad=8)
