@@ -325,8 +325,13 @@ def dgemm(torch, mb, reps):
     ms = timed(torch, lambda: mb.gemm("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n), max(3, reps // 2))
     ref = A.view(n, n).t() @ B.view(n, n).t()
     e = err_stats(torch, C.view(n, n).t(), ref)
+    # torch.mm (the ROCm fp64 library gemm) on the same operands as a
+    # yardstick: column-major C^T = B^T A^T on the row-major views.
+    out = torch.empty(n, n, device="cuda", dtype=torch.float64)
+    tms = timed(torch, lambda: torch.mm(B.view(n, n), A.view(n, n), out=out), max(3, reps // 2))
     emit({"config": "dgemm", "routine": "dgemm NN 8192^3", "precision": "fp64", "ms": ms,
-          "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK["fp64"], "diff_vs_torch_fp64": e})
+          "tflops": fl / ms / 1e9, "roofline_frac": fl / ms / 1e9 / PEAK["fp64"], "diff_vs_torch_fp64": e,
+          "torch_mm_ms": tms, "torch_mm_tflops": fl / tms / 1e9})
 
 
 def gemm16(torch, mb, reps):

@@ -135,12 +135,52 @@ def test_dgemm_vs_oracle(ta, tb):
         assert (np.abs(got.reshape(n, m).T - want.reshape(n, m).T) <= bound).all()
 
 
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
+@pytest.mark.parametrize("k,pad", [(512, 0), (520, 0), (512, 1)])
+def test_dgemm_large_sampled(ta, tb, k, pad):
+    """2120 x 2088 fp64.  With k % 16 == 0 and 16-byte aligned operands the
+    whole 128 x 128 tiles run the interior kernel and the 72-row / 40-column
+    strips the generic one; k = 520 or an odd ld take the generic kernel
+    throughout.  Checked on 4096 sampled entries plus every entry of the rows
+    and columns next to the seams and the last ones, against fp64 dot
+    products: |C - C_ref| <= 4k u64 (|alpha| |A||B| + |beta C|)."""
+    m, n = 2048 + 72, 2048 + 40
+    g = torch.Generator(device="cuda").manual_seed(31)
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    lda, ldb = ar + pad, br + pad
+    A = torch.randn(lda * ac, device="cuda", generator=g, dtype=torch.float64)
+    B = torch.randn(ldb * bc, device="cuda", generator=g, dtype=torch.float64)
+    C0 = torch.randn(m * n, device="cuda", generator=g, dtype=torch.float64)
+    C = C0.clone()
+    alpha, beta = 0.75, -1.5
+    assert mb.gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, m) == 0
+    torch.cuda.synchronize()
+    a = A.cpu().numpy()[: lda * ac].reshape(ac, lda).T[:ar]
+    b = B.cpu().numpy()[: ldb * bc].reshape(bc, ldb).T[:br]
+    opa = a if ta == "N" else a.T
+    opb = b if tb == "N" else b.T
+    c0 = C0.cpu().numpy().reshape(n, m).T
+    got = C.cpu().numpy().reshape(n, m).T
+    rng = np.random.default_rng(32)
+    seams = [0, 2047, 2048, m - 1]
+    rows = np.concatenate([rng.integers(0, m, 4096), np.repeat(seams, n), np.tile(np.arange(m), 4)])
+    cols = np.concatenate([rng.integers(0, n, 4096), np.tile(np.arange(n), 4), np.repeat([0, 2047, 2048, n - 1], m)])
+    ra, cb = opa[rows], opb[:, cols].T
+    want = alpha * np.einsum("ij,ij->i", ra, cb) + beta * c0[rows, cols]
+    bound = 4 * k * 2.0 ** -53 * (abs(alpha) * np.einsum("ij,ij->i", np.abs(ra), np.abs(cb)) +
+                                  abs(beta) * np.abs(c0[rows, cols])) + 1e-300
+    err = np.abs(got[rows, cols] - want)
+    assert (err <= bound).all(), float((err / bound).max())
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
 def test_large_sampled(ta, tb, prec):
-    """2048 x 1536 x 2304 (ragged against the 128-tiles) on the vector path,
-    priced on 4096 sampled entries by the oracle; plus linearity in alpha."""
-    m, n, k = 2048 + 64, 1536 + 32, 2304
+    """2112 x 2080 x 2304 (ragged against the 128-tiles, 17 x 17 of them) on
+    the vector path, priced on 4096 sampled entries by the oracle; plus
+    linearity in alpha."""
+    m, n, k = 2048 + 64, 2048 + 32, 2304
     g = torch.Generator(device="cuda").manual_seed(14)
     ar, ac = (m, k) if ta == "N" else (k, m)
     br, bc = (k, n) if tb == "N" else (n, k)
