@@ -59,10 +59,17 @@ hipError_t launch_fp_split(bool at, bool bt, const GemmParams<T>& p, hipStream_t
     dim3 grid;
     if (!grid_for<GemmParams<T>, BM, BN>(q, 0, grid)) return hipErrorInvalidValue;
     const dim3 block(256);
-    if (!at && !bt) gemm_fp_interior_kernel<T, false, false, BM, BN, BK><<<grid, block, 0, s>>>(q);
-    else if (!at && bt) gemm_fp_interior_kernel<T, false, true, BM, BN, BK><<<grid, block, 0, s>>>(q);
-    else if (at && !bt) gemm_fp_interior_kernel<T, true, false, BM, BN, BK><<<grid, block, 0, s>>>(q);
-    else gemm_fp_interior_kernel<T, true, true, BM, BN, BK><<<grid, block, 0, s>>>(q);
+    if constexpr (sizeof(T) == 8 && BM == 128 && BN == 128 && BK == 16) {
+        if (!at && !bt) dgemm_dma_kernel<false, false><<<grid, block, 0, s>>>(q);
+        else if (!at && bt) dgemm_dma_kernel<false, true><<<grid, block, 0, s>>>(q);
+        else if (at && !bt) dgemm_dma_kernel<true, false><<<grid, block, 0, s>>>(q);
+        else dgemm_dma_kernel<true, true><<<grid, block, 0, s>>>(q);
+    } else {
+        if (!at && !bt) gemm_fp_interior_kernel<T, false, false, BM, BN, BK><<<grid, block, 0, s>>>(q);
+        else if (!at && bt) gemm_fp_interior_kernel<T, false, true, BM, BN, BK><<<grid, block, 0, s>>>(q);
+        else if (at && !bt) gemm_fp_interior_kernel<T, true, false, BM, BN, BK><<<grid, block, 0, s>>>(q);
+        else gemm_fp_interior_kernel<T, true, true, BM, BN, BK><<<grid, block, 0, s>>>(q);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mi < p.m) {  // rows [mi, m) x all columns

@@ -388,4 +388,134 @@ __global__ __launch_bounds__(256) void gemm_fp_interior_kernel(GemmArgs<T, T, T,
     gemm_fp_tile<T, T, T, T, AT, BT, BM, BN, BK, true>(p, tm, tn, lds);
 }
 
+// fp64 interior tile with direct-to-LDS panel loads (no register staging):
+// 128 x 128 x 16, 2 x 2 waves of 64 x 64 (4 x 4 blocks of
+// v_mfma_f64_16x16x4f64), two LDS buffers, one vmcnt wait and one barrier
+// per k-tile.  Only for interior tiles: operands 16-byte aligned with even
+// ld, k a multiple of 16, the tile inside m x n (no masking at all).
+// Each operand panel is copied by global_load_lds_dwordx4 (a wave-load moves
+// 64 chunks of two doubles to 1 KB of consecutive LDS) in one of two forms:
+//   MN (the panel's rows are contiguous in HBM: op(A) for A 'N', op(B)^T for
+//     B 'T'): LDS [k][128 + 16] doubles, one wave-load per k row; lane l of
+//     a fragment read takes (k = l/16, row = l%16): 16 consecutive doubles
+//     per k, the four k rows 1152 B apart.
+//   K (k contiguous: op(A) for A 'T', op(B)^T for B 'N'): LDS [row][16]
+//     doubles = 8 chunks per row, chunk c of row r stored in slot
+//     c ^ (r % 8), one wave-load per 8 rows (each lane fetches the global
+//     chunk that belongs in its slot); a fragment read's 16 rows then hit 8
+//     distinct slots at two row parities, no bank conflict.
+template <bool MN>
+struct Dma64Panel {
+    static constexpr bool ROWS_CONTIGUOUS = MN;
+    static constexpr int S = 144;                      // MN row stride (doubles)
+    static constexpr int DOUBLES = MN ? 16 * S : 128 * 16;
+    // Global source of this lane for wave-load v of the k-tile at k0 (the
+    // panel's first row r0), and the LDS destination base (wave-uniform).
+    __device__ static inline const double* src(const double* X, int64_t ld, int64_t r0, int64_t k0, int w, int lane) {
+        if (MN) return X + r0 + 2 * lane + (k0 + w) * ld;         // k row w
+        const int r = 8 * w + (lane >> 3), c = (lane & 7) ^ (r & 7);  // rows 8w .. 8w+7
+        return X + k0 + 2 * c + (r0 + r) * ld;
+    }
+    __device__ static inline int dst(int w) { return MN ? w * S : w * 128; }
+    // Fragment element of lane l: (row = rb + l%16, k = kb + l/16).
+    __device__ static inline int frag(int rb, int kb, int lane) {
+        const int row = rb + (lane & 15), k = kb + (lane >> 4);
+        if (MN) return k * S + row;
+        return row * 16 + (((k >> 1) ^ (row & 7)) << 1) + (k & 1);
+    }
+};
+
+template <bool AT, bool BT>
+__global__ __launch_bounds__(256) void dgemm_dma_kernel(GemmArgs<double, double, double, double> p) {
+    constexpr int BM = 128, BN = 128, BK = 16;
+    typedef Dma64Panel<!AT> PA;  // op(A) rows contiguous iff A is 'N'
+    typedef Dma64Panel<BT> PB;   // op(B)^T rows contiguous iff B is 'T'
+    constexpr int BUF = PA::DOUBLES + PB::DOUBLES;
+    __shared__ __attribute__((aligned(16))) double lds[2 * BUF];
+    int tm, tn;
+    tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
+    MBH_LDS double* const L = (MBH_LDS double*)lds;
+    // 16 wave-loads per operand per k-tile, four per wave: w = wave + 4v.
+    const double* ga[4];
+    const double* gb[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        ga[v] = PA::src(p.A, p.lda, m0, 0, wave + 4 * v, lane);
+        gb[v] = PB::src(p.B, p.ldb, n0, 0, wave + 4 * v, lane);
+    }
+    // Per k-tile the sources advance by 16 k: MN panels by 16 rows of ld,
+    // K panels by 16 doubles.
+    const int64_t sa = PA::ROWS_CONTIGUOUS ? 16 * p.lda : 16;
+    const int64_t sb = PB::ROWS_CONTIGUOUS ? 16 * p.ldb : 16;
+#define MBH_DMA64(kt, BASE)                                                                                     \
+    do {                                                                                                        \
+        _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                                         \
+            __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(ga[v] + (int64_t)(kt) * sa),              \
+                                             (MBH_LDS void*)(L + (BASE) + PA::dst(wave + 4 * v)), 16, 0, 0);    \
+            __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gb[v] + (int64_t)(kt) * sb),              \
+                                             (MBH_LDS void*)(L + (BASE) + PA::DOUBLES + PB::dst(wave + 4 * v)), \
+                                             16, 0, 0);                                                         \
+        }                                                                                                       \
+        __builtin_amdgcn_sched_barrier(0);                                                                      \
+    } while (0)
+
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    auto ktile = [&](int base) {
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = L[base + PA::frag(wm0 + 16 * i, 4 * ks, lane)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = L[base + PA::DOUBLES + PB::frag(wn0 + 16 * j, 4 * ks, lane)];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+        }
+    };
+    const int nk = (int)(p.k / BK);
+    MBH_DMA64(0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int kt = 0;
+    for (; kt + 1 < nk; kt += 2) {
+        MBH_DMA64(kt + 1, BUF);
+        ktile(0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (kt + 2 < nk) MBH_DMA64(kt + 2, 0);
+        ktile(BUF);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+    if (kt < nk) ktile(0);
+#undef MBH_DMA64
+
+    // Register r of block (i, j) holds C(m0 + wm0 + 16i + lane%16,
+    // n0 + wn0 + 16j + 4r + lane/16); the tile is interior.
+    const double alpha = p.alpha, beta = p.beta;
+    MBH_GLOBAL double* const C = (MBH_GLOBAL double*)p.C;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                MBH_GLOBAL double* cp = C + (m0 + wm0 + 16 * i + (lane & 15)) +
+                                        (n0 + wn0 + 16 * j + acc_col<16>(r, lane)) * p.ldc;
+                double v = alpha * acc[i][j][r];
+                if (beta != 0.0) v += beta * *cp;
+                *cp = v;
+            }
+}
+
 }  // namespace mbh
