@@ -42,16 +42,19 @@ hipError_t launch_fp(bool at, bool bt, GemmArgs<TA, TB, TC, T> p, int tri_grid, 
 }
 
 // Plain gemm (no triangle mask, full grid, every k-tile) on whole tiles: the
-// block of whole BM x BN tiles runs the interior kernel (unmasked loads
-// only) when both operands allow 16-byte loads and k is a multiple of BK;
-// the rows below it (all n columns) and the columns right of it go to the
-// generic kernel as sub-problems, with EM x EN tiles.  Offsetting op(A) by
-// a multiple of BM rows and op(B) by a multiple of BN columns keeps the
-// 16-byte alignment the vector flags record.
-template <typename T, int BM, int BN, int BK, int EM, int EN, int EK>
+// block of whole BM x BN tiles runs an interior kernel (unmasked loads only:
+// the direct-to-LDS gemm_dma_kernel when DMA, else the register-staged
+// tile) when both operands allow 16-byte loads and k is a multiple of the
+// interior kernel's k-tile; the rows below it (all n columns) and the
+// columns right of it go to the generic kernel as sub-problems, with EM x EN
+// tiles.  Offsetting op(A) by a multiple of BM rows and op(B) by a multiple
+// of BN columns keeps the 16-byte alignment the vector flags record.
+template <typename T, int BM, int BN, int BK, int EM, int EN, int EK, bool DMA>
 hipError_t launch_fp_split(bool at, bool bt, const GemmParams<T>& p, hipStream_t s) {
+    static_assert(!DMA || (BM == 128 && BN == 128), "the direct-to-LDS kernel has 128 x 128 tiles");
+    constexpr int IK = DMA ? DmaPanel<T, true>::BK : BK;
     const int64_t mi = p.m / BM * BM, ni = p.n / BN * BN;
-    if (!p.a_vec || !p.b_vec || p.k % BK != 0 || mi == 0 || ni == 0)
+    if (!p.a_vec || !p.b_vec || p.k % IK != 0 || mi == 0 || ni == 0)
         return launch_fp<T, T, T, T, EM, EN, EK>(at, bt, p, 0, s);
     GemmParams<T> q = p;
     q.m = mi;
@@ -59,11 +62,11 @@ hipError_t launch_fp_split(bool at, bool bt, const GemmParams<T>& p, hipStream_t
     dim3 grid;
     if (!grid_for<GemmParams<T>, BM, BN>(q, 0, grid)) return hipErrorInvalidValue;
     const dim3 block(256);
-    if constexpr (sizeof(T) == 8 && BM == 128 && BN == 128 && BK == 16) {
-        if (!at && !bt) dgemm_dma_kernel<false, false><<<grid, block, 0, s>>>(q);
-        else if (!at && bt) dgemm_dma_kernel<false, true><<<grid, block, 0, s>>>(q);
-        else if (at && !bt) dgemm_dma_kernel<true, false><<<grid, block, 0, s>>>(q);
-        else dgemm_dma_kernel<true, true><<<grid, block, 0, s>>>(q);
+    if constexpr (DMA) {
+        if (!at && !bt) gemm_dma_kernel<T, false, false><<<grid, block, 0, s>>>(q);
+        else if (!at && bt) gemm_dma_kernel<T, false, true><<<grid, block, 0, s>>>(q);
+        else if (at && !bt) gemm_dma_kernel<T, true, false><<<grid, block, 0, s>>>(q);
+        else gemm_dma_kernel<T, true, true><<<grid, block, 0, s>>>(q);
     } else {
         if (!at && !bt) gemm_fp_interior_kernel<T, false, false, BM, BN, BK><<<grid, block, 0, s>>>(q);
         else if (!at && bt) gemm_fp_interior_kernel<T, false, true, BM, BN, BK><<<grid, block, 0, s>>>(q);
@@ -361,7 +364,7 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
                 // Plain products: interior kernel plus edge strips (8192^3:
                 // 126.8 -> 131.7 TFLOP/s, bit-identical).
                 if (!tri && !tri_grid && !krange)
-                    return launch_fp_split<float, 128, 128, 16, 64, 64, 16>(at, bt, p, s);
+                    return launch_fp_split<float, 128, 128, 16, 64, 64, 16, true>(at, bt, p, s);
                 return launch_fp<float, float, float, float, 128, 128, 16>(at, bt, p, tri_grid, s);
             }
             return launch_fp<float, float, float, float, 64, 64, 16>(at, bt, p, tri_grid, s);
@@ -387,7 +390,7 @@ hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha
     // generic kernel measured 63.5 with 128 x 128 tiles, 60.6 with 64 x 64
     // and 62.1 with 128 x 64).
     if (!tri && !tri_grid && !krange && tile_count(m, n, 128, 128, 0) >= 256)
-        return launch_fp_split<double, 128, 128, 16, 64, 64, 16>(at, bt, p, s);
+        return launch_fp_split<double, 128, 128, 16, 64, 64, 16, true>(at, bt, p, s);
     return launch_fp<double, double, double, double, 64, 64, 16>(at, bt, p, tri_grid, s);
 }
 

@@ -388,132 +388,164 @@ __global__ __launch_bounds__(256) void gemm_fp_interior_kernel(GemmArgs<T, T, T,
     gemm_fp_tile<T, T, T, T, AT, BT, BM, BN, BK, true>(p, tm, tn, lds);
 }
 
-// fp64 interior tile with direct-to-LDS panel loads (no register staging):
-// 128 x 128 x 16, 2 x 2 waves of 64 x 64 (4 x 4 blocks of
-// v_mfma_f64_16x16x4f64), two LDS buffers, one vmcnt wait and one barrier
-// per k-tile.  Only for interior tiles: operands 16-byte aligned with even
-// ld, k a multiple of 16, the tile inside m x n (no masking at all).
+// Interior tile with direct-to-LDS panel loads (no register staging), fp64
+// or fp32: 128 x 128 x BK, 2 x 2 waves of 64 x 64 (4 x 4 blocks of
+// v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, whose operands share one
+// layout: lane l holds (row l%16, k l/16)), two LDS buffers, one vmcnt wait
+// and one barrier per k-tile.  BK = 128 B / sizeof(T) (16 doubles, 32
+// floats), so a k-contiguous panel row is one 128-byte line.  Only for
+// interior tiles: operands 16-byte aligned with ld a multiple of the 16-byte
+// vector, k a multiple of BK, the tile inside m x n (no masking at all).
 // Each operand panel is copied by global_load_lds_dwordx4 (a wave-load moves
-// 64 chunks of two doubles to 1 KB of consecutive LDS) in one of two forms:
+// 64 16-byte chunks to 1 KB of consecutive LDS) in one of two forms:
 //   MN (the panel's rows are contiguous in HBM: op(A) for A 'N', op(B)^T for
-//     B 'T'): LDS [k][128 + 16] doubles, one wave-load per k row; lane l of
-//     a fragment read takes (k = l/16, row = l%16): 16 consecutive doubles
-//     per k, the four k rows 1152 B apart.
-//   K (k contiguous: op(A) for A 'T', op(B)^T for B 'N'): LDS [row][16]
-//     doubles = 8 chunks per row, chunk c of row r stored in slot
-//     c ^ (r % 8), one wave-load per 8 rows (each lane fetches the global
-//     chunk that belongs in its slot); a fragment read's 16 rows then hit 8
-//     distinct slots at two row parities, no bank conflict.
-template <bool MN>
-struct Dma64Panel {
+//     B 'T'): k-major, one wave-load per KPW = 1 (fp64) or 2 (fp32) k rows of
+//     128 elements, the wave-load groups 1152 B apart; lane l of a fragment
+//     read takes (k = l/16, row = l%16): 16 consecutive elements per k.
+//   K (k contiguous: op(A) for A 'T', op(B)^T for B 'N'): [row][BK], chunk c
+//     of row r stored in slot c ^ (r % 8), one wave-load per 8 rows (each
+//     lane fetches the global chunk that belongs in its slot), so the 16 rows
+//     of a fragment read spread over distinct slots.
+template <typename T, bool MN>
+struct DmaPanel {
     static constexpr bool ROWS_CONTIGUOUS = MN;
-    static constexpr int S = 144;                      // MN row stride (doubles)
-    static constexpr int DOUBLES = MN ? 16 * S : 128 * 16;
-    // Global source of this lane for wave-load v of the k-tile at k0 (the
-    // panel's first row r0), and the LDS destination base (wave-uniform).
-    __device__ static inline const double* src(const double* X, int64_t ld, int64_t r0, int64_t k0, int w, int lane) {
-        if (MN) return X + r0 + 2 * lane + (k0 + w) * ld;         // k row w
-        const int r = 8 * w + (lane >> 3), c = (lane & 7) ^ (r & 7);  // rows 8w .. 8w+7
-        return X + k0 + 2 * c + (r0 + r) * ld;
+    static constexpr int CE = 16 / (int)sizeof(T);        // elements per chunk
+    static constexpr int BK = 8 * CE;                       // k per tile (128-byte rows)
+    static constexpr int KPW = 1024 / (128 * (int)sizeof(T));  // MN: k rows per wave-load
+    static constexpr int LPR = 64 / KPW;                    // MN: lanes per k row
+    static constexpr int GS = 1152 / (int)sizeof(T);        // MN: wave-load group stride
+    static constexpr int ELEMS = MN ? (BK / KPW) * GS : 128 * BK;
+    // Global source of this lane for wave-load w of the k-tile at k0 (the
+    // panel's first row r0); w in [0, 16).
+    __device__ static inline const T* src(const T* X, int64_t ld, int64_t r0, int64_t k0, int w, int lane) {
+        if (MN) return X + r0 + CE * (lane % LPR) + (k0 + w * KPW + lane / LPR) * ld;
+        const int r = 8 * w + (lane >> 3), c = (lane & 7) ^ (r & 7);
+        return X + k0 + CE * c + (r0 + r) * ld;
     }
-    __device__ static inline int dst(int w) { return MN ? w * S : w * 128; }
+    // LDS destination (wave-uniform) of wave-load w.
+    __device__ static inline int dst(int w) { return MN ? w * GS : w * 8 * BK; }
     // Fragment element of lane l: (row = rb + l%16, k = kb + l/16).
     __device__ static inline int frag(int rb, int kb, int lane) {
         const int row = rb + (lane & 15), k = kb + (lane >> 4);
-        if (MN) return k * S + row;
-        return row * 16 + (((k >> 1) ^ (row & 7)) << 1) + (k & 1);
+        if (MN) return (k / KPW) * GS + (k % KPW) * 128 + row;
+        return row * BK + (((k / CE) ^ (row & 7)) * CE) + (k % CE);
     }
 };
 
-template <bool AT, bool BT>
-__global__ __launch_bounds__(256) void dgemm_dma_kernel(GemmArgs<double, double, double, double> p) {
-    constexpr int BM = 128, BN = 128, BK = 16;
-    typedef Dma64Panel<!AT> PA;  // op(A) rows contiguous iff A is 'N'
-    typedef Dma64Panel<BT> PB;   // op(B)^T rows contiguous iff B is 'T'
-    constexpr int BUF = PA::DOUBLES + PB::DOUBLES;
-    __shared__ __attribute__((aligned(16))) double lds[2 * BUF];
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef f64x4 acc_t;
+    __device__ static inline acc_t mma(double x, double y, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+    }
+    // C column offset of register r (operands swapped: C^T is computed);
+    // the f64 form interleaves the four lane groups row by row.
+    __device__ static inline int col(int r, int lane) { return 4 * r + (lane >> 4); }
+};
+template <>
+struct Mfma16<float> {
+    typedef f32x4 acc_t;
+    __device__ static inline acc_t mma(float x, float y, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+    }
+    // The f32 form gives lane group g the four consecutive outputs 4g .. 4g+3.
+    __device__ static inline int col(int r, int lane) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T, bool AT, bool BT>
+__global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
+    constexpr int BM = 128, BN = 128;
+    typedef DmaPanel<T, !AT> PA;  // op(A) rows contiguous iff A is 'N'
+    typedef DmaPanel<T, BT> PB;   // op(B)^T rows contiguous iff B is 'T'
+    typedef Mfma16<T> M;
+    constexpr int BK = PA::BK;
+    constexpr int BUF = PA::ELEMS + PB::ELEMS;
+    __shared__ __attribute__((aligned(16))) T lds[2 * BUF];
     int tm, tn;
     tile_coords(p.tiles_m, p.tiles_n, tm, tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
-    MBH_LDS double* const L = (MBH_LDS double*)lds;
+    MBH_LDS T* const L = (MBH_LDS T*)lds;
     // 16 wave-loads per operand per k-tile, four per wave: w = wave + 4v.
-    const double* ga[4];
-    const double* gb[4];
+    const T* ga[4];
+    const T* gb[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         ga[v] = PA::src(p.A, p.lda, m0, 0, wave + 4 * v, lane);
         gb[v] = PB::src(p.B, p.ldb, n0, 0, wave + 4 * v, lane);
     }
-    // Per k-tile the sources advance by 16 k: MN panels by 16 rows of ld,
-    // K panels by 16 doubles.
-    const int64_t sa = PA::ROWS_CONTIGUOUS ? 16 * p.lda : 16;
-    const int64_t sb = PB::ROWS_CONTIGUOUS ? 16 * p.ldb : 16;
-#define MBH_DMA64(kt, BASE)                                                                                     \
+    // Per k-tile the sources advance by BK: MN panels by BK rows of ld, K
+    // panels by BK elements.
+    const int64_t sa = PA::ROWS_CONTIGUOUS ? BK * p.lda : BK;
+    const int64_t sb = PB::ROWS_CONTIGUOUS ? BK * p.ldb : BK;
+#define MBH_DMAK(kt, BASE)                                                                                      \
     do {                                                                                                        \
         _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                                         \
             __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(ga[v] + (int64_t)(kt) * sa),              \
                                              (MBH_LDS void*)(L + (BASE) + PA::dst(wave + 4 * v)), 16, 0, 0);    \
             __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(gb[v] + (int64_t)(kt) * sb),              \
-                                             (MBH_LDS void*)(L + (BASE) + PA::DOUBLES + PB::dst(wave + 4 * v)), \
+                                             (MBH_LDS void*)(L + (BASE) + PA::ELEMS + PB::dst(wave + 4 * v)),   \
                                              16, 0, 0);                                                         \
         }                                                                                                       \
         __builtin_amdgcn_sched_barrier(0);                                                                      \
     } while (0)
 
-    f64x4 acc[4][4];
+    typename M::acc_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
     auto ktile = [&](int base) {
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
-            double a[4], b[4];
+            T a[4], b[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) a[i] = L[base + PA::frag(wm0 + 16 * i, 4 * ks, lane)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = L[base + PA::DOUBLES + PB::frag(wn0 + 16 * j, 4 * ks, lane)];
+            for (int j = 0; j < 4; ++j) b[j] = L[base + PA::ELEMS + PB::frag(wn0 + 16 * j, 4 * ks, lane)];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < 4; ++j) acc[i][j] = M::mma(b[j], a[i], acc[i][j]);
         }
     };
     const int nk = (int)(p.k / BK);
-    MBH_DMA64(0, 0);
+    MBH_DMAK(0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     int kt = 0;
     for (; kt + 1 < nk; kt += 2) {
-        MBH_DMA64(kt + 1, BUF);
+        MBH_DMAK(kt + 1, BUF);
         ktile(0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (kt + 2 < nk) MBH_DMA64(kt + 2, 0);
+        if (kt + 2 < nk) MBH_DMAK(kt + 2, 0);
         ktile(BUF);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
     if (kt < nk) ktile(0);
-#undef MBH_DMA64
+#undef MBH_DMAK
 
     // Register r of block (i, j) holds C(m0 + wm0 + 16i + lane%16,
-    // n0 + wn0 + 16j + 4r + lane/16); the tile is interior.
-    const double alpha = p.alpha, beta = p.beta;
-    MBH_GLOBAL double* const C = (MBH_GLOBAL double*)p.C;
+    // n0 + wn0 + 16j + M::col(r, lane)); the tile is interior.
+    const T alpha = p.alpha, beta = p.beta;
+    MBH_GLOBAL T* const C = (MBH_GLOBAL T*)p.C;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                MBH_GLOBAL double* cp = C + (m0 + wm0 + 16 * i + (lane & 15)) +
-                                        (n0 + wn0 + 16 * j + acc_col<16>(r, lane)) * p.ldc;
-                double v = alpha * acc[i][j][r];
-                if (beta != 0.0) v += beta * *cp;
+                MBH_GLOBAL T* cp = C + (m0 + wm0 + 16 * i + (lane & 15)) + (n0 + wn0 + 16 * j + M::col(r, lane)) * p.ldc;
+                T v = alpha * acc[i][j][r];
+                if (beta != T(0)) v += beta * *cp;
                 *cp = v;
             }
 }
