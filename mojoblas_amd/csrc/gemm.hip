@@ -93,6 +93,25 @@ hipError_t launch_fp_split(bool at, bool bt, const GemmParams<T>& p, hipStream_t
     return hipSuccess;
 }
 
+// Whole products on the direct-to-LDS kernel, with any triangle mask,
+// triangle grid or k-range: for a C of whole 128 x 128 tiles, k a multiple
+// of the kernel's k-tile and operands that allow 16-byte loads.
+template <typename T>
+bool dma_whole_ok(const GemmParams<T>& p) {
+    return p.m % 128 == 0 && p.n % 128 == 0 && p.k % DmaPanel<T, true>::BK == 0 && p.a_vec && p.b_vec;
+}
+template <typename T>
+hipError_t launch_dma_whole(bool at, bool bt, GemmParams<T> p, int tri_grid, hipStream_t s) {
+    dim3 grid;
+    if (!grid_for<GemmParams<T>, 128, 128>(p, tri_grid, grid)) return hipErrorInvalidValue;
+    const dim3 block(256);
+    if (!at && !bt) gemm_dma_kernel<T, false, false><<<grid, block, 0, s>>>(p);
+    else if (!at && bt) gemm_dma_kernel<T, false, true><<<grid, block, 0, s>>>(p);
+    else if (at && !bt) gemm_dma_kernel<T, true, false><<<grid, block, 0, s>>>(p);
+    else gemm_dma_kernel<T, true, true><<<grid, block, 0, s>>>(p);
+    return hipGetLastError();
+}
+
 // Split-precision path: pack op(A) and op(B)^T into bf16 planes, then run the
 // packed macro-tile kernel.  256x256 tiles (8 waves) when that still gives one
 // tile per CU, else 128x128 (4 waves).
@@ -365,6 +384,7 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
                 // 126.8 -> 131.7 TFLOP/s, bit-identical).
                 if (!tri && !tri_grid && !krange)
                     return launch_fp_split<float, 128, 128, 16, 64, 64, 16, true>(at, bt, p, s);
+                if (dma_whole_ok(p)) return launch_dma_whole<float>(at, bt, p, tri_grid, s);
                 return launch_fp<float, float, float, float, 128, 128, 16>(at, bt, p, tri_grid, s);
             }
             return launch_fp<float, float, float, float, 64, 64, 16>(at, bt, p, tri_grid, s);
@@ -391,6 +411,8 @@ hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha
     // and 62.1 with 128 x 64).
     if (!tri && !tri_grid && !krange && tile_count(m, n, 128, 128, 0) >= 256)
         return launch_fp_split<double, 128, 128, 16, 64, 64, 16, true>(at, bt, p, s);
+    if (tile_count(m, n, 128, 128, tri_grid) >= (krange ? 1024 : 256) && dma_whole_ok(p))
+        return launch_dma_whole<double>(at, bt, p, tri_grid, s);
     return launch_fp<double, double, double, double, 64, 64, 16>(at, bt, p, tri_grid, s);
 }
 

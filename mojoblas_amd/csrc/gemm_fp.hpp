@@ -389,7 +389,8 @@ __global__ __launch_bounds__(256) void gemm_fp_interior_kernel(GemmArgs<T, T, T,
 }
 
 // Interior tile with direct-to-LDS panel loads (no register staging), fp64
-// or fp32: 128 x 128 x BK, 2 x 2 waves of 64 x 64 (4 x 4 blocks of
+// or fp32, for a C of whole tiles (or the block of whole tiles of a plain
+// product): 128 x 128 x BK, 2 x 2 waves of 64 x 64 (4 x 4 blocks of
 // v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, whose operands share one
 // layout: lane l holds (row l%16, k l/16)), two LDS buffers, one vmcnt wait
 // and one barrier per k-tile.  BK = 128 B / sizeof(T) (16 doubles, 32
@@ -463,8 +464,11 @@ __global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
     constexpr int BK = PA::BK;
     constexpr int BUF = PA::ELEMS + PB::ELEMS;
     __shared__ __attribute__((aligned(16))) T lds[2 * BUF];
+    // Tile enumeration, k-range and store mask as in gemm_fp_kernel
+    // (triangle grids for syrk / syr2k, triangular operands for trmm).
     int tm, tn;
-    tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
+    else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -515,22 +519,23 @@ __global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
                 for (int j = 0; j < 4; ++j) acc[i][j] = M::mma(b[j], a[i], acc[i][j]);
         }
     };
-    const int nk = (int)(p.k / BK);
-    MBH_DMAK(0, 0);
+    int kt0, kt1;
+    kt_range(p.krange, tm, tn, BM, BN, BK, (int)(p.k / BK), kt0, kt1);
+    if (kt0 < kt1) MBH_DMAK(kt0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    int kt = 0;
-    for (; kt + 1 < nk; kt += 2) {
+    int kt = kt0;
+    for (; kt + 1 < kt1; kt += 2) {
         MBH_DMAK(kt + 1, BUF);
         ktile(0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (kt + 2 < nk) MBH_DMAK(kt + 2, 0);
+        if (kt + 2 < kt1) MBH_DMAK(kt + 2, 0);
         ktile(BUF);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
-    if (kt < nk) ktile(0);
+    if (kt < kt1) ktile(0);
 #undef MBH_DMAK
 
     // Register r of block (i, j) holds C(m0 + wm0 + 16i + lane%16,
@@ -543,7 +548,9 @@ __global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                MBH_GLOBAL T* cp = C + (m0 + wm0 + 16 * i + (lane & 15)) + (n0 + wn0 + 16 * j + M::col(r, lane)) * p.ldc;
+                const int64_t row = m0 + wm0 + 16 * i + (lane & 15), col = n0 + wn0 + 16 * j + M::col(r, lane);
+                if (!tri_keep(p.tri, row, col)) continue;
+                MBH_GLOBAL T* cp = C + row + col * p.ldc;
                 T v = alpha * acc[i][j][r];
                 if (beta != T(0)) v += beta * *cp;
                 *cp = v;

@@ -260,3 +260,60 @@ def test_config3_ssyrk_8192x2048(prec):
     up_r, up_c = np.minimum(i, j), np.maximum(i, j)
     strict = up_r < up_c
     assert np.isnan(c[up_r[strict] + up_c[strict] * n]).all()
+
+
+@pytest.mark.parametrize("routine", ["trmm", "syrk", "syr2k"])
+@pytest.mark.parametrize("prec", ["fp32", "f64"])
+def test_whole_tile_products_sampled(routine, prec):
+    """4096-sized strmm / ssyrk / ssyr2k (and the fp64 forms): C is whole
+    128 x 128 tiles and k a multiple of the k-tile, so the triangular grids,
+    k-ranges and store masks run on the direct-to-LDS kernel.  4096 sampled
+    entries against fp64 dot products (bound (2k+2) u |alpha| |A||B| for
+    fp32, 4k u64 for fp64); the untouched triangle stays NaN."""
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = torch.Generator(device="cuda").manual_seed(41)
+    n, k = 4096, 256
+    rng = np.random.default_rng(42)
+    kw = {} if prec == "f64" else {"precision": prec}
+    alpha = 0.5
+    if routine == "trmm":
+        a = torch.randn(n * n, device="cuda", generator=g, dtype=dt)
+        b0 = torch.randn(n * n, device="cuda", generator=g, dtype=dt)
+        B = b0.clone()
+        assert mb.trmm("L", "L", "N", "N", n, n, alpha, a, n, B, n, **kw) == 0
+        torch.cuda.synchronize()
+        am = np.tril(a.cpu().numpy().reshape(n, n).T)
+        bm = b0.cpu().numpy().reshape(n, n).T
+        rows, cols = rng.integers(0, n, 4096), rng.integers(0, n, 4096)
+        ra, cb = am[rows].astype(np.float64), bm[:, cols].T.astype(np.float64)
+        got = B.cpu().numpy().reshape(n, n).T[rows, cols].astype(np.float64)
+        kk = n
+    else:
+        a = torch.randn(n * k, device="cuda", generator=g, dtype=dt)
+        b = torch.randn(n * k, device="cuda", generator=g, dtype=dt)
+        C = torch.full((n * n,), float("nan"), device="cuda", dtype=dt)
+        if routine == "syrk":
+            assert mb.syrk("L", "N", n, k, alpha, a, n, 0.0, C, n, **kw) == 0
+        else:
+            assert mb.syr2k("L", "N", n, k, alpha, a, n, b, n, 0.0, C, n, **kw) == 0
+        torch.cuda.synchronize()
+        am = a.cpu().numpy().reshape(k, n).T.astype(np.float64)
+        bm = b.cpu().numpy().reshape(k, n).T.astype(np.float64)
+        i, j = rng.integers(0, n, 4096), rng.integers(0, n, 4096)
+        rows, cols = np.maximum(i, j), np.minimum(i, j)
+        if routine == "syrk":
+            ra, cb = am[rows], am[cols]
+        else:
+            ra = np.concatenate([am[rows], bm[rows]], axis=1)
+            cb = np.concatenate([bm[cols], am[cols]], axis=1)
+        c = C.cpu().numpy().reshape(n, n).T
+        got = c[rows, cols].astype(np.float64)
+        strict = np.minimum(i, j) < np.maximum(i, j)
+        assert np.isnan(c[np.minimum(i, j)[strict], np.maximum(i, j)[strict]]).all()
+        kk = ra.shape[1]
+    want = alpha * np.einsum("ij,ij->i", ra, cb)
+    absab = abs(alpha) * np.einsum("ij,ij->i", np.abs(ra), np.abs(cb))
+    u = 2.0 ** -53 if prec == "f64" else H.U32
+    bound = (4 * kk * u if prec == "f64" else (2 * kk + 2) * u) * absab + 1e-300
+    err = np.abs(got - want)
+    assert (err <= bound).all(), float((err / bound).max())
