@@ -444,6 +444,21 @@ hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double
     // 64 x 64 x 16, faster there).
     if (shape == 1) return launch_fp<double, TA, TB, TC, 128, 32, 32>(at, bt, p, 0, s);
     if (shape == 2) return launch_fp<double, TA, TB, TC, 32, 128, 32>(at, bt, p, 0, s);
+    // fp32 operands and fp64 C in whole 128 x 128 tiles, enough of them:
+    // the direct-to-LDS tile with fp64 products.
+    if constexpr (sizeof(TA) == 4 && sizeof(TB) == 4 && sizeof(TC) == 8) {
+        if (m % 128 == 0 && n % 128 == 0 && k % 32 == 0 && p.a_vec && p.b_vec &&
+            tile_count(m, n, 128, 128, 0) >= (krange ? 1024 : 256)) {
+            dim3 grid;
+            if (!grid_for<GemmArgs<TA, TB, TC, double>, 128, 128>(p, 0, grid)) return hipErrorInvalidValue;
+            const dim3 block(256);
+            if (!at && !bt) gemm_dma_f64acc_kernel<false, false><<<grid, block, 0, s>>>(p);
+            else if (!at && bt) gemm_dma_f64acc_kernel<false, true><<<grid, block, 0, s>>>(p);
+            else if (at && !bt) gemm_dma_f64acc_kernel<true, false><<<grid, block, 0, s>>>(p);
+            else gemm_dma_f64acc_kernel<true, true><<<grid, block, 0, s>>>(p);
+            return hipGetLastError();
+        }
+    }
     return launch_fp<double, TA, TB, TC, 64, 64, 32>(at, bt, p, 0, s);
 }
 

@@ -455,15 +455,22 @@ struct Mfma16<float> {
     __device__ static inline int col(int r, int lane) { return 4 * (lane >> 4) + r; }
 };
 
-template <typename T, bool AT, bool BT>
-__global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
+// One tile: panels of storage type T (both operands), MFMA in TS (T or the
+// wider double, the fragment read widening each element), C in TC.
+template <typename T, typename TS, typename TC, bool AT, bool BT>
+struct DmaTileLds {
+    static constexpr int BUF = DmaPanel<T, !AT>::ELEMS + DmaPanel<T, BT>::ELEMS;
+    static constexpr int BYTES = 2 * BUF * (int)sizeof(T);
+};
+
+template <typename T, typename TS, typename TC, bool AT, bool BT>
+__device__ inline void gemm_dma_tile(const GemmArgs<T, T, TC, TS>& p, void* lds) {
     constexpr int BM = 128, BN = 128;
     typedef DmaPanel<T, !AT> PA;  // op(A) rows contiguous iff A is 'N'
     typedef DmaPanel<T, BT> PB;   // op(B)^T rows contiguous iff B is 'T'
-    typedef Mfma16<T> M;
+    typedef Mfma16<TS> M;
     constexpr int BK = PA::BK;
     constexpr int BUF = PA::ELEMS + PB::ELEMS;
-    __shared__ __attribute__((aligned(16))) T lds[2 * BUF];
     // Tile enumeration, k-range and store mask as in gemm_fp_kernel
     // (triangle grids for syrk / syr2k, triangular operands for trmm).
     int tm, tn;
@@ -504,15 +511,15 @@ __global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = TS(0);
     auto ktile = [&](int base) {
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
-            T a[4], b[4];
+            TS a[4], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = L[base + PA::frag(wm0 + 16 * i, 4 * ks, lane)];
+            for (int i = 0; i < 4; ++i) a[i] = (TS)L[base + PA::frag(wm0 + 16 * i, 4 * ks, lane)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = L[base + PA::ELEMS + PB::frag(wn0 + 16 * j, 4 * ks, lane)];
+            for (int j = 0; j < 4; ++j) b[j] = (TS)L[base + PA::ELEMS + PB::frag(wn0 + 16 * j, 4 * ks, lane)];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -540,8 +547,8 @@ __global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
 
     // Register r of block (i, j) holds C(m0 + wm0 + 16i + lane%16,
     // n0 + wn0 + 16j + M::col(r, lane)); the tile is interior.
-    const T alpha = p.alpha, beta = p.beta;
-    MBH_GLOBAL T* const C = (MBH_GLOBAL T*)p.C;
+    const TS alpha = p.alpha, beta = p.beta;
+    MBH_GLOBAL TC* const C = (MBH_GLOBAL TC*)p.C;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -550,11 +557,26 @@ __global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = m0 + wm0 + 16 * i + (lane & 15), col = n0 + wn0 + 16 * j + M::col(r, lane);
                 if (!tri_keep(p.tri, row, col)) continue;
-                MBH_GLOBAL T* cp = C + row + col * p.ldc;
-                T v = alpha * acc[i][j][r];
-                if (beta != T(0)) v += beta * *cp;
-                *cp = v;
+                MBH_GLOBAL TC* cp = C + row + col * p.ldc;
+                TS v = alpha * acc[i][j][r];
+                if (beta != TS(0)) v += beta * (TS)*cp;
+                *cp = (TC)v;
             }
+}
+
+template <typename T, bool AT, bool BT>
+__global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[DmaTileLds<T, T, T, AT, BT>::BYTES];
+    gemm_dma_tile<T, T, T, AT, BT>(p, lds);
+}
+
+// fp32 operands, fp64 products and accumulation, fp64 C (the refinement
+// residual of the triangular solve): fp32 panels in LDS, each element
+// widened at the fragment read, v_mfma_f64_16x16x4f64 on 32-deep k-tiles.
+template <bool AT, bool BT>
+__global__ __launch_bounds__(256) void gemm_dma_f64acc_kernel(GemmArgs<float, float, double, double> p) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[DmaTileLds<float, double, double, AT, BT>::BYTES];
+    gemm_dma_tile<float, double, double, AT, BT>(p, lds);
 }
 
 }  // namespace mbh

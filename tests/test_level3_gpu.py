@@ -317,3 +317,34 @@ def test_whole_tile_products_sampled(routine, prec):
     bound = (4 * kk * u if prec == "f64" else (2 * kk + 2) * u) * absab + 1e-300
     err = np.abs(got - want)
     assert (err <= bound).all(), float((err / bound).max())
+
+
+@pytest.mark.parametrize("side,uplo,ta", [("L", "L", "N"), ("L", "U", "T"), ("R", "U", "N"), ("R", "L", "T")])
+def test_refined_strsm_whole_tiles(side, uplo, ta):
+    """strsm at 4096 (whole 128-tiles, so the fp64 refinement residual runs
+    on the direct-to-LDS kernel): against an fp64 solve of the same fp32
+    inputs, the refined solution must be within 4 u32 max|X| and well below
+    the unrefined one's error (a residual that came out zero or wrong would
+    leave the unrefined error or worse)."""
+    m = n = 4096
+    g = torch.Generator(device="cuda").manual_seed(43)
+    a = (torch.rand(m, m, device="cuda", generator=g, dtype=torch.float64) * 2 - 1) / m
+    a = (torch.tril(a, -1) if uplo == "L" else torch.triu(a, 1)) + torch.diag(
+        1 + torch.rand(m, device="cuda", generator=g, dtype=torch.float64))
+    A = a.t().contiguous().float().reshape(-1)  # column-major buffer
+    b = (torch.rand(n, m, device="cuda", generator=g) * 2 - 1).reshape(-1)
+    alpha = 0.75
+    Xs = []
+    for refine in (0, 1):
+        B = b.clone()
+        assert mb.trsm(side, uplo, ta, "N", m, n, alpha, A, m, B, m, precision="fp32", refine=refine) == 0
+        Xs.append(B.view(n, m).t().double())
+    torch.cuda.synchronize()
+    a32 = A.view(m, m).t().double()
+    opa = a32 if ta == "N" else a32.t().contiguous()
+    upper = (uplo == "U") != (ta == "T")
+    rhs = alpha * b.view(n, m).t().double()
+    ref = torch.linalg.solve_triangular(opa, rhs, upper=upper, left=(side == "L"))
+    e0, e1 = ((X - ref).abs().max().item() for X in Xs)
+    assert e1 <= 4 * 2.0 ** -24 * ref.abs().max().item(), (e0, e1)
+    assert e1 <= 0.6 * e0, (e0, e1)
