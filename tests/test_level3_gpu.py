@@ -262,14 +262,19 @@ def test_config3_ssyrk_8192x2048(prec):
     assert np.isnan(c[up_r[strict] + up_c[strict] * n]).all()
 
 
-@pytest.mark.parametrize("routine", ["trmm", "syrk", "syr2k"])
+WHOLE_CASES = [("trmm", "L", "L", "N"), ("trmm", "L", "U", "N"), ("trmm", "R", "L", "T"), ("trmm", "R", "U", "N"),
+               ("syrk", None, "L", "N"), ("syrk", None, "U", "T"), ("syr2k", None, "L", "N"), ("syr2k", None, "U", "T")]
+
+
+@pytest.mark.parametrize("routine,side,uplo,trans", WHOLE_CASES)
 @pytest.mark.parametrize("prec", ["fp32", "f64"])
-def test_whole_tile_products_sampled(routine, prec):
+def test_whole_tile_products_sampled(routine, side, uplo, trans, prec):
     """4096-sized strmm / ssyrk / ssyr2k (and the fp64 forms): C is whole
-    128 x 128 tiles and k a multiple of the k-tile, so the triangular grids,
-    k-ranges and store masks run on the direct-to-LDS kernel.  4096 sampled
-    entries against fp64 dot products (bound (2k+2) u |alpha| |A||B| for
-    fp32, 4k u64 for fp64); the untouched triangle stays NaN."""
+    128 x 128 tiles and k a multiple of the k-tile, so the triangular grids
+    (lower and upper), the four k-ranges and the store masks run on the
+    direct-to-LDS kernel.  4096 sampled entries against fp64 dot products
+    (bound (2k+2) u |alpha| |A||B| for fp32, 4k u64 for fp64); the
+    untouched triangle of C stays NaN."""
     dt = torch.float64 if prec == "f64" else torch.float32
     g = torch.Generator(device="cuda").manual_seed(41)
     n, k = 4096, 256
@@ -280,27 +285,35 @@ def test_whole_tile_products_sampled(routine, prec):
         a = torch.randn(n * n, device="cuda", generator=g, dtype=dt)
         b0 = torch.randn(n * n, device="cuda", generator=g, dtype=dt)
         B = b0.clone()
-        assert mb.trmm("L", "L", "N", "N", n, n, alpha, a, n, B, n, **kw) == 0
+        assert mb.trmm(side, uplo, trans, "N", n, n, alpha, a, n, B, n, **kw) == 0
         torch.cuda.synchronize()
-        am = np.tril(a.cpu().numpy().reshape(n, n).T)
+        am = a.cpu().numpy().reshape(n, n).T
+        am = np.tril(am) if uplo == "L" else np.triu(am)
+        opa = am if trans == "N" else am.T
         bm = b0.cpu().numpy().reshape(n, n).T
         rows, cols = rng.integers(0, n, 4096), rng.integers(0, n, 4096)
-        ra, cb = am[rows].astype(np.float64), bm[:, cols].T.astype(np.float64)
+        if side == "L":  # B := alpha op(A) B0
+            ra, cb = opa[rows].astype(np.float64), bm[:, cols].T.astype(np.float64)
+        else:  # B := alpha B0 op(A)
+            ra, cb = bm[rows].astype(np.float64), opa[:, cols].T.astype(np.float64)
         got = B.cpu().numpy().reshape(n, n).T[rows, cols].astype(np.float64)
-        kk = n
     else:
         a = torch.randn(n * k, device="cuda", generator=g, dtype=dt)
         b = torch.randn(n * k, device="cuda", generator=g, dtype=dt)
         C = torch.full((n * n,), float("nan"), device="cuda", dtype=dt)
+        lda = n if trans == "N" else k
         if routine == "syrk":
-            assert mb.syrk("L", "N", n, k, alpha, a, n, 0.0, C, n, **kw) == 0
+            assert mb.syrk(uplo, trans, n, k, alpha, a, lda, 0.0, C, n, **kw) == 0
         else:
-            assert mb.syr2k("L", "N", n, k, alpha, a, n, b, n, 0.0, C, n, **kw) == 0
+            assert mb.syr2k(uplo, trans, n, k, alpha, a, lda, b, lda, 0.0, C, n, **kw) == 0
         torch.cuda.synchronize()
-        am = a.cpu().numpy().reshape(k, n).T.astype(np.float64)
-        bm = b.cpu().numpy().reshape(k, n).T.astype(np.float64)
+        # op rows: trans N -> rows of the n x k A; T -> columns of the k x n A
+        am = a.cpu().numpy().reshape(k, n).T if trans == "N" else a.cpu().numpy().reshape(n, k)
+        bm = b.cpu().numpy().reshape(k, n).T if trans == "N" else b.cpu().numpy().reshape(n, k)
+        am, bm = am.astype(np.float64), bm.astype(np.float64)
         i, j = rng.integers(0, n, 4096), rng.integers(0, n, 4096)
-        rows, cols = np.maximum(i, j), np.minimum(i, j)
+        lo, hi = np.minimum(i, j), np.maximum(i, j)
+        rows, cols = (hi, lo) if uplo == "L" else (lo, hi)
         if routine == "syrk":
             ra, cb = am[rows], am[cols]
         else:
@@ -308,9 +321,10 @@ def test_whole_tile_products_sampled(routine, prec):
             cb = np.concatenate([bm[cols], am[cols]], axis=1)
         c = C.cpu().numpy().reshape(n, n).T
         got = c[rows, cols].astype(np.float64)
-        strict = np.minimum(i, j) < np.maximum(i, j)
-        assert np.isnan(c[np.minimum(i, j)[strict], np.maximum(i, j)[strict]]).all()
-        kk = ra.shape[1]
+        strict = lo < hi
+        other_r, other_c = (lo, hi) if uplo == "L" else (hi, lo)
+        assert np.isnan(c[other_r[strict], other_c[strict]]).all()
+    kk = ra.shape[1]
     want = alpha * np.einsum("ij,ij->i", ra, cb)
     absab = abs(alpha) * np.einsum("ij,ij->i", np.abs(ra), np.abs(cb))
     u = 2.0 ** -53 if prec == "f64" else H.U32
