@@ -544,19 +544,6 @@ __device__ inline void packed_frag_mfma(const PackedFrag<NPL, TM, TN>& f, f32x16
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.bh[j], f.ah[i], acc[i][j], 0, 0, 0);
 }
 
-// Scheduling of one "read the next k-step while multiplying the current one"
-// region: NREAD ds_reads spread evenly over NMFMA MFMAs.
-template <int NREAD, int NMFMA>
-__device__ inline void packed_interleave() {
-    constexpr int PER = NMFMA / NREAD;
-#pragma unroll
-    for (int q = 0; q < NREAD; ++q) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);    // DS read
-        __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);  // MFMA
-    }
-    if (NMFMA > PER * NREAD) __builtin_amdgcn_sched_group_barrier(0x008, NMFMA - PER * NREAD, 0);
-}
-
 // BF16x3 / bf16 tile with the fragment reads software-pipelined: two register
 // sets hold consecutive k-steps, and the reads of step s+1 are issued between
 // the MFMAs of step s, so a wave never waits for LDS with an empty MFMA queue
