@@ -223,5 +223,7 @@ def test_buffer_entries_validate_without_gpu():
 def test_device_matrix_rejects_bad_shapes_without_gpu():
     with pytest.raises(TypeError):
         mb.DeviceMatrix(2, 2, np.int32)
+    with pytest.raises(TypeError):
+        mb.DeviceMatrix(2, 2, "float8")
     with pytest.raises(ValueError):
         mb.DeviceMatrix(4, 2, np.float32, ld=3)

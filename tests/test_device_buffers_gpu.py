@@ -87,3 +87,30 @@ def test_pinned_host_buffer_round_trip():
             assert np.array_equal(D.to_numpy().ravel(order="F"), host)
     finally:
         assert L.mbh_host_free(p) == 0
+
+
+@pytest.mark.parametrize("fmt", ["float16", "bfloat16"])
+def test_16bit_gemm_on_library_buffers(fmt):
+    """hgemm / bgemm with DeviceMatrix operands (padded ld) on a library
+    stream: integer data, so every partial sum is exact in fp32 and C must
+    equal the correctly rounded 16-bit result bit for bit."""
+    rng = np.random.default_rng(84)
+    m, n, k = 200, 136, 96
+    av = rng.integers(-4, 5, (m, k)).astype(np.float64)
+    bv = rng.integers(-4, 5, (k, n)).astype(np.float64)
+    cv = rng.integers(-4, 5, (m, n)).astype(np.float64)
+    if fmt == "float16":
+        enc = lambda x: np.asarray(x, np.float16)
+        bits = lambda x: np.asarray(x).view(np.uint16)
+        dt = np.float16
+    else:
+        enc = lambda x: H.round_bf16(np.asarray(x, np.float32))
+        bits = lambda x: np.asarray(x)
+        dt = "bfloat16"
+    want = bits(enc((-1.0 * (av @ bv) + 2.0 * cv).astype(np.float32)))
+    with mb.Stream() as s, mb.DeviceMatrix.from_numpy(enc(av), ld=m + 8, dtype=dt) as A, \
+            mb.DeviceMatrix.from_numpy(enc(bv), dtype=dt) as B, \
+            mb.DeviceMatrix.from_numpy(enc(cv), ld=m + 3, dtype=dt) as C:
+        assert mb.gemm("N", "N", m, n, k, -1.0, A, A.ld, B, B.ld, 2.0, C, C.ld, stream=s.handle) == 0
+        got = bits(C.to_numpy(stream=s.handle))
+    assert np.array_equal(got, want)
