@@ -42,17 +42,17 @@ hipError_t launch_fp(bool at, bool bt, GemmArgs<TA, TB, TC, T> p, int tri_grid, 
 }
 
 // Plain gemm (no triangle mask, full grid, every k-tile) on whole tiles: the
-// block of whole BM x BN tiles runs an interior kernel (unmasked loads only:
-// the direct-to-LDS gemm_dma_kernel when DMA, else the register-staged
-// tile) when both operands allow 16-byte loads and k is a multiple of the
-// interior kernel's k-tile; the rows below it (all n columns) and the
-// columns right of it go to the generic kernel as sub-problems, with EM x EN
-// tiles.  Offsetting op(A) by a multiple of BM rows and op(B) by a multiple
-// of BN columns keeps the 16-byte alignment the vector flags record.
-template <typename T, int BM, int BN, int BK, int EM, int EN, int EK, bool DMA>
+// block of whole 128 x 128 tiles runs the direct-to-LDS kernel (unmasked
+// loads only; with the masked path present, the register-staged tile's
+// k-loop was mostly branches around it) when both operands allow 16-byte
+// loads and k is a multiple of its k-tile; the rows below it (all n
+// columns) and the columns right of it go to the generic kernel as
+// sub-problems, with EM x EN tiles.  Offsetting op(A) by a multiple of 128
+// rows and op(B) by a multiple of 128 columns keeps the 16-byte alignment
+// the vector flags record.
+template <typename T, int EM, int EN, int EK>
 hipError_t launch_fp_split(bool at, bool bt, const GemmParams<T>& p, hipStream_t s) {
-    static_assert(!DMA || (BM == 128 && BN == 128), "the direct-to-LDS kernel has 128 x 128 tiles");
-    constexpr int IK = DMA ? DmaPanel<T, true>::BK : BK;
+    constexpr int BM = 128, BN = 128, IK = DmaPanel<T, true>::BK;
     const int64_t mi = p.m / BM * BM, ni = p.n / BN * BN;
     if (!p.a_vec || !p.b_vec || p.k % IK != 0 || mi == 0 || ni == 0)
         return launch_fp<T, T, T, T, EM, EN, EK>(at, bt, p, 0, s);
@@ -62,17 +62,10 @@ hipError_t launch_fp_split(bool at, bool bt, const GemmParams<T>& p, hipStream_t
     dim3 grid;
     if (!grid_for<GemmParams<T>, BM, BN>(q, 0, grid)) return hipErrorInvalidValue;
     const dim3 block(256);
-    if constexpr (DMA) {
-        if (!at && !bt) gemm_dma_kernel<T, false, false><<<grid, block, 0, s>>>(q);
-        else if (!at && bt) gemm_dma_kernel<T, false, true><<<grid, block, 0, s>>>(q);
-        else if (at && !bt) gemm_dma_kernel<T, true, false><<<grid, block, 0, s>>>(q);
-        else gemm_dma_kernel<T, true, true><<<grid, block, 0, s>>>(q);
-    } else {
-        if (!at && !bt) gemm_fp_interior_kernel<T, false, false, BM, BN, BK><<<grid, block, 0, s>>>(q);
-        else if (!at && bt) gemm_fp_interior_kernel<T, false, true, BM, BN, BK><<<grid, block, 0, s>>>(q);
-        else if (at && !bt) gemm_fp_interior_kernel<T, true, false, BM, BN, BK><<<grid, block, 0, s>>>(q);
-        else gemm_fp_interior_kernel<T, true, true, BM, BN, BK><<<grid, block, 0, s>>>(q);
-    }
+    if (!at && !bt) gemm_dma_kernel<T, false, false><<<grid, block, 0, s>>>(q);
+    else if (!at && bt) gemm_dma_kernel<T, false, true><<<grid, block, 0, s>>>(q);
+    else if (at && !bt) gemm_dma_kernel<T, true, false><<<grid, block, 0, s>>>(q);
+    else gemm_dma_kernel<T, true, true><<<grid, block, 0, s>>>(q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mi < p.m) {  // rows [mi, m) x all columns
@@ -383,7 +376,7 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
                 // Plain products: interior kernel plus edge strips (8192^3:
                 // 126.8 -> 131.7 TFLOP/s, bit-identical).
                 if (!tri && !tri_grid && !krange)
-                    return launch_fp_split<float, 128, 128, 16, 64, 64, 16, true>(at, bt, p, s);
+                    return launch_fp_split<float, 64, 64, 16>(at, bt, p, s);
                 if (dma_whole_ok(p)) return launch_dma_whole<float>(at, bt, p, tri_grid, s);
                 return launch_fp<float, float, float, float, 128, 128, 16>(at, bt, p, tri_grid, s);
             }
@@ -410,7 +403,7 @@ hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha
     // generic kernel measured 63.5 with 128 x 128 tiles, 60.6 with 64 x 64
     // and 62.1 with 128 x 64).
     if (!tri && !tri_grid && !krange && tile_count(m, n, 128, 128, 0) >= 256)
-        return launch_fp_split<double, 128, 128, 16, 64, 64, 16, true>(at, bt, p, s);
+        return launch_fp_split<double, 64, 64, 16>(at, bt, p, s);
     if (tile_count(m, n, 128, 128, tri_grid) >= (krange ? 1024 : 256) && dma_whole_ok(p))
         return launch_dma_whole<double>(at, bt, p, tri_grid, s);
     return launch_fp<double, double, double, double, 64, 64, 16>(at, bt, p, tri_grid, s);

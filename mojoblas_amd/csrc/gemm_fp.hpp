@@ -75,12 +75,9 @@ struct PanelLoader {
     typedef TIN vin_t __attribute__((ext_vector_type(VEC)));
     TIN r[NV][VEC];
 
-    // ALWAYS_FULL: the caller guarantees an interior panel (aligned, all
-    // rows and k in range), so only the unmasked vector loads are compiled.
-    template <bool ALWAYS_FULL = false>
     __device__ inline void load(const TIN* __restrict__ src, int64_t ld, int64_t row0, int64_t nrows, int64_t k0,
                                 int64_t ktot, bool vec_ok, int tid) {
-        const bool full = ALWAYS_FULL || (vec_ok && (row0 + ROWS <= nrows) && (k0 + BK <= ktot));
+        const bool full = vec_ok && (row0 + ROWS <= nrows) && (k0 + BK <= ktot);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const int idx = tid + NT * v;
@@ -163,12 +160,7 @@ struct FpTileLds {
 // One BM x BN tile (tm, tn) of C := alpha*op(A)*op(B) + beta*C on 256
 // threads (2 x 2 waves); `lds` holds FpTileLds::BYTES, 16-byte aligned.
 // Shared by the grid kernel below and the persistent batched one.
-// INTERIOR: every panel of the tile is interior (16-byte aligned operands,
-// the tile inside m x n, k a multiple of BK), so the k-loop holds only
-// unmasked loads: with the masked path present the compiler's k-loop was
-// mostly branches around it.
-template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK,
-          bool INTERIOR = false>
+template <typename T, typename TA, typename TB, typename TC, bool AT, bool BT, int BM, int BN, int BK>
 __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, int tn, void* lds) {
     typedef MfmaFp<T> M;
     typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
@@ -200,8 +192,8 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
     int kt0, kt1;
     kt_range(p.krange, tm, tn, BM, BN, BK, (int)((p.k + BK - 1) / BK), kt0, kt1);
     if (kt0 < kt1) {
-        la.template load<INTERIOR>(p.A, p.lda, m0, p.m, (int64_t)kt0 * BK, p.k, p.a_vec, tid);
-        lb.template load<INTERIOR>(p.B, p.ldb, n0, p.n, (int64_t)kt0 * BK, p.k, p.b_vec, tid);
+        la.load(p.A, p.lda, m0, p.m, (int64_t)kt0 * BK, p.k, p.a_vec, tid);
+        lb.load(p.B, p.ldb, n0, p.n, (int64_t)kt0 * BK, p.k, p.b_vec, tid);
         la.template store<SA>(As[0], tid);
         lb.template store<SB>(Bs[0], tid);
     }
@@ -212,8 +204,8 @@ __device__ inline void gemm_fp_tile(const GemmArgs<TA, TB, TC, T>& p, int tm, in
     for (int kt = kt0; kt < kt1; ++kt) {
         const int cur = (kt - kt0) & 1;
         if (kt + 1 < kt1) {
-            la.template load<INTERIOR>(p.A, p.lda, m0, p.m, (int64_t)(kt + 1) * BK, p.k, p.a_vec, tid);
-            lb.template load<INTERIOR>(p.B, p.ldb, n0, p.n, (int64_t)(kt + 1) * BK, p.k, p.b_vec, tid);
+            la.load(p.A, p.lda, m0, p.m, (int64_t)(kt + 1) * BK, p.k, p.a_vec, tid);
+            lb.load(p.B, p.ldb, n0, p.n, (int64_t)(kt + 1) * BK, p.k, p.b_vec, tid);
         }
         const MBH_LDS TA* Ab = As[cur] + lgrp * SA + wm0 + lrow;
         const MBH_LDS TB* Bb = Bs[cur] + lgrp * SB + wn0 + lrow;
@@ -377,16 +369,6 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p)
     gemm_fp_tile<T, TA, TB, TC, AT, BT, BM, BN, BK>(p, tm, tn, lds);
 }
 
-// The grid kernel over interior tiles only (see gemm_fp_tile's INTERIOR);
-// the launcher sends the edge strips of C to gemm_fp_kernel.
-template <typename T, bool AT, bool BT, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void gemm_fp_interior_kernel(GemmArgs<T, T, T, T> p) {
-    typedef FpTileLds<T, T, T, AT, BT, BM, BN, BK> L;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
-    int tm, tn;
-    tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
-    gemm_fp_tile<T, T, T, T, AT, BT, BM, BN, BK, true>(p, tm, tn, lds);
-}
 
 // Interior tile with direct-to-LDS panel loads (no register staging), fp64
 // or fp32, for a C of whole tiles (or the block of whole tiles of a plain
