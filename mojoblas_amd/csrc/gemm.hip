@@ -244,8 +244,12 @@ hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_g
 // Native 16-bit path.  op(A) (rows m) is k-contiguous in memory iff A is
 // transposed, op(B)^T (rows n) iff B is not; such an operand with K a
 // multiple of 64, a 16-byte aligned base and ld % 8 == 0 is read in place by
-// the main kernel.  Any other operand is first re-laid into the packed
-// single plane (bits unchanged; one launch packs whichever operands need it).
+// the main kernel.  So is an operand whose rows are the contiguous index (A
+// not transposed, B transposed) under the same conditions with its row count
+// a multiple of 8: it is loaded as k-rows (MN form) and its fragments are
+// taken with transposing LDS reads.  Any other operand is first re-laid into
+// the packed single plane (bits unchanged; one launch packs whichever
+// operands need it).
 template <bool F16, int BT, int WM_, int WN_>
 hipError_t run_packed16(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* A,
                         int64_t lda, const uint16_t* B, int64_t ldb, float beta, uint16_t* C, int64_t ldc,
@@ -255,8 +259,10 @@ hipError_t run_packed16(bool at, bool bt, int64_t m, int64_t n, int64_t k, float
     const int64_t ktiles = (k + PK - 1) / PK;
     const int64_t ntile = tm * tn;
     if (ntile <= 0 || ntile > INT_MAX || ktiles > INT_MAX) return hipErrorInvalidValue;
-    const bool direct_a = at && k % PK == 0 && vec_ok(A, lda);
-    const bool direct_b = !bt && k % PK == 0 && vec_ok(B, ldb);
+    const bool mn_a = !at && k % PK == 0 && m % 8 == 0 && vec_ok(A, lda);
+    const bool mn_b = bt && k % PK == 0 && n % 8 == 0 && vec_ok(B, ldb);
+    const bool direct_a = mn_a || (at && k % PK == 0 && vec_ok(A, lda));
+    const bool direct_b = mn_b || (!bt && k % PK == 0 && vec_ok(B, ldb));
     const size_t a_el = direct_a ? 0 : (size_t)tm * BT * ktiles * PK;
     const size_t b_el = direct_b ? 0 : (size_t)tn * BT * ktiles * PK;
     uint16_t* base = nullptr;
@@ -300,7 +306,11 @@ hipError_t run_packed16(bool at, bool bt, int64_t m, int64_t n, int64_t k, float
     p.tiles_m = (int)tm;
     p.tiles_n = (int)tn;
     timing_mark(tmk, 1, s);
-    gemm_packed16_kernel<F16, BT, BT, WM_, WN_><<<dim3((unsigned)ntile), dim3(64 * WM_ * WN_), 0, s>>>(p);
+    const dim3 grid((unsigned)ntile), block(64 * WM_ * WN_);
+    if (mn_a && mn_b) gemm_packed16_kernel<F16, BT, BT, WM_, WN_, true, true><<<grid, block, 0, s>>>(p);
+    else if (mn_a) gemm_packed16_kernel<F16, BT, BT, WM_, WN_, true, false><<<grid, block, 0, s>>>(p);
+    else if (mn_b) gemm_packed16_kernel<F16, BT, BT, WM_, WN_, false, true><<<grid, block, 0, s>>>(p);
+    else gemm_packed16_kernel<F16, BT, BT, WM_, WN_><<<grid, block, 0, s>>>(p);
     timing_mark(tmk, 2, s);
     return hipGetLastError();
 }

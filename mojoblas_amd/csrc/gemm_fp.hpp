@@ -419,6 +419,20 @@ struct DmaPanel {
 // region: NREAD ds_reads spread evenly over NMFMA MFMAs.
 template <int NREAD, int NMFMA>
 __device__ inline void packed_interleave() {
+    if constexpr (NREAD > NMFMA) {  // more reads than MFMAs: PER or PER + 1 before each
+        constexpr int PER = NREAD / NMFMA, EXTRA = NREAD % NMFMA;
+#pragma unroll
+        for (int q = 0; q < EXTRA; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x100, PER + 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        }
+#pragma unroll
+        for (int q = EXTRA; q < NMFMA; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x100, PER, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        }
+        return;
+    }
     constexpr int PER = NMFMA / NREAD;
 #pragma unroll
     for (int q = 0; q < NREAD; ++q) {

@@ -55,8 +55,10 @@ struct PackedGemmParams {
     const int* sB;
     int kblocks;
     // Native 16-bit gemm only: a_ld / b_ld != 0 means Ah / Bh is the caller's
-    // k-contiguous operand itself (row r at Ah + r*a_ld, K a multiple of 32),
-    // read in place instead of from a packed plane.
+    // operand itself, read in place instead of from a packed plane (K a
+    // multiple of 64): k-contiguous (row r at Ah + r*a_ld), or, in the
+    // kernels built for an MN-form operand, row-contiguous (k-row k at
+    // Ah + k*a_ld, row count a multiple of 8).
     int64_t a_ld, b_ld;
 };
 
@@ -492,21 +494,70 @@ struct PackedFrag {
 };
 
 // LDS -> registers: chunk `ch` (swizzled) of the wave's rows in each plane
-// (CPR chunks per LDS row).
-template <int NPL, int TM, int TN, int CPR = 4>
+// (CPR chunks per LDS row).  RA / RB false: that operand is read elsewhere.
+template <int NPL, int TM, int TN, int CPR = 4, bool RA = true, bool RB = true>
 __device__ inline void packed_frag_read(PackedFrag<NPL, TM, TN>& f, const MBH_LDS u32x4* buf, int oa1, int ob0,
                                         int ob1, int wm0, int wn0, int lrow, int ch) {
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
+    for (int i = 0; i < TM && RA; ++i) {
         const int idx = (wm0 + i * 32 + lrow) * CPR + ch;
         f.ah[i] = __builtin_bit_cast(bf16x8, (u32x4)buf[idx]);
         if (NPL == 2) f.al[i] = __builtin_bit_cast(bf16x8, (u32x4)buf[oa1 + idx]);
     }
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
+    for (int j = 0; j < TN && RB; ++j) {
         const int idx = (wn0 + j * 32 + lrow) * CPR + ch;
         f.bh[j] = __builtin_bit_cast(bf16x8, (u32x4)buf[ob0 + idx]);
         if (NPL == 2) f.bl[j] = __builtin_bit_cast(bf16x8, (u32x4)buf[ob1 + idx]);
+    }
+}
+
+// MN form of a 16-bit operand's 64-deep k-tile in LDS, for an operand whose
+// rows (m of op(A), n of op(B)) are the contiguous index in HBM (A not
+// transposed, B transposed), so that it can be read in place: LDS holds
+// [k][BR rows] (2*BR bytes per k-row).  A wave-load writes 1 KiB, RPW whole
+// k-rows; wave-load g starts at byte g*GS (GS = 1 KiB + 64).  The k-rows are
+// permuted over the wave-loads: k = 4q + r sits in wave-load
+// g = 4*(q / RPW) + r as its (q % RPW)-th k-row, so the four consecutive k
+// of one transposing read lie in wave-loads g = 0..3 mod 4, at bank offsets
+// 64 bytes apart.
+template <int BR>
+struct MnPanel {
+    static constexpr int RPW = 512 / BR;  // k-rows per wave-load
+    static constexpr int LPK = BR / 8;    // lanes (16-byte chunks) per k-row
+    static constexpr int NG = 64 / RPW;   // wave-loads per k-tile
+    static constexpr int GS = 1088;       // bytes per wave-load slot
+    static constexpr int CH = NG * GS / 16;  // 16-byte chunks per buffer
+    static_assert(BR == 64 || BR == 128 || BR == 256, "row block");
+    // k-row held by k-row slot `sub` of wave-load g.
+    __device__ static int k_of(int g, int sub) { return 4 * ((g >> 2) * RPW + sub) + (g & 3); }
+    // Byte offset, for one lane, of the transposing read that hands it rows
+    // w0 + (lane & 31) at k = 16*s + 8*(lane >> 5) + 4*h + 0..3: in each
+    // group of 16 lanes, ds_read_b64_tr_b16 gives lane 4a + j element j of
+    // the 8-byte reads of lanes a, a + 4, a + 8, a + 12, so lane t of group
+    // G reads rows 16*(G & 1) + 4*(t & 3) .. + 3 at k = 8*(G >> 1) + (t >> 2)
+    // (plus 16 s + 4 h).  This is the lane part; step and half add
+    // step_off(s) + h * 2 * BR.
+    __device__ static int lane_off(int lane, int w0) {
+        const int G = lane >> 4, t = lane & 15, u = 2 * (G >> 1);
+        return (4 * (u / RPW) + (t >> 2)) * GS + (u % RPW) * 2 * BR + (w0 + 16 * (G & 1) + 4 * (t & 3)) * 2;
+    }
+    static constexpr int step_off(int s) { return 4 * ((4 * s) / RPW) * GS + ((4 * s) % RPW) * 2 * BR; }
+};
+
+// Fragments of k-step S of T 32-row blocks from an MN-form panel at byte
+// `base` (lane part `loff` from MnPanel::lane_off): two transposing reads
+// per fragment (k 0..3 and 4..7 of the lane's half of the step).
+template <int BR, int T, int S>
+__device__ inline void mn_frag_read(bf16x8 (&x)[T], const MBH_LDS u32x4* buf, int base, int loff) {
+    typedef short v4i16 __attribute__((ext_vector_type(4)));
+    typedef MnPanel<BR> P;
+    const MBH_LDS char* b = (const MBH_LDS char*)buf + base + loff + P::step_off(S);
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+        const v4i16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((MBH_LDS v4i16*)(b + i * 64));
+        const v4i16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((MBH_LDS v4i16*)(b + i * 64 + 2 * BR));
+        x[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
     }
 }
 
@@ -560,14 +611,17 @@ __device__ inline void packed_frag_mfma(const PackedFrag<NPL, TM, TN>& f, f32x16
 // p.a_ld / p.b_ld is set, the caller's k-contiguous matrix read in place
 // (lane chunk (row, c) at X + row*ld + 8c, stride BKT/8 chunks; rows past the
 // operand's last are clamped to it: they only reach C entries that are never
-// stored).  BKT: k per LDS row, 32 (64-byte rows, two 16-deep MFMA steps per
+// stored).  AMN / BMN: that operand is the caller's row-contiguous matrix,
+// loaded whole k-rows at a time into an MnPanel (lane chunk at
+// X + k*ld + row0 + 8c, stride BKT*ld/8 chunks, chunks past the last row
+// clamped to the last 8 rows) and read with mn_frag_read.  BKT: k per LDS row, 32 (64-byte rows, two 16-deep MFMA steps per
 // k-tile) or, in the 16-bit kernel, 64 (128-byte rows, so an in-place row
 // read is one whole 128-byte line, and four steps per barrier); 64-deep rows
 // are swizzled over 8 chunks by swz8(row) = (row >> 1) & 7, which spreads
 // the 32-row fragment reads (even and odd rows on separate halves of the
 // banks) over distinct 16-byte bank slots.
 template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N, int CK = 0, bool F16 = false, bool STRIDED = false,
-          int BKT = 32>
+          int BKT = 32, bool AMN = false, bool BMN = false>
 __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, int tn, u32x4* lds0, u32x4* lds1) {
     static_assert(NPROD == 1 || NPROD == 3, "bf16 modes only");
     static_assert(!F16 || NPROD == 1, "f16 operands use one plane");
@@ -575,6 +629,7 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     static_assert(BKT == 32 || (BKT == 64 && STRIDED), "64-deep k-tiles: 16-bit kernel only");
     constexpr int NW = WAVES_M * WAVES_N;
     static_assert(BKT == 32 || NW % 2 == 0, "swz8 of a wave-load's rows assumes an even wave count");
+    static_assert(!(AMN || BMN) || (STRIDED && BKT == 64), "MN-form panels: 16-bit kernel only");
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
     constexpr int TM = WM / 32, TN = WN / 32;
     constexpr int NPL = (NPROD == 3) ? 2 : 1;
@@ -585,7 +640,8 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     constexpr int A_WL = A_CH / (64 * NW), B_WL = B_CH / (64 * NW);
     static_assert(A_WL * 64 * NW == A_CH && B_WL * 64 * NW == B_CH && A_WL >= 1 && B_WL >= 1,
                   "tile not divisible into wave-loads");
-    constexpr int NREAD = NPL * (TM + TN), NMFMA = NPROD * TM * TN;
+    // An MN-form operand takes two (transposing) reads per fragment.
+    constexpr int NREAD = NPL * ((AMN ? 2 : 1) * TM + (BMN ? 2 : 1) * TN), NMFMA = NPROD * TM * TN;
     typedef const MBH_GLOBAL u32x4* gptr;
 
     const int tid = threadIdx.x, lane = tid & 63;
@@ -610,7 +666,14 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     }
     MBH_LDS u32x4* const L0 = (MBH_LDS u32x4*)lds0;
     MBH_LDS u32x4* const L1 = (MBH_LDS u32x4*)lds1;
-    constexpr int OA1 = A_CH, OB0 = NPL * A_CH, OB1 = NPL * A_CH + B_CH;
+    // MN-form operands: same wave-load count (BR/8 per k-tile), wave-load
+    // slots of MnPanel::GS bytes.
+    typedef MnPanel<AMN ? BM : 256> PA;
+    typedef MnPanel<BMN ? BN : 256> PB;
+    static_assert(!AMN || PA::NG == A_WL * NW, "MN panel wave-loads (A)");
+    static_assert(!BMN || PB::NG == B_WL * NW, "MN panel wave-loads (B)");
+    constexpr int AWS = AMN ? PA::GS / 16 : 64, BWS = BMN ? PB::GS / 16 : 64;  // chunks per wave-load slot
+    constexpr int OA1 = A_CH, OB0 = AMN ? PA::CH : NPL * A_CH, OB1 = NPL * A_CH + B_CH;
     // STRIDED: per wave-load pointers and k-tile strides (in 16-byte chunks).
     gptr pa[A_WL], pb[B_WL];
     size_t sa = A_CH, sb = B_CH;
@@ -619,7 +682,12 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
 #pragma unroll
         for (int v = 0; v < A_WL; ++v) {
             const int blk = (wave + v * NW) * 64;
-            if (p.a_ld) {
+            if constexpr (AMN) {  // k-row k_of(g, sub), rows from 8*(lane % LPK); chunks past m clamped
+                const int g = wave + v * NW, sub = lane / PA::LPK;
+                int64_t row = (int64_t)tm * BM + 8 * (lane % PA::LPK);
+                row = row < p.m ? row : p.m - 8;
+                pa[v] = (gptr)(p.Ah + (int64_t)PA::k_of(g, sub) * p.a_ld + row);
+            } else if (p.a_ld) {
                 int64_t row = (int64_t)tm * BM + (wave + v * NW) * RW + lroww;
                 row = row < p.m ? row : p.m - 1;
                 pa[v] = (gptr)(p.Ah + row * p.a_ld) + c;
@@ -630,7 +698,12 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
 #pragma unroll
         for (int v = 0; v < B_WL; ++v) {
             const int blk = (wave + v * NW) * 64;
-            if (p.b_ld) {
+            if constexpr (BMN) {
+                const int g = wave + v * NW, sub = lane / PB::LPK;
+                int64_t row = (int64_t)tn * BN + 8 * (lane % PB::LPK);
+                row = row < p.n ? row : p.n - 8;
+                pb[v] = (gptr)(p.Bh + (int64_t)PB::k_of(g, sub) * p.b_ld + row);
+            } else if (p.b_ld) {
                 int64_t row = (int64_t)tn * BN + (wave + v * NW) * RW + lroww;
                 row = row < p.n ? row : p.n - 1;
                 pb[v] = (gptr)(p.Bh + row * p.b_ld) + c;
@@ -638,8 +711,8 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
                 pb[v] = gB0 + blk;
             }
         }
-        if (p.a_ld) sa = CPR;
-        if (p.b_ld) sb = CPR;
+        if (p.a_ld) sa = AMN ? (size_t)p.a_ld * BKT / 8 : CPR;
+        if (p.b_ld) sb = BMN ? (size_t)p.b_ld * BKT / 8 : CPR;
     }
 
 #define MBH_DMA(kt, LB)                                                                                 \
@@ -647,10 +720,10 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
         if constexpr (STRIDED) {                                                                        \
             _Pragma("unroll") for (int v = 0; v < A_WL; ++v)                                            \
                 __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(pa[v] + (size_t)(kt) * sa),   \
-                                                 (MBH_LDS void*)((LB) + (wave + v * NW) * 64), 16, 0, 0); \
+                                                 (MBH_LDS void*)((LB) + (wave + v * NW) * AWS), 16, 0, 0); \
             _Pragma("unroll") for (int v = 0; v < B_WL; ++v)                                            \
                 __builtin_amdgcn_global_load_lds((const MBH_GLOBAL void*)(pb[v] + (size_t)(kt) * sb),   \
-                                                 (MBH_LDS void*)((LB) + OB0 + (wave + v * NW) * 64), 16, 0, 0); \
+                                                 (MBH_LDS void*)((LB) + OB0 + (wave + v * NW) * BWS), 16, 0, 0); \
         } else {                                                                                        \
             const size_t ga = (size_t)(kt) * A_CH, gb = (size_t)(kt) * B_CH;                            \
             _Pragma("unroll") for (int v = 0; v < A_WL; ++v) {                                          \
@@ -672,9 +745,11 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
         }                                                                                               \
         __builtin_amdgcn_sched_barrier(0);                                                              \
     } while (0)
-#define MBH_STEP(FN, BUF, CH, FC)                                                                       \
+#define MBH_STEP(FN, BUF, CH, FC, S)                                                                    \
     do {                                                                                                \
-        packed_frag_read<NPL, TM, TN, CPR>(FN, BUF, OA1, OB0, OB1, wm0, wn0, lrow, CH);                 \
+        packed_frag_read<NPL, TM, TN, CPR, !AMN, !BMN>(FN, BUF, OA1, OB0, OB1, wm0, wn0, lrow, CH);     \
+        if constexpr (AMN) mn_frag_read<BM, TM, S>(FN.ah, BUF, 0, a_loff);                              \
+        if constexpr (BMN) mn_frag_read<BN, TN, S>(FN.bh, BUF, OB0 * 16, b_loff);                       \
         packed_frag_mfma<NPROD, NPL, TM, TN, F16>(FC, acc);                                             \
         packed_interleave<NREAD, NMFMA>();                                                              \
         __builtin_amdgcn_sched_barrier(0);                                                              \
@@ -698,12 +773,15 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     const int lrow = lane & 31, lgrp = lane >> 5, lsw = swz(lrow);
     // Chunks of the k-steps (16 k = two chunks; lanes 32-63 take the second).
     const int ch0 = lgrp ^ lsw, ch1 = (2 + lgrp) ^ lsw, ch2 = (4 + lgrp) ^ lsw, ch3 = (6 + lgrp) ^ lsw;
+    const int a_loff = AMN ? PA::lane_off(lane, wm0) : 0, b_loff = BMN ? PB::lane_off(lane, wn0) : 0;
     PackedFrag<NPL, TM, TN> f0, f1;
     if (kt0 < kt1) {
         MBH_DMA(kt0, L0);
         MBH_SYNC();
         if (kt0 + 1 < kt1) MBH_DMA(kt0 + 1, L1);
-        packed_frag_read<NPL, TM, TN, CPR>(f0, L0, OA1, OB0, OB1, wm0, wn0, lrow, ch0);
+        packed_frag_read<NPL, TM, TN, CPR, !AMN, !BMN>(f0, L0, OA1, OB0, OB1, wm0, wn0, lrow, ch0);
+        if constexpr (AMN) mn_frag_read<BM, TM, 0>(f0.ah, L0, 0, a_loff);
+        if constexpr (BMN) mn_frag_read<BN, TN, 0>(f0.bh, L0, OB0 * 16, b_loff);
     }
     // Straight-line body, one exit: the accumulators stay in place.  The last
     // step's read of the next k-tile may read a stale buffer (never used).
@@ -712,29 +790,29 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
     int kt = kt0;
     for (; kt + 1 < kt1; kt += 2) {
         // L0: k-tile kt (f0 = its step 0); L1: k-tile kt+1 in flight
-        MBH_STEP(f1, L0, ch1, f0);
+        MBH_STEP(f1, L0, ch1, f0, 1);
         if constexpr (KS == 4) {
-            MBH_STEP(f0, L0, ch2, f1);
-            MBH_STEP(f1, L0, ch3, f0);
+            MBH_STEP(f0, L0, ch2, f1, 2);
+            MBH_STEP(f1, L0, ch3, f0, 3);
         }
         MBH_SYNC();
         if (kt + 2 < kt1) MBH_DMA(kt + 2, L0);
-        MBH_STEP(f0, L1, ch0, f1);
+        MBH_STEP(f0, L1, ch0, f1, 0);
         // L1: k-tile kt+1 (f0 = its step 0); L0: k-tile kt+2 in flight
-        MBH_STEP(f1, L1, ch1, f0);
+        MBH_STEP(f1, L1, ch1, f0, 1);
         if constexpr (KS == 4) {
-            MBH_STEP(f0, L1, ch2, f1);
-            MBH_STEP(f1, L1, ch3, f0);
+            MBH_STEP(f0, L1, ch2, f1, 2);
+            MBH_STEP(f1, L1, ch3, f0, 3);
         }
         MBH_SYNC();
         if (kt + 3 < kt1) MBH_DMA(kt + 3, L1);
-        MBH_STEP(f0, L0, ch0, f1);
+        MBH_STEP(f0, L0, ch0, f1, 0);
     }
     if (kt < kt1) {  // odd count: k-tile kt in L0, f0 = its step 0
-        MBH_STEP(f1, L0, ch1, f0);
+        MBH_STEP(f1, L0, ch1, f0, 1);
         if constexpr (KS == 4) {
-            MBH_STEP(f0, L0, ch2, f1);
-            MBH_STEP(f1, L0, ch3, f0);
+            MBH_STEP(f0, L0, ch2, f1, 2);
+            MBH_STEP(f1, L0, ch3, f0, 3);
         }
         packed_frag_mfma<NPROD, NPL, TM, TN, F16>(f1, acc);
     }
@@ -765,18 +843,20 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
 constexpr int GEMM16_BK = 64;
 
 // Native 16-bit gemm: one plane per operand holding f16 (F16) or bf16 values
-// (packed, or the caller's k-contiguous operand read in place), products on
+// (packed, or the caller's operand read in place: k-contiguous, or
+// row-contiguous in the AMN / BMN instantiations), products on
 // the MFMA of that format with fp32 accumulation, C stored in the same 16-bit
 // format.
-template <bool F16, int BM, int BN, int WAVES_M, int WAVES_N>
+template <bool F16, int BM, int BN, int WAVES_M, int WAVES_N, bool AMN = false, bool BMN = false>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed16_kernel(PackedGemmParams p) {
-    typedef PackedTileLds<1, BM, BN, GEMM16_BK / 8> L;
-    __shared__ u32x4 lds0[L::TOTAL / 2];
-    __shared__ u32x4 lds1[L::TOTAL / 2];
+    constexpr int CH = (AMN ? MnPanel<BM>::CH : BM * GEMM16_BK / 8) + (BMN ? MnPanel<BN>::CH : BN * GEMM16_BK / 8);
+    __shared__ u32x4 lds0[CH];
+    __shared__ u32x4 lds1[CH];
     int tm, tn;
     if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
     else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
-    gemm_packed_tile_pipe<1, BM, BN, WAVES_M, WAVES_N, F16 ? 1 : 2, F16, true, GEMM16_BK>(p, tm, tn, lds0, lds1);
+    gemm_packed_tile_pipe<1, BM, BN, WAVES_M, WAVES_N, F16 ? 1 : 2, F16, true, GEMM16_BK, AMN, BMN>(p, tm, tn, lds0,
+                                                                                                 lds1);
 }
 
 }  // namespace mbh

@@ -135,10 +135,14 @@ def test_integer_data_bit_exact(ta, tb, fmt):
 def test_in_place_operands_bit_exact(ta, tb, fmt):
     # k % 64 == 0 and ld % 8 == 0: a k-contiguous operand (op(A) when A is
     # transposed, op(B)^T when B is not) is read in place, with its rows past
-    # the last clamped; the other one is packed.  A base one element off the
-    # 16-byte alignment, or k = 96, falls back to packing.
+    # the last clamped.  The other one is read in place too (k-rows in LDS,
+    # transposing fragment reads) when its row count is a multiple of 8
+    # (8, 136, 72, 264, 520, 1000, 1016, 64), else packed (1, 129, 65, 300,
+    # 257).  A base one element off the 16-byte alignment, or k = 96, falls
+    # back to packing.
     rng = np.random.default_rng(26)
-    for (m, n, k) in [(1, 1, 64), (129, 65, 64), (300, 257, 128), (64, 300, 256), (65, 33, 96)]:
+    for (m, n, k) in [(1, 1, 64), (129, 65, 64), (300, 257, 128), (64, 300, 256), (65, 33, 96), (8, 8, 64),
+                      (136, 72, 64), (264, 520, 128), (1000, 1016, 192)]:
         for shift in (0, 1):
             av, bv, cv, lda, ldb, ldc, a, b, c = int_case(rng, fmt, ta, tb, m, n, k, pad=8)
             a = np.concatenate([np.zeros(shift, np.uint16), a])
@@ -156,12 +160,16 @@ def test_in_place_operands_bit_exact(ta, tb, fmt):
 @pytest.mark.parametrize("fmt", FMTS)
 @pytest.mark.parametrize("shape,ta,tb", [((2048, 1024, 128), "N", "N"), ((2048, 1024, 128), "T", "T"),
                                          ((2048, 1024, 96), "T", "N"), ((4096, 4096, 64), "N", "N"),
-                                         ((4096, 4096, 192), "T", "N"), ((4096, 4096, 72), "T", "N")])
+                                         ((4096, 4096, 192), "T", "N"), ((4096, 4096, 72), "T", "N"),
+                                         ((2048, 1024, 64), "N", "T"), ((4104, 4088, 128), "N", "T"),
+                                         ((4104, 4088, 128), "T", "T"), ((2056, 1032, 192), "N", "N")])
 def test_large_tiles_bit_exact(shape, ta, tb, fmt):
-    # 2048 x 1024 takes the 128 x 128 tiles, 4096^2 the 256 x 256 ones; with
-    # k a multiple of 64 (ld padded by 8) the k-contiguous operands are read
-    # in place, with k = 96 or 72 both are packed (a k-tail inside the last
-    # 64-deep k-tile).
+    # 2048 x 1024 (and 2056 x 1032) take the 128 x 128 tiles, 4096^2 (and
+    # 4104 x 4088) the 256 x 256 ones; with k a multiple of 64 (ld padded by
+    # 8) every operand is read in place (those whose rows are contiguous
+    # through transposing LDS reads; 4104, 4088, 2056, 1032: a last tile
+    # with 8 valid rows, chunks past them clamped), with k = 96 or 72 both
+    # are packed (a k-tail inside the last 64-deep k-tile).
     m, n, k = shape
     rng = np.random.default_rng(22)
     av, bv, cv, lda, ldb, ldc, a, b, c = int_case(rng, fmt, ta, tb, m, n, k, pad=8)
