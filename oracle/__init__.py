@@ -7,6 +7,7 @@ never by the product package.
 Arrays are numpy, column-major (Fortran order) 2-D views or flat buffers with
 explicit leading dimensions, exactly like the BLAS signatures.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -31,41 +32,65 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def _bind(L):
+    """Declare the C signatures on a loaded build of this library."""
+    L.orc_dgemm.argtypes = [_cs, _cs, _i64, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
+    L.orc_dtrsm.argtypes = [_cs, _cs, _cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64]
+    L.orc_dtrmm.argtypes = [_cs, _cs, _cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64]
+    L.orc_dsyrk.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _dbl, _pd, _i64]
+    L.orc_dsymm.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
+    L.orc_dsyr2k.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
+    L.orc_dgemv.argtypes = [_cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
+    L.orc_dtrsv.argtypes = [_cs, _cs, _cs, _i64, _pd, _i64, _pd, _i64]
+    for f in (L.orc_dgemm, L.orc_dtrsm, L.orc_dtrmm, L.orc_dsyrk, L.orc_dsymm, L.orc_dsyr2k):
+        f.restype = ctypes.c_int
+    L.orc_sgemm_sample_f64.argtypes = [
+        ctypes.c_char, ctypes.c_char, _i64, _dbl, _pf, _i64, _pf, _i64, _dbl, _pf, _i64, _pl, _pl, _pd]
+    L.orc_sgemm_sample_f64.restype = None
+    L.orc_sgemm_sample_abs.argtypes = [
+        ctypes.c_char, ctypes.c_char, _i64, _pf, _i64, _pf, _i64, _i64, _pl, _pl, _pd]
+    L.orc_sgemm_sample_abs.restype = None
+    L.orc_fill_f32.argtypes = [_pf, _i64, ctypes.c_uint64, ctypes.c_int, _dbl, _dbl]
+    L.orc_fill_f32.restype = None
+    L.base_threads.restype = ctypes.c_int
+    L.base_sgemm.argtypes = [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, _flt, _pf, _i64, _pf, _i64,
+                             _flt, _pf, _i64]
+    L.base_sgemm.restype = None
+    L.base_strsm_left.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, _i64, _i64, _flt, _pf, _i64,
+                                  _pf, _i64]
+    L.base_strsm_left.restype = None
+    L.base_ssyrk_n.argtypes = [ctypes.c_char, _i64, _i64, _flt, _pf, _i64, _flt, _pf, _i64]
+    L.base_ssyrk_n.restype = None
+    L.base_ssymm.argtypes = [ctypes.c_char, ctypes.c_char, _i64, _i64, _flt, _pf, _i64, _pf, _i64, _flt, _pf,
+                             _i64]
+    L.base_ssymm.restype = None
+    L.base_strmm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, _i64, _i64, _flt, _pf,
+                             _i64, _pf, _i64]
+    L.base_strmm.restype = None
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_dgemm.argtypes = [_cs, _cs, _i64, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
-        L.orc_dtrsm.argtypes = [_cs, _cs, _cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64]
-        L.orc_dtrmm.argtypes = [_cs, _cs, _cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64]
-        L.orc_dsyrk.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _dbl, _pd, _i64]
-        L.orc_dsymm.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
-        L.orc_dsyr2k.argtypes = [_cs, _cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
-        L.orc_dgemv.argtypes = [_cs, _i64, _i64, _dbl, _pd, _i64, _pd, _i64, _dbl, _pd, _i64]
-        L.orc_dtrsv.argtypes = [_cs, _cs, _cs, _i64, _pd, _i64, _pd, _i64]
-        for f in (L.orc_dgemm, L.orc_dtrsm, L.orc_dtrmm, L.orc_dsyrk, L.orc_dsymm, L.orc_dsyr2k):
-            f.restype = ctypes.c_int
-        L.orc_sgemm_sample_f64.argtypes = [
-            ctypes.c_char, ctypes.c_char, _i64, _dbl, _pf, _i64, _pf, _i64, _dbl, _pf, _i64, _pl, _pl, _pd]
-        L.orc_sgemm_sample_f64.restype = None
-        L.orc_sgemm_sample_abs.argtypes = [
-            ctypes.c_char, ctypes.c_char, _i64, _pf, _i64, _pf, _i64, _i64, _pl, _pl, _pd]
-        L.orc_sgemm_sample_abs.restype = None
-        L.orc_fill_f32.argtypes = [_pf, _i64, ctypes.c_uint64, ctypes.c_int, _dbl, _dbl]
-        L.orc_fill_f32.restype = None
-        L.base_threads.restype = ctypes.c_int
-        L.base_sgemm.argtypes = [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, _flt, _pf, _i64, _pf, _i64,
-                                 _flt, _pf, _i64]
-        L.base_sgemm.restype = None
-        L.base_strsm_left.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, _i64, _i64, _flt, _pf, _i64,
-                                      _pf, _i64]
-        L.base_strsm_left.restype = None
-        L.base_ssyrk_n.argtypes = [ctypes.c_char, _i64, _i64, _flt, _pf, _i64, _flt, _pf, _i64]
-        L.base_ssyrk_n.restype = None
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+@contextlib.contextmanager
+def swapped(path):
+    """Run every wrapper below against another build of this library (the
+    tests use it for builds with a planted mistake, to show that a check
+    rejects them)."""
+    global _lib
+    old = lib()
+    _lib = _bind(ctypes.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = old
 
 
 def _p(x, t):
@@ -172,3 +197,11 @@ def base_strsm_left(uplo, trans, diag, m, n, alpha, a, lda, b, ldb):
 
 def base_ssyrk_n(uplo, n, k, alpha, a, lda, beta, c, ldc):
     lib().base_ssyrk_n(_s(uplo), n, k, alpha, _f(a), lda, beta, _f(c), ldc)
+
+
+def base_ssymm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc):
+    lib().base_ssymm(_s(side), _s(uplo), m, n, alpha, _f(a), lda, _f(b), ldb, beta, _f(c), ldc)
+
+
+def base_strmm(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb):
+    lib().base_strmm(_s(side), _s(uplo), _s(trans), _s(diag), m, n, alpha, _f(a), lda, _f(b), ldb)

@@ -16,7 +16,13 @@
  *                 outer-product micro-kernel (16 vector accumulators).
  *     others   -> _gemm_naive (gemm.mojo:127-377): column-parallel (n>=64)
  *                 axpy sweep over l, skipping B(l,j)==0, strided for TN/TT.
- *   base_strsm  trsm.mojo:20-275, serial column loop, zero-pivot skip.
+ *   base_strsm_left  trsm.mojo:96-168 (side L), serial column loop,
+ *                    zero-pivot skip.
+ *   base_ssyrk_n     syrk.mojo:89-166 (trans N), column-parallel axpy.
+ *   base_ssymm       symm.mojo:88-239, column-parallel fused axpy + dot
+ *                    (side L) or axpy sweep (side R), zero skip on B(l,j).
+ *   base_strmm       trmm.mojo:90-303, every side / uplo / trans branch
+ *                    with the reference's threading (only L,T parallel).
  *
  * Threads: OpenMP, the analogue of Mojo `parallelize`; base_threads() reports
  * how many the runtime uses.
@@ -223,6 +229,165 @@ void base_ssyrk_n(char uplo, int64_t n, int64_t k, float alpha, const float* a, 
             float temp = alpha * av;
             const float* al = a + l * lda;
             for (int64_t i = i0; i < i1; ++i) cj[i] += temp * al[i];
+        }
+    }
+}
+
+/* symm.mojo:88-239.  Side L: per column j (parallel when n >= 64), C(:,j)
+ * scaled by beta, then for each l with B(l,j) != 0 a fused axpy of
+ * alpha*B(l,j)*A(:,l) over the stored part of column l and a dot product
+ * temp2 = A(:,l).B(:,j) over the same rows, finished on the diagonal
+ * (upper: l from m-1 down, rows i < l; lower: l upward, rows i > l).  The
+ * reference skips the whole step, temp2 included, when B(l,j) == 0
+ * (symm.mojo:114,133); restated as written.  Side R: per column j,
+ * C(:,j) += alpha*A(j,j)*B(:,j) + sum_{k != j} alpha*A(k,j)*B(:,k), with
+ * A(k,j) read from the stored triangle. */
+void base_ssymm(char side, char uplo, int64_t m, int64_t n, float alpha, const float* a, int64_t lda,
+                const float* b, int64_t ldb, float beta, float* c, int64_t ldc) {
+    int left = (side == 'L' || side == 'l'), upper = (uplo == 'U' || uplo == 'u');
+    if (m == 0 || n == 0 || (alpha == 0.0f && beta == 1.0f)) return;
+#pragma omp parallel for schedule(static) if (n >= PAR_THRESHOLD)
+    for (int64_t j = 0; j < n; ++j) {
+        float* cj = c + j * ldc;
+        const float* bj = b + j * ldb;
+        if (beta == 0.0f)
+            for (int64_t i = 0; i < m; ++i) cj[i] = 0.0f;
+        else if (beta != 1.0f)
+            for (int64_t i = 0; i < m; ++i) cj[i] *= beta;
+        if (left) {
+            if (upper) {
+                for (int64_t l = m - 1; l >= 0; --l) {
+                    if (bj[l] == 0.0f) continue;
+                    float t1 = alpha * bj[l], t2 = 0.0f;
+                    const float* al = a + l * lda;
+                    for (int64_t i = 0; i < l; ++i) {
+                        cj[i] += t1 * al[i];
+                        t2 += al[i] * bj[i];
+                    }
+                    cj[l] = cj[l] + t1 * al[l] + alpha * t2;
+                }
+            } else {
+                for (int64_t l = 0; l < m; ++l) {
+                    if (bj[l] == 0.0f) continue;
+                    float t1 = alpha * bj[l], t2 = 0.0f;
+                    const float* al = a + l * lda;
+                    for (int64_t i = l + 1; i < m; ++i) {
+                        cj[i] += t1 * al[i];
+                        t2 += al[i] * bj[i];
+                    }
+                    cj[l] = cj[l] + t1 * al[l] + alpha * t2;
+                }
+            }
+        } else {
+            float td = alpha * a[j + j * lda];
+            for (int64_t i = 0; i < m; ++i) cj[i] += td * bj[i];
+            for (int64_t k = 0; k < n; ++k) {
+                if (k == j) continue;
+                /* A(k,j) of the symmetric matrix from the stored triangle. */
+                int stored_kj = upper ? (k < j) : (k > j);
+                float tk = alpha * (stored_kj ? a[k + j * lda] : a[j + k * lda]);
+                const float* bk = b + k * ldb;
+                for (int64_t i = 0; i < m; ++i) cj[i] += tk * bk[i];
+            }
+        }
+    }
+}
+
+static void trmm_scale_col(float* x, int64_t m, float t) {
+    for (int64_t i = 0; i < m; ++i) x[i] *= t;
+}
+static void trmm_axpy_col(float* y, const float* x, int64_t m, float t) {
+    for (int64_t i = 0; i < m; ++i) y[i] += t * x[i];
+}
+
+/* trmm.mojo:90-303: B := alpha*op(A)*B (side L) or alpha*B*op(A) (side R),
+ * loop for loop, with the reference's zero skips (B(k,j) == 0 on the left,
+ * A(k,j) == 0 on the right) and its threading: only the left transposed
+ * branches run columns in parallel (trmm.mojo:149-185); the others are
+ * serial, as the reference leaves them. */
+void base_strmm(char side, char uplo, char trans, char diag, int64_t m, int64_t n, float alpha, const float* a,
+                int64_t lda, float* b, int64_t ldb) {
+    int left = (side == 'L' || side == 'l'), upper = (uplo == 'U' || uplo == 'u');
+    int notr = is_n(trans), nounit = is_n(diag);
+    if (m == 0 || n == 0) return;
+    if (alpha == 0.0f) {
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < m; ++i) b[i + j * ldb] = 0.0f;
+        return;
+    }
+    if (left && notr) {
+        for (int64_t j = 0; j < n; ++j) {
+            float* bj = b + j * ldb;
+            if (upper) {
+                for (int64_t k = 0; k < m; ++k) {
+                    if (bj[k] == 0.0f) continue;
+                    float t = alpha * bj[k];
+                    trmm_axpy_col(bj, a + k * lda, k, t);
+                    if (nounit) t *= a[k + k * lda];
+                    bj[k] = t;
+                }
+            } else {
+                for (int64_t k = m - 1; k >= 0; --k) {
+                    if (bj[k] == 0.0f) continue;
+                    float t = alpha * bj[k];
+                    bj[k] = nounit ? t * a[k + k * lda] : t;
+                    trmm_axpy_col(bj + k + 1, a + k * lda + k + 1, m - k - 1, t);
+                }
+            }
+        }
+    } else if (left) {
+#pragma omp parallel for schedule(static) if (n >= PAR_THRESHOLD)
+        for (int64_t j = 0; j < n; ++j) {
+            float* bj = b + j * ldb;
+            if (upper) {
+                for (int64_t i = m - 1; i >= 0; --i) {
+                    float t = bj[i];
+                    if (nounit) t *= a[i + i * lda];
+                    for (int64_t kk = 0; kk < i; ++kk) t += a[kk + i * lda] * bj[kk];
+                    bj[i] = alpha * t;
+                }
+            } else {
+                for (int64_t i = 0; i < m; ++i) {
+                    float t = bj[i];
+                    if (nounit) t *= a[i + i * lda];
+                    for (int64_t kk = i + 1; kk < m; ++kk) t += a[kk + i * lda] * bj[kk];
+                    bj[i] = alpha * t;
+                }
+            }
+        }
+    } else if (notr) {
+        if (upper) {
+            for (int64_t j = n - 1; j >= 0; --j) {
+                float* bj = b + j * ldb;
+                trmm_scale_col(bj, m, nounit ? alpha * a[j + j * lda] : alpha);
+                for (int64_t kk = 0; kk < j; ++kk)
+                    if (a[kk + j * lda] != 0.0f) trmm_axpy_col(bj, b + kk * ldb, m, alpha * a[kk + j * lda]);
+            }
+        } else {
+            for (int64_t j = 0; j < n; ++j) {
+                float* bj = b + j * ldb;
+                trmm_scale_col(bj, m, nounit ? alpha * a[j + j * lda] : alpha);
+                for (int64_t kk = j + 1; kk < n; ++kk)
+                    if (a[kk + j * lda] != 0.0f) trmm_axpy_col(bj, b + kk * ldb, m, alpha * a[kk + j * lda]);
+            }
+        }
+    } else {
+        if (upper) {
+            for (int64_t k = 0; k < n; ++k) {
+                float* bk = b + k * ldb;
+                for (int64_t j = 0; j < k; ++j)
+                    if (a[j + k * lda] != 0.0f) trmm_axpy_col(b + j * ldb, bk, m, alpha * a[j + k * lda]);
+                float t = nounit ? alpha * a[k + k * lda] : alpha;
+                if (t != 1.0f) trmm_scale_col(bk, m, t);
+            }
+        } else {
+            for (int64_t k = n - 1; k >= 0; --k) {
+                float* bk = b + k * ldb;
+                for (int64_t j = k + 1; j < n; ++j)
+                    if (a[j + k * lda] != 0.0f) trmm_axpy_col(b + j * ldb, bk, m, alpha * a[j + k * lda]);
+                float t = nounit ? alpha * a[k + k * lda] : alpha;
+                if (t != 1.0f) trmm_scale_col(bk, m, t);
+            }
         }
     }
 }
