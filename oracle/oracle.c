@@ -371,6 +371,7 @@ void orc_sgemm_sample_f64(char transa, char transb, int64_t k, double alpha,
                           const float* a, int64_t lda, const float* b, int64_t ldb,
                           double beta, const float* c0, int64_t nsamp,
                           const int64_t* rows, const int64_t* cols, double* out) {
+#pragma omp parallel for schedule(dynamic, 64) if (nsamp * k > 1000000)
     for (int64_t s = 0; s < nsamp; ++s) {
         int64_t i = rows[s], j = cols[s];
         double acc = 0.0;
@@ -384,6 +385,7 @@ void orc_sgemm_sample_abs(char transa, char transb, int64_t k,
                           const float* a, int64_t lda, const float* b, int64_t ldb,
                           int64_t nsamp, const int64_t* rows, const int64_t* cols,
                           double* out_abs) {
+#pragma omp parallel for schedule(dynamic, 64) if (nsamp * k > 1000000)
     for (int64_t s = 0; s < nsamp; ++s) {
         int64_t i = rows[s], j = cols[s];
         double acc = 0.0;
