@@ -23,6 +23,18 @@
  *    synchronise; the returned buffers are final.
  *  - `_dev` entries take device pointers and a hipStream_t passed as void*
  *    (NULL = the default stream); they are stream-ordered and asynchronous.
+ *    Scratch they need comes from a grow-only workspace kept per (device,
+ *    stream); a call holds its stream's workspace from validation until its
+ *    last kernel is queued, so calls from several host threads on one stream
+ *    queue one after another.  A call captured into a hipGraph must not need
+ *    more workspace than the stream already holds (run it once eagerly
+ *    first); otherwise it returns MBH_E_ARG (a gemm before queueing
+ *    anything; a level-3 routine possibly after part of its work, so such a
+ *    capture is to be discarded).
+ *    Workspace handed out during capture stays allocated until
+ *    mbh_workspace_release or mbh_stream_destroy on that stream.
+ *  - `_host` entries are synchronous and thread-safe: each calling thread
+ *    stages through its own stream and device buffers.
  *  - Quick returns follow BLAS: m==0 or n==0 returns at once; alpha==0 or
  *    k==0 only scales C by beta (beta==0 writes zeros without reading C).
  */
@@ -60,6 +72,11 @@ const char* mbh_version(void);
 const char* mbh_last_error(void);                 /* thread-local message */
 int mbh_set_default_precision(int precision);     /* for _host entries; returns the old value */
 int mbh_get_default_precision(void);
+/* Precision of the _host entries on the calling thread only (no counterpart
+ * in the reference, whose signatures carry no precision): -1 clears it, so
+ * the thread follows the library default again.  Returns the thread's old
+ * setting (-1 if none) or MBH_E_PRECISION. */
+int mbh_set_thread_precision(int precision);
 int mbh_device_sync(void);
 /* Opt-in timing for measurement harnesses (no counterpart in the reference,
  * which has no GPU kernels to time): while enabled, every
@@ -214,6 +231,12 @@ int mbh_strsv_dev(char uplo, char trans, char diag, int64_t n, const float* a, i
                   float* x, int64_t incx, void* stream);
 int mbh_dtrsv_dev(char uplo, char trans, char diag, int64_t n, const double* a, int64_t lda,
                   double* x, int64_t incx, void* stream);
+/* Status of the last trsv _dev call queued on `stream` (no counterpart in the
+ * reference, whose trsv cannot time out): synchronises the stream and sets
+ * *failed nonzero if a block's bounded wait for an earlier block ran out, in
+ * which case that call's x is not the solution.  Without this query such a
+ * give-up in a _dev call is silent. */
+int mbh_trsv_status(void* stream, int* failed);
 
 /* ---- variable-size batched gemm (no counterpart in the reference) --------
  * count independent problems C_p := alpha*A_p*B_p + beta*C_p (each with the
@@ -302,8 +325,14 @@ int mbh_set_matrix(int64_t rows, int64_t cols, int64_t elem_size, const void* ho
 int mbh_get_matrix(int64_t rows, int64_t cols, int64_t elem_size, const void* dev, int64_t ldd, void* host,
                    int64_t ldh, void* stream);
 int mbh_stream_create(void** stream);
-int mbh_stream_destroy(void* stream);
+int mbh_stream_destroy(void* stream);  /* also releases the stream's workspace */
 int mbh_stream_sync(void* stream);
+/* Workspace of the _dev entries on one stream (no counterpart in the
+ * reference): release synchronises the stream and frees every workspace
+ * buffer it holds (graphs captured on it must not be replayed afterwards);
+ * bytes reports how much it holds.  NULL = the default stream. */
+int mbh_workspace_release(void* stream);
+int mbh_workspace_bytes(void* stream, size_t* bytes);
 
 #ifdef __cplusplus
 }

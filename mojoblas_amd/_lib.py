@@ -27,6 +27,7 @@ SIGNATURES = {
     "mbh_last_error": ([], ctypes.c_char_p),
     "mbh_set_default_precision": ([i32], i32),
     "mbh_get_default_precision": ([], i32),
+    "mbh_set_thread_precision": ([i32], i32),
     "mbh_device_sync": ([], i32),
     "mbh_timing_enable": ([i32], i32),
     "mbh_timing_read": ([vp, i32], i32),
@@ -66,6 +67,7 @@ SIGNATURES = {
     "mbh_dtrsv_host": ([ch, ch, ch, i64, vp, i64, vp, i64], i32),
     "mbh_strsv_dev": ([ch, ch, ch, i64, vp, i64, vp, i64, vp], i32),
     "mbh_dtrsv_dev": ([ch, ch, ch, i64, vp, i64, vp, i64, vp], i32),
+    "mbh_trsv_status": ([vp, ctypes.POINTER(i32)], i32),
     "mbh_vbatch_plan_create": ([i64, vp, vp, vp, vp, vp, vp, i32, vp], i32),
     "mbh_sgemm_vbatched_exec": ([vp, f32, vp, vp, f32, vp, vp], i32),
     "mbh_vbatch_plan_info": ([vp, vp], i32),
@@ -84,6 +86,8 @@ SIGNATURES = {
     "mbh_stream_create": ([ctypes.POINTER(vp)], i32),
     "mbh_stream_destroy": ([vp], i32),
     "mbh_stream_sync": ([vp], i32),
+    "mbh_workspace_release": ([vp], i32),
+    "mbh_workspace_bytes": ([vp, ctypes.POINTER(ctypes.c_size_t)], i32),
 }
 
 

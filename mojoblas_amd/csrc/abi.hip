@@ -9,6 +9,9 @@
 #include <mutex>
 #include <string>
 
+#include <sys/syscall.h>
+#include <unistd.h>
+
 #include "../../include/mojoblas_hip.h"
 #include "abi_internal.hpp"
 #include "dispatch.hpp"
@@ -29,6 +32,11 @@ void set_error(const char* fmt, ...) {
 }
 
 int hip_fail(hipError_t e, const char* where) {
+    if (e == hipErrorStreamCaptureUnsupported) {
+        set_error("%s: its workspace would have to grow while the stream is being captured; run the same call "
+                  "(or a larger one) on this stream once before capture", where);
+        return MBH_E_ARG;
+    }
     set_error("%s: %s", where, hipGetErrorString(e));
     return MBH_E_HIP;
 }
@@ -114,47 +122,112 @@ int default_precision() {
     return g_default_prec;
 }
 
+static thread_local int t_prec = -1;
+
+// Precision of the host entries on the calling thread: the thread's own
+// setting (mbh_set_thread_precision) if it has one, else the library default.
+int host_precision() { return t_prec >= 0 ? t_prec : default_precision(); }
+
 bool gemm_precision_ok(int prec) { return prec == MBH_PREC_FP32 || prec == MBH_PREC_BF16X3 || prec == MBH_PREC_FP8; }
 
 // ---- device staging for the host-pointer entries ----------------------------
-// Grow-only per-slot device buffers, guarded by one mutex that is held for the
-// whole host call (host entries are synchronous).
+// Each host thread stages through its own HIP stream (non-blocking, default
+// priority) and its own grow-only device buffers, so host calls from
+// different threads run concurrently and share nothing; the stream's
+// workspace is leased for the call like any other.  A thread's stream and
+// buffers are released when the thread exits (the main thread's at process
+// exit, by the operating system).
+namespace {
+constexpr int kMaxDev = 16;
+constexpr int kStageSlots = 8;
 struct Slot {
     void* p = nullptr;
     size_t cap = 0;
 };
-static std::mutex g_stage_mu;
-static Slot g_slots[8];
+struct DevStage {
+    hipStream_t s = nullptr;
+    Slot slots[kStageSlots];
+};
+struct HostCtx {
+    DevStage dev[kMaxDev];
+    ~HostCtx() {
+        // The main thread's context dies in exit(), when the HIP runtime may
+        // already be going away: leave it to the operating system.
+        if (syscall(SYS_gettid) == getpid()) return;
+        for (int d = 0; d < kMaxDev; ++d) {
+            DevStage& st = dev[d];
+            if (!st.s) continue;
+            if (hipSetDevice(d) != hipSuccess) continue;
+            (void)workspace_release(st.s);
+            for (Slot& sl : st.slots)
+                if (sl.p) (void)hipFree(sl.p);
+            (void)hipStreamDestroy(st.s);
+        }
+    }
+};
+thread_local HostCtx t_host;
+
+DevStage* current_stage(hipError_t* err) {
+    int d = 0;
+    if ((*err = hipGetDevice(&d)) != hipSuccess) return nullptr;
+    if (d < 0 || d >= kMaxDev) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    DevStage& st = t_host.dev[d];
+    if (!st.s && (*err = hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking)) != hipSuccess) {
+        st.s = nullptr;
+        return nullptr;
+    }
+    return &st;
+}
+}  // namespace
+
+HostStage::HostStage() {
+    DevStage* st = current_stage(&err_);
+    if (!st) return;
+    s_ = st->s;
+    lease_.reset(new StreamLease(s_));
+    err_ = lease_->error();
+}
 
 hipError_t slot_get(int i, size_t bytes, void** out) {
-    Slot& s = g_slots[i];
+    hipError_t e = hipSuccess;
+    DevStage* st = current_stage(&e);
+    if (!st) return e;
+    if (i < 0 || i >= kStageSlots) return hipErrorInvalidValue;
+    Slot& sl = st->slots[i];
     if (bytes == 0) bytes = 16;
-    if (s.cap < bytes) {
-        if (s.p) {
-            hipError_t e = hipFree(s.p);
-            if (e != hipSuccess) return e;
-            s.p = nullptr;
-            s.cap = 0;
+    if (sl.cap < bytes) {
+        if (sl.p) {
+            // earlier work on this thread's stream may still read the buffer
+            if ((e = hipStreamSynchronize(st->s)) != hipSuccess) return e;
+            if ((e = hipFree(sl.p)) != hipSuccess) return e;
+            sl.p = nullptr;
+            sl.cap = 0;
         }
-        hipError_t e = hipMalloc(&s.p, bytes);
-        if (e != hipSuccess) return e;
-        s.cap = bytes;
+        if ((e = hipMalloc(&sl.p, bytes)) != hipSuccess) return e;
+        sl.cap = bytes;
     }
-    *out = s.p;
+    *out = sl.p;
     return hipSuccess;
 }
 
-std::unique_lock<std::mutex> stage_lock() { return std::unique_lock<std::mutex>(g_stage_mu); }
-
 // Copy an r x c column-major host matrix (leading dim ld) to a packed device
-// buffer (leading dim max(1, r)), and back.
-hipError_t put_matrix(void* dev, const void* host, int64_t r, int64_t c, int64_t ld, size_t es) {
+// buffer (leading dim max(1, r)) on stream s; and back, after which s is
+// synchronised so the host buffer is final.
+hipError_t put_matrix(void* dev, const void* host, int64_t r, int64_t c, int64_t ld, size_t es, hipStream_t s) {
     if (r <= 0 || c <= 0) return hipSuccess;
-    return hipMemcpy2D(dev, (size_t)r * es, host, (size_t)ld * es, (size_t)r * es, (size_t)c, hipMemcpyHostToDevice);
+    return hipMemcpy2DAsync(dev, (size_t)r * es, host, (size_t)ld * es, (size_t)r * es, (size_t)c,
+                            hipMemcpyHostToDevice, s);
 }
-hipError_t get_matrix(void* host, const void* dev, int64_t r, int64_t c, int64_t ld, size_t es) {
-    if (r <= 0 || c <= 0) return hipSuccess;
-    return hipMemcpy2D(host, (size_t)ld * es, dev, (size_t)r * es, (size_t)r * es, (size_t)c, hipMemcpyDeviceToHost);
+hipError_t get_matrix(void* host, const void* dev, int64_t r, int64_t c, int64_t ld, size_t es, hipStream_t s) {
+    if (r > 0 && c > 0) {
+        hipError_t e = hipMemcpy2DAsync(host, (size_t)ld * es, dev, (size_t)r * es, (size_t)r * es, (size_t)c,
+                                        hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(s);
 }
 
 template <typename T>
@@ -166,7 +239,9 @@ static int gemm_host(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha,
     const bool at = !flag_n(ta), bt = !flag_n(tb);
     const bool need_ab = !(alpha == T(0) || k == 0);
     const int64_t ar = at ? k : m, ac = at ? m : k, br = bt ? n : k, bc = bt ? k : n;
-    auto lk = stage_lock();
+    HostStage st;
+    if (st.error() != hipSuccess) return hip_fail(st.error(), "host staging");
+    const hipStream_t hs = st.stream();
     void *dA = nullptr, *dB = nullptr, *dC = nullptr;
     hipError_t e;
     const size_t es = sizeof(T);
@@ -174,19 +249,19 @@ static int gemm_host(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha,
     if (need_ab) {
         if ((e = slot_get(0, (size_t)ar * ac * es, &dA)) != hipSuccess) return hip_fail(e, "hipMalloc(A)");
         if ((e = slot_get(1, (size_t)br * bc * es, &dB)) != hipSuccess) return hip_fail(e, "hipMalloc(B)");
-        if ((e = put_matrix(dA, a, ar, ac, lda, es)) != hipSuccess) return hip_fail(e, "copy A");
-        if ((e = put_matrix(dB, b, br, bc, ldb, es)) != hipSuccess) return hip_fail(e, "copy B");
+        if ((e = put_matrix(dA, a, ar, ac, lda, es, hs)) != hipSuccess) return hip_fail(e, "copy A");
+        if ((e = put_matrix(dB, b, br, bc, ldb, es, hs)) != hipSuccess) return hip_fail(e, "copy B");
     }
     if (beta != T(0))
-        if ((e = put_matrix(dC, c, m, n, ldc, es)) != hipSuccess) return hip_fail(e, "copy C");
+        if ((e = put_matrix(dC, c, m, n, ldc, es, hs)) != hipSuccess) return hip_fail(e, "copy C");
     if constexpr (sizeof(T) == 4)
         e = sgemm(at, bt, m, n, k, alpha, (const float*)dA, max1(ar), (const float*)dB, max1(br), beta, (float*)dC, m,
-                  prec, nullptr);
+                  prec, hs);
     else
         e = dgemm(at, bt, m, n, k, alpha, (const double*)dA, max1(ar), (const double*)dB, max1(br), beta,
-                  (double*)dC, m, nullptr);
+                  (double*)dC, m, hs);
     if (e != hipSuccess) return hip_fail(e, "gemm launch");
-    if ((e = get_matrix(c, dC, m, n, ldc, es)) != hipSuccess) return hip_fail(e, "copy C back");
+    if ((e = get_matrix(c, dC, m, n, ldc, es, hs)) != hipSuccess) return hip_fail(e, "copy C back");
     return 0;
 }
 
@@ -199,7 +274,9 @@ static int gemm16_host(bool f16, char ta, char tb, int64_t m, int64_t n, int64_t
     const bool at = !flag_n(ta), bt = !flag_n(tb);
     const bool need_ab = !(alpha == 0.0f || k == 0);
     const int64_t ar = at ? k : m, ac = at ? m : k, br = bt ? n : k, bc = bt ? k : n;
-    auto lk = stage_lock();
+    HostStage st;
+    if (st.error() != hipSuccess) return hip_fail(st.error(), "host staging");
+    const hipStream_t hs = st.stream();
     void *dA = nullptr, *dB = nullptr, *dC = nullptr;
     hipError_t e;
     const size_t es = sizeof(uint16_t);
@@ -207,15 +284,15 @@ static int gemm16_host(bool f16, char ta, char tb, int64_t m, int64_t n, int64_t
     if (need_ab) {
         if ((e = slot_get(0, (size_t)ar * ac * es, &dA)) != hipSuccess) return hip_fail(e, "hipMalloc(A)");
         if ((e = slot_get(1, (size_t)br * bc * es, &dB)) != hipSuccess) return hip_fail(e, "hipMalloc(B)");
-        if ((e = put_matrix(dA, a, ar, ac, lda, es)) != hipSuccess) return hip_fail(e, "copy A");
-        if ((e = put_matrix(dB, b, br, bc, ldb, es)) != hipSuccess) return hip_fail(e, "copy B");
+        if ((e = put_matrix(dA, a, ar, ac, lda, es, hs)) != hipSuccess) return hip_fail(e, "copy A");
+        if ((e = put_matrix(dB, b, br, bc, ldb, es, hs)) != hipSuccess) return hip_fail(e, "copy B");
     }
     if (beta != 0.0f)
-        if ((e = put_matrix(dC, c, m, n, ldc, es)) != hipSuccess) return hip_fail(e, "copy C");
+        if ((e = put_matrix(dC, c, m, n, ldc, es, hs)) != hipSuccess) return hip_fail(e, "copy C");
     e = gemm16(f16, at, bt, m, n, k, alpha, (const uint16_t*)dA, max1(ar), (const uint16_t*)dB, max1(br), beta,
-               (uint16_t*)dC, m, nullptr);
+               (uint16_t*)dC, m, hs);
     if (e != hipSuccess) return hip_fail(e, "gemm launch");
-    if ((e = get_matrix(c, dC, m, n, ldc, es)) != hipSuccess) return hip_fail(e, "copy C back");
+    if ((e = get_matrix(c, dC, m, n, ldc, es, hs)) != hipSuccess) return hip_fail(e, "copy C back");
     return 0;
 }
 
@@ -225,6 +302,8 @@ static int gemm16_dev(bool f16, const char* name, char ta, char tb, int64_t m, i
     int info = check_gemm(ta, tb, m, n, k, lda, ldb, ldc);
     if (info) return info;
     if (m == 0 || n == 0) return 0;
+    StreamLease lease((hipStream_t)stream);
+    if (lease.error() != hipSuccess) return hip_fail(lease.error(), name);
     hipError_t e = gemm16(f16, !flag_n(ta), !flag_n(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
                           (hipStream_t)stream);
     return e == hipSuccess ? 0 : hip_fail(e, name);
@@ -251,6 +330,16 @@ int mbh_set_default_precision(int precision) {
 }
 int mbh_get_default_precision(void) { return default_precision(); }
 
+int mbh_set_thread_precision(int precision) {
+    if (precision != -1 && !gemm_precision_ok(precision)) {
+        set_error("mbh_set_thread_precision: invalid precision %d", precision);
+        return MBH_E_PRECISION;
+    }
+    const int old = t_prec;
+    t_prec = precision;
+    return old;
+}
+
 int mbh_device_sync(void) {
     hipError_t e = hipDeviceSynchronize();
     return e == hipSuccess ? 0 : hip_fail(e, "hipDeviceSynchronize");
@@ -258,7 +347,7 @@ int mbh_device_sync(void) {
 
 int mbh_sgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const float* a,
                    int64_t lda, const float* b, int64_t ldb, float beta, float* c, int64_t ldc) {
-    return gemm_host<float>(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, default_precision());
+    return gemm_host<float>(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, host_precision());
 }
 
 int mbh_dgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
@@ -276,6 +365,8 @@ int mbh_sgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, flo
         return MBH_E_PRECISION;
     }
     if (m == 0 || n == 0) return 0;
+    StreamLease lease((hipStream_t)stream);
+    if (lease.error() != hipSuccess) return hip_fail(lease.error(), "mbh_sgemm_dev");
     hipError_t e = sgemm(!flag_n(transa), !flag_n(transb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, precision,
                          (hipStream_t)stream);
     return e == hipSuccess ? 0 : hip_fail(e, "mbh_sgemm_dev");
@@ -286,6 +377,8 @@ int mbh_dgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, dou
     int info = check_gemm(transa, transb, m, n, k, lda, ldb, ldc);
     if (info) return info;
     if (m == 0 || n == 0) return 0;
+    StreamLease lease((hipStream_t)stream);
+    if (lease.error() != hipSuccess) return hip_fail(lease.error(), "mbh_dgemm_dev");
     hipError_t e = dgemm(!flag_n(transa), !flag_n(transb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
                          (hipStream_t)stream);
     return e == hipSuccess ? 0 : hip_fail(e, "mbh_dgemm_dev");

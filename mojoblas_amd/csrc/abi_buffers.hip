@@ -115,13 +115,34 @@ int mbh_stream_create(void** stream) {
 
 int mbh_stream_destroy(void* stream) {
     if (!stream) return 0;
-    const hipError_t e = hipStreamDestroy((hipStream_t)stream);
+    // The stream's workspace goes with it (release synchronises the stream).
+    hipError_t e = mbh::workspace_release((hipStream_t)stream);
+    if (e != hipSuccess) return mbh::hip_fail(e, "mbh_stream_destroy: workspace");
+    e = hipStreamDestroy((hipStream_t)stream);
     return e == hipSuccess ? 0 : mbh::hip_fail(e, "mbh_stream_destroy");
 }
 
 int mbh_stream_sync(void* stream) {
     const hipError_t e = hipStreamSynchronize((hipStream_t)stream);
     return e == hipSuccess ? 0 : mbh::hip_fail(e, "mbh_stream_sync");
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mbh_workspace_release(void* stream) {
+    const hipError_t e = mbh::workspace_release((hipStream_t)stream);
+    return e == hipSuccess ? 0 : mbh::hip_fail(e, "mbh_workspace_release");
+}
+
+int mbh_workspace_bytes(void* stream, size_t* bytes) {
+    if (!bytes) {
+        mbh::set_error("mbh_workspace_bytes: null output pointer");
+        return MBH_E_ARG;
+    }
+    const hipError_t e = mbh::workspace_bytes((hipStream_t)stream, bytes);
+    return e == hipSuccess ? 0 : mbh::hip_fail(e, "mbh_workspace_bytes");
 }
 
 }  // extern "C"

@@ -21,18 +21,26 @@ struct Staged {
 };
 
 template <typename T>
-hipError_t stage_in(int slot, const T* host, int64_t r, int64_t c, int64_t ld, Staged& out) {
+hipError_t stage_in(int slot, const T* host, int64_t r, int64_t c, int64_t ld, Staged& out, hipStream_t s) {
     out.r = r;
     out.c = c;
     hipError_t e = slot_get(slot, (size_t)max1(r) * max1(c) * sizeof(T), &out.p);
     if (e != hipSuccess) return e;
-    return put_matrix(out.p, host, r, c, ld, sizeof(T));
+    return put_matrix(out.p, host, r, c, ld, sizeof(T), s);
 }
 
 template <typename T>
-hipError_t stage_out(T* host, const Staged& st, int64_t ld) {
-    return get_matrix(host, st.p, st.r, st.c, ld, sizeof(T));
+hipError_t stage_out(T* host, const Staged& st, int64_t ld, hipStream_t s) {
+    return get_matrix(host, st.p, st.r, st.c, ld, sizeof(T), s);
 }
+
+#define MBH_LEASE(name)                                                               \
+    StreamLease lease((hipStream_t)stream);                                           \
+    if (lease.error() != hipSuccess) return hip_fail(lease.error(), name)
+#define MBH_HOST_STAGE(name)                                                          \
+    HostStage st;                                                                     \
+    if (st.error() != hipSuccess) return hip_fail(st.error(), name ": host staging"); \
+    const hipStream_t hs = st.stream()
 
 template <typename T>
 int trsm_dev(char side, char uplo, char ta, char diag, int64_t m, int64_t n, T alpha, const T* a, int64_t lda, T* b,
@@ -48,6 +56,7 @@ int trsm_dev(char side, char uplo, char ta, char diag, int64_t m, int64_t n, T a
         return MBH_E_ARG;
     }
     if (m == 0 || n == 0) return 0;
+    MBH_LEASE("trsm");
     return finish(trsm<T>(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, prec, refine, (hipStream_t)stream),
                   "trsm");
 }
@@ -59,15 +68,15 @@ int trsm_host(char side, char uplo, char ta, char diag, int64_t m, int64_t n, T 
     if (info) return info;
     if (m == 0 || n == 0) return 0;
     const int64_t na = flag_l(side) ? m : n;
-    auto lk = stage_lock();
+    MBH_HOST_STAGE("trsm");
     Staged A, B;
     hipError_t e;
-    if ((e = stage_in(0, a, na, na, lda, A)) != hipSuccess) return hip_fail(e, "trsm: stage A");
-    if ((e = stage_in(1, b, m, n, ldb, B)) != hipSuccess) return hip_fail(e, "trsm: stage B");
-    if ((e = trsm<T>(side, uplo, ta, diag, m, n, alpha, (const T*)A.p, max1(na), (T*)B.p, m, default_precision(), 1,
-                     nullptr)) != hipSuccess)
+    if ((e = stage_in(0, a, na, na, lda, A, hs)) != hipSuccess) return hip_fail(e, "trsm: stage A");
+    if ((e = stage_in(1, b, m, n, ldb, B, hs)) != hipSuccess) return hip_fail(e, "trsm: stage B");
+    if ((e = trsm<T>(side, uplo, ta, diag, m, n, alpha, (const T*)A.p, max1(na), (T*)B.p, m, host_precision(), 1,
+                     hs)) != hipSuccess)
         return hip_fail(e, "trsm");
-    return finish(stage_out(b, B, ldb), "trsm: copy B back");
+    return finish(stage_out(b, B, ldb, hs), "trsm: copy B back");
 }
 
 template <typename T>
@@ -80,6 +89,7 @@ int trmm_dev(char side, char uplo, char ta, char diag, int64_t m, int64_t n, T a
         return MBH_E_PRECISION;
     }
     if (m == 0 || n == 0) return 0;
+    MBH_LEASE("trmm");
     return finish(trmm<T>(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, prec, (hipStream_t)stream), "trmm");
 }
 
@@ -90,16 +100,16 @@ int trmm_host(char side, char uplo, char ta, char diag, int64_t m, int64_t n, T 
     if (info) return info;
     if (m == 0 || n == 0) return 0;
     const int64_t na = flag_l(side) ? m : n;
-    auto lk = stage_lock();
+    MBH_HOST_STAGE("trmm");
     Staged A, B;
     hipError_t e;
     if (alpha != T(0))
-        if ((e = stage_in(0, a, na, na, lda, A)) != hipSuccess) return hip_fail(e, "trmm: stage A");
-    if ((e = stage_in(1, b, m, n, ldb, B)) != hipSuccess) return hip_fail(e, "trmm: stage B");
-    if ((e = trmm<T>(side, uplo, ta, diag, m, n, alpha, (const T*)A.p, max1(na), (T*)B.p, m, default_precision(),
-                     nullptr)) != hipSuccess)
+        if ((e = stage_in(0, a, na, na, lda, A, hs)) != hipSuccess) return hip_fail(e, "trmm: stage A");
+    if ((e = stage_in(1, b, m, n, ldb, B, hs)) != hipSuccess) return hip_fail(e, "trmm: stage B");
+    if ((e = trmm<T>(side, uplo, ta, diag, m, n, alpha, (const T*)A.p, max1(na), (T*)B.p, m, host_precision(),
+                     hs)) != hipSuccess)
         return hip_fail(e, "trmm");
-    return finish(stage_out(b, B, ldb), "trmm: copy B back");
+    return finish(stage_out(b, B, ldb, hs), "trmm: copy B back");
 }
 
 template <typename T>
@@ -112,6 +122,7 @@ int syrk_dev(char uplo, char trans, int64_t n, int64_t k, T alpha, const T* a, i
         return MBH_E_PRECISION;
     }
     if (n == 0) return 0;
+    MBH_LEASE("syrk");
     return finish(syrk<T>(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, prec, (hipStream_t)stream), "syrk");
 }
 
@@ -123,15 +134,15 @@ int syrk_host(char uplo, char trans, int64_t n, int64_t k, T alpha, const T* a, 
     if (n == 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return 0;
     const bool tr = !flag_n(trans);
     const int64_t ar = tr ? k : n, ac = tr ? n : k;
-    auto lk = stage_lock();
+    MBH_HOST_STAGE("syrk");
     Staged A, C;
     hipError_t e;
-    if ((e = stage_in(0, a, ar, ac, lda, A)) != hipSuccess) return hip_fail(e, "syrk: stage A");
-    if ((e = stage_in(2, c, n, n, ldc, C)) != hipSuccess) return hip_fail(e, "syrk: stage C");
-    if ((e = syrk<T>(uplo, trans, n, k, alpha, (const T*)A.p, max1(ar), beta, (T*)C.p, n, default_precision(),
-                     nullptr)) != hipSuccess)
+    if ((e = stage_in(0, a, ar, ac, lda, A, hs)) != hipSuccess) return hip_fail(e, "syrk: stage A");
+    if ((e = stage_in(2, c, n, n, ldc, C, hs)) != hipSuccess) return hip_fail(e, "syrk: stage C");
+    if ((e = syrk<T>(uplo, trans, n, k, alpha, (const T*)A.p, max1(ar), beta, (T*)C.p, n, host_precision(),
+                     hs)) != hipSuccess)
         return hip_fail(e, "syrk");
-    return finish(stage_out(c, C, ldc), "syrk: copy C back");
+    return finish(stage_out(c, C, ldc, hs), "syrk: copy C back");
 }
 
 template <typename T>
@@ -144,6 +155,7 @@ int syr2k_dev(char uplo, char trans, int64_t n, int64_t k, T alpha, const T* a, 
         return MBH_E_PRECISION;
     }
     if (n == 0) return 0;
+    MBH_LEASE("syr2k");
     return finish(syr2k<T>(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc, prec, (hipStream_t)stream),
                   "syr2k");
 }
@@ -156,16 +168,16 @@ int syr2k_host(char uplo, char trans, int64_t n, int64_t k, T alpha, const T* a,
     if (n == 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return 0;
     const bool tr = !flag_n(trans);
     const int64_t ar = tr ? k : n, ac = tr ? n : k;
-    auto lk = stage_lock();
+    MBH_HOST_STAGE("syr2k");
     Staged A, B, C;
     hipError_t e;
-    if ((e = stage_in(0, a, ar, ac, lda, A)) != hipSuccess) return hip_fail(e, "syr2k: stage A");
-    if ((e = stage_in(1, b, ar, ac, ldb, B)) != hipSuccess) return hip_fail(e, "syr2k: stage B");
-    if ((e = stage_in(2, c, n, n, ldc, C)) != hipSuccess) return hip_fail(e, "syr2k: stage C");
+    if ((e = stage_in(0, a, ar, ac, lda, A, hs)) != hipSuccess) return hip_fail(e, "syr2k: stage A");
+    if ((e = stage_in(1, b, ar, ac, ldb, B, hs)) != hipSuccess) return hip_fail(e, "syr2k: stage B");
+    if ((e = stage_in(2, c, n, n, ldc, C, hs)) != hipSuccess) return hip_fail(e, "syr2k: stage C");
     if ((e = syr2k<T>(uplo, trans, n, k, alpha, (const T*)A.p, max1(ar), (const T*)B.p, max1(ar), beta, (T*)C.p, n,
-                      default_precision(), nullptr)) != hipSuccess)
+                      host_precision(), hs)) != hipSuccess)
         return hip_fail(e, "syr2k");
-    return finish(stage_out(c, C, ldc), "syr2k: copy C back");
+    return finish(stage_out(c, C, ldc, hs), "syr2k: copy C back");
 }
 
 template <typename T>
@@ -178,6 +190,7 @@ int symm_dev(char side, char uplo, int64_t m, int64_t n, T alpha, const T* a, in
         return MBH_E_PRECISION;
     }
     if (m == 0 || n == 0) return 0;
+    MBH_LEASE("symm");
     return finish(symm<T>(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, prec, (hipStream_t)stream), "symm");
 }
 
@@ -188,16 +201,16 @@ int symm_host(char side, char uplo, int64_t m, int64_t n, T alpha, const T* a, i
     if (info) return info;
     if (m == 0 || n == 0 || (alpha == T(0) && beta == T(1))) return 0;
     const int64_t na = flag_l(side) ? m : n;
-    auto lk = stage_lock();
+    MBH_HOST_STAGE("symm");
     Staged A, B, C;
     hipError_t e;
-    if ((e = stage_in(0, a, na, na, lda, A)) != hipSuccess) return hip_fail(e, "symm: stage A");
-    if ((e = stage_in(1, b, m, n, ldb, B)) != hipSuccess) return hip_fail(e, "symm: stage B");
-    if ((e = stage_in(2, c, m, n, ldc, C)) != hipSuccess) return hip_fail(e, "symm: stage C");
+    if ((e = stage_in(0, a, na, na, lda, A, hs)) != hipSuccess) return hip_fail(e, "symm: stage A");
+    if ((e = stage_in(1, b, m, n, ldb, B, hs)) != hipSuccess) return hip_fail(e, "symm: stage B");
+    if ((e = stage_in(2, c, m, n, ldc, C, hs)) != hipSuccess) return hip_fail(e, "symm: stage C");
     if ((e = symm<T>(side, uplo, m, n, alpha, (const T*)A.p, max1(na), (const T*)B.p, m, beta, (T*)C.p, m,
-                     default_precision(), nullptr)) != hipSuccess)
+                     host_precision(), hs)) != hipSuccess)
         return hip_fail(e, "symm");
-    return finish(stage_out(c, C, ldc), "symm: copy C back");
+    return finish(stage_out(c, C, ldc, hs), "symm: copy C back");
 }
 
 }  // namespace

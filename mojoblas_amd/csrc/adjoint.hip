@@ -64,6 +64,8 @@ int mbh_sgemm_backward(char transa, char transb, int64_t m, int64_t n, int64_t k
         return MBH_E_PRECISION;
     }
     hipStream_t s = (hipStream_t)stream;
+    StreamLease lease(s);
+    if (lease.error() != hipSuccess) return hip_fail(lease.error(), "mbh_sgemm_backward");
     hipError_t e = hipSuccess;
     if (da) {
         if (!at)  // dA (m x k) = alpha * dC * op(B)^T
@@ -101,6 +103,8 @@ int mbh_strsm_backward(char side, char uplo, char transa, char diag, int64_t m, 
         set_error("mbh_strsm_backward: invalid precision %d", precision);
         return MBH_E_PRECISION;
     }
+    StreamLease lease((hipStream_t)stream);
+    if (lease.error() != hipSuccess) return hip_fail(lease.error(), "mbh_strsm_backward");
     if (m == 0 || n == 0) {
         if (da && na > 0) {
             hipError_t e = scale<float>(na, na, 0.0f, da, ldda, (hipStream_t)stream);

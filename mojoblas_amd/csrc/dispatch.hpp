@@ -35,10 +35,40 @@ hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double
                        int krange = 0);
 
 // Device scratch owned by the library: grow-only buffers per (device, stream,
-// slot); slot 0 belongs to the gemm pack pass, slot 1 to the level-3 drivers.
-// A buffer stays valid until the next request on the same (stream, slot) that
-// asks for more.
+// slot); slot 0 belongs to the gemm pack pass, slot 1 to the level-3 drivers,
+// slots 2 and 3 to gemv partials and trsv.  The calling thread must hold the
+// stream's lease (StreamLease) for the whole call; a buffer stays valid until
+// the next request on the same (stream, slot) that asks for more.  Returns
+// hipErrorStreamCaptureUnsupported when a slot would have to grow while the
+// stream is being captured, hipErrorInvalidHandle without a lease.
 hipError_t workspace(hipStream_t s, int slot, size_t bytes, void** out);
+
+// Holds the lease on a stream's workspace for one library call: other
+// threads' calls on the same stream wait, so calls queue their kernels whole.
+// Nested leases on the same stream by one thread are no-ops.
+class StreamLease {
+   public:
+    explicit StreamLease(hipStream_t s);
+    ~StreamLease();
+    StreamLease(const StreamLease&) = delete;
+    StreamLease& operator=(const StreamLease&) = delete;
+    hipError_t error() const { return err_; }
+
+   private:
+    void* held_ = nullptr;
+    void* prev_ = nullptr;
+    hipError_t err_ = hipSuccess;
+};
+
+// Frees every workspace buffer of a stream (after synchronising it),
+// including those retired while graphs captured on it may still use them.
+hipError_t workspace_release(hipStream_t s);
+// Bytes of workspace a stream holds.
+hipError_t workspace_bytes(hipStream_t s, size_t* bytes);
+// trsv's status word (nonzero when a block's bounded wait gave up): recorded
+// per stream by the queued trsv, read back after synchronising the stream.
+void workspace_note_trsv(hipStream_t s, int* fail);
+hipError_t workspace_trsv_status(hipStream_t s, int* failed);
 
 // C := beta*C on an m x n block (beta == 0 writes zeros without reading);
 // `tri` restricts to one triangle.

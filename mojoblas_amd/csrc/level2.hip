@@ -429,6 +429,7 @@ hipError_t trsv(char uplo, char trans, char diag, int64_t n, const T* A, int64_t
     trsv_lookback_kernel<T><<<dim3((unsigned)nblk), TV_NT, 0, s>>>(n, A, lda, eff_lower ? 1 : 0, tr ? 1 : 0, avec, inv,
                                                                   xc, flags, ticket, (int)nblk, fail);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    workspace_note_trsv(s, fail);
     if (fail_out) *fail_out = fail;
     if (incx == 1) return hipSuccess;
     vec_copy_kernel<T><<<grid1(n), 256, 0, s>>>(n, xc, 1, x, incx);
@@ -468,6 +469,8 @@ int gemv_dev(char trans, int64_t m, int64_t n, T alpha, const T* a, int64_t lda,
              T* y, int64_t incy, void* stream) {
     int info = check_gemv(trans, m, n, lda, incx, incy);
     if (info) return info;
+    StreamLease lease((hipStream_t)stream);
+    if (lease.error() != hipSuccess) return hip_fail(lease.error(), "gemv");
     return finish(gemv<T>(trans != 'N', m, n, alpha, a, lda, x, incx, beta, y, incy, (hipStream_t)stream), "gemv");
 }
 
@@ -481,27 +484,31 @@ int gemv_host(char trans, int64_t m, int64_t n, T alpha, const T* a, int64_t lda
     if (m == 0 || n == 0 || (alpha == T(0) && beta == T(1))) return 0;
     const bool tr = trans != 'N';
     const int64_t lenx = tr ? m : n, leny = tr ? n : m, sx = span(lenx, incx), sy = span(leny, incy);
-    auto lk = stage_lock();
+    HostStage st;
+    if (st.error() != hipSuccess) return hip_fail(st.error(), "gemv: host staging");
+    const hipStream_t hs = st.stream();
     void *dA = nullptr, *dx = nullptr, *dy = nullptr;
     hipError_t e;
     if (alpha != T(0)) {
         if ((e = slot_get(0, (size_t)m * n * sizeof(T), &dA)) != hipSuccess) return hip_fail(e, "gemv: A");
-        if ((e = put_matrix(dA, a, m, n, lda, sizeof(T))) != hipSuccess) return hip_fail(e, "gemv: copy A");
+        if ((e = put_matrix(dA, a, m, n, lda, sizeof(T), hs)) != hipSuccess) return hip_fail(e, "gemv: copy A");
         if ((e = slot_get(1, (size_t)sx * sizeof(T), &dx)) != hipSuccess) return hip_fail(e, "gemv: x");
-        if ((e = put_matrix(dx, x, sx, 1, sx, sizeof(T))) != hipSuccess) return hip_fail(e, "gemv: copy x");
+        if ((e = put_matrix(dx, x, sx, 1, sx, sizeof(T), hs)) != hipSuccess) return hip_fail(e, "gemv: copy x");
     }
     if ((e = slot_get(2, (size_t)sy * sizeof(T), &dy)) != hipSuccess) return hip_fail(e, "gemv: y");
-    if ((e = put_matrix(dy, y, sy, 1, sy, sizeof(T))) != hipSuccess) return hip_fail(e, "gemv: copy y");
-    if ((e = gemv<T>(tr, m, n, alpha, (const T*)dA, max1(m), (const T*)dx, incx, beta, (T*)dy, incy, nullptr)) !=
+    if ((e = put_matrix(dy, y, sy, 1, sy, sizeof(T), hs)) != hipSuccess) return hip_fail(e, "gemv: copy y");
+    if ((e = gemv<T>(tr, m, n, alpha, (const T*)dA, max1(m), (const T*)dx, incx, beta, (T*)dy, incy, hs)) !=
         hipSuccess)
         return hip_fail(e, "gemv");
-    return finish(get_matrix(y, dy, sy, 1, sy, sizeof(T)), "gemv: copy y back");
+    return finish(get_matrix(y, dy, sy, 1, sy, sizeof(T), hs), "gemv: copy y back");
 }
 
 template <typename T>
 int trsv_dev(char uplo, char trans, char diag, int64_t n, const T* a, int64_t lda, T* x, int64_t incx, void* stream) {
     int info = check_trsv(uplo, trans, diag, n, lda, incx);
     if (info) return info;
+    StreamLease lease((hipStream_t)stream);
+    if (lease.error() != hipSuccess) return hip_fail(lease.error(), "trsv");
     return finish(trsv<T>(uplo, trans, diag, n, a, lda, x, incx, (hipStream_t)stream), "trsv");
 }
 
@@ -511,17 +518,19 @@ int trsv_host(char uplo, char trans, char diag, int64_t n, const T* a, int64_t l
     if (info) return info;
     if (n == 0) return 0;
     const int64_t sx = span(n, incx);
-    auto lk = stage_lock();
+    HostStage st;
+    if (st.error() != hipSuccess) return hip_fail(st.error(), "trsv: host staging");
+    const hipStream_t hs = st.stream();
     void *dA = nullptr, *dx = nullptr;
     hipError_t e;
     if ((e = slot_get(0, (size_t)n * n * sizeof(T), &dA)) != hipSuccess) return hip_fail(e, "trsv: A");
-    if ((e = put_matrix(dA, a, n, n, lda, sizeof(T))) != hipSuccess) return hip_fail(e, "trsv: copy A");
+    if ((e = put_matrix(dA, a, n, n, lda, sizeof(T), hs)) != hipSuccess) return hip_fail(e, "trsv: copy A");
     if ((e = slot_get(1, (size_t)sx * sizeof(T), &dx)) != hipSuccess) return hip_fail(e, "trsv: x");
-    if ((e = put_matrix(dx, x, sx, 1, sx, sizeof(T))) != hipSuccess) return hip_fail(e, "trsv: copy x");
+    if ((e = put_matrix(dx, x, sx, 1, sx, sizeof(T), hs)) != hipSuccess) return hip_fail(e, "trsv: copy x");
     int* fail = nullptr;
-    if ((e = trsv<T>(uplo, trans, diag, n, (const T*)dA, max1(n), (T*)dx, incx, nullptr, &fail)) != hipSuccess)
+    if ((e = trsv<T>(uplo, trans, diag, n, (const T*)dA, max1(n), (T*)dx, incx, hs, &fail)) != hipSuccess)
         return hip_fail(e, "trsv");
-    if ((e = get_matrix(x, dx, sx, 1, sx, sizeof(T))) != hipSuccess) return hip_fail(e, "trsv: copy x back");
+    if ((e = get_matrix(x, dx, sx, 1, sx, sizeof(T), hs)) != hipSuccess) return hip_fail(e, "trsv: copy x back");
     int failed = 0;
     if ((e = hipMemcpy(&failed, fail, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(e, "trsv: status");
@@ -569,6 +578,14 @@ int mbh_strsv_dev(char uplo, char trans, char diag, int64_t n, const float* a, i
 int mbh_dtrsv_dev(char uplo, char trans, char diag, int64_t n, const double* a, int64_t lda, double* x, int64_t incx,
                   void* stream) {
     return trsv_dev<double>(uplo, trans, diag, n, a, lda, x, incx, stream);
+}
+int mbh_trsv_status(void* stream, int* failed) {
+    if (!failed) {
+        set_error("mbh_trsv_status: null output pointer");
+        return MBH_E_ARG;
+    }
+    const hipError_t e = workspace_trsv_status((hipStream_t)stream, failed);
+    return e == hipSuccess ? 0 : hip_fail(e, "mbh_trsv_status");
 }
 
 }  // extern "C"

@@ -227,3 +227,65 @@ def test_device_matrix_rejects_bad_shapes_without_gpu():
         mb.DeviceMatrix(2, 2, "float8")
     with pytest.raises(ValueError):
         mb.DeviceMatrix(4, 2, np.float32, ld=3)
+
+
+def test_thread_precision_validated_and_thread_local_without_gpu():
+    import threading
+
+    L = _lib.lib()
+    assert L.mbh_set_thread_precision(-1) == -1  # nothing set on this thread yet
+    assert L.mbh_set_thread_precision(7) == -3  # MBH_E_PRECISION, setting unchanged
+    assert L.mbh_set_thread_precision(3) == -3  # bf16 is the batched gemm's mode only
+    assert L.mbh_set_thread_precision(1) == -1
+    seen = []
+    t = threading.Thread(target=lambda: seen.append(L.mbh_set_thread_precision(2)))
+    t.start()
+    t.join()
+    assert seen == [-1]  # another thread starts with no setting of its own
+    assert L.mbh_set_thread_precision(-1) == 1
+    assert L.mbh_get_default_precision() == 0  # the library default is untouched
+
+
+def test_python_rejects_bf16_precision_outside_batched_without_gpu():
+    a = np.zeros(4, np.float32)
+    for call in (lambda: mb.gemm("N", "N", 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a.copy(), 2, precision="bf16"),
+                 lambda: mb.trsm("L", "L", "N", "N", 2, 2, 1.0, a, 2, a.copy(), 2, precision="bf16"),
+                 lambda: mb.syrk("L", "N", 2, 2, 1.0, a, 2, 0.0, a.copy(), 2, precision=3),
+                 lambda: mb.gemm("N", "N", 2, 2, 2, 1.0, a, 2, a, 2, 0.0, a.copy(), 2, precision="tf32")):
+        with pytest.raises(ValueError):
+            call()
+
+
+@pytest.mark.parametrize("routine", ["gemm_a", "gemm_b", "gemm_c", "gemm_t", "trsm", "trmm", "syrk", "syr2k",
+                                     "symm", "gemv_x", "gemv_y", "trsv"])
+def test_python_rejects_short_buffers_without_gpu(routine):
+    """A buffer shorter than the storage a call reaches is refused before the
+    C-ABI sees it (host entries would otherwise copy past the array's end)."""
+    f = np.float32
+    ok, short = np.zeros(64, f), np.zeros(5, f)
+    calls = {
+        "gemm_a": lambda: mb.gemm("N", "N", 4, 4, 4, 1.0, short, 4, ok, 4, 0.0, ok.copy(), 4),
+        "gemm_b": lambda: mb.gemm("N", "T", 4, 4, 4, 1.0, ok, 4, short, 4, 0.0, ok.copy(), 4),
+        "gemm_c": lambda: mb.gemm("N", "N", 4, 4, 4, 1.0, ok, 4, ok, 4, 0.0, short, 4),
+        "gemm_t": lambda: mb.gemm("T", "N", 3, 4, 5, 1.0, np.zeros(14, f), 5, ok, 5, 0.0, ok.copy(), 3),
+        "trsm": lambda: mb.trsm("L", "L", "N", "N", 4, 4, 1.0, ok, 4, short, 4),
+        "trmm": lambda: mb.trmm("R", "U", "N", "N", 2, 4, 1.0, short, 4, ok.copy(), 2),
+        "syrk": lambda: mb.syrk("L", "T", 4, 4, 1.0, short, 4, 0.0, ok.copy(), 4),
+        "syr2k": lambda: mb.syr2k("L", "N", 4, 4, 1.0, ok, 4, short, 4, 0.0, ok.copy(), 4),
+        "symm": lambda: mb.symm("L", "U", 4, 4, 1.0, ok, 4, ok, 4, 1.0, short, 4),
+        "gemv_x": lambda: mb.gemv("N", 4, 4, 1.0, ok, 4, np.zeros(4, f), 2, 0.0, ok.copy(), 1),
+        "gemv_y": lambda: mb.gemv("T", 4, 4, 1.0, ok, 4, ok, 1, 0.0, np.zeros(3, f), 1),
+        "trsv": lambda: mb.trsv("U", "N", "N", 4, ok, 5, np.zeros(3, f), 1),
+    }
+    with pytest.raises(ValueError, match="holds"):
+        calls[routine]()
+
+
+def test_python_length_check_spares_unread_operands_without_gpu(capsys):
+    """alpha == 0 does not read A or B (gemm.mojo:215-237); invalid arguments
+    are the C-ABI's to report as info codes, not the length check's."""
+    c = np.full(4, 7, np.float32)
+    assert mb.gemm("N", "N", 0, 2, 2, 1.0, np.zeros(1, np.float32), 1, np.zeros(1, np.float32), 2, 0.0, c, 1) == 0
+    assert mb.gemm("N", "N", 2, 2, 2, 1.0, np.zeros(1, np.float32), 1, c, 2, 0.0, c, 2) == 8
+    assert "gemm: Info 8" in capsys.readouterr().out
+    assert (c == 7).all()
