@@ -104,6 +104,24 @@ int mbh_dgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, dou
                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                   double* c, int64_t ldc, void* stream);
 
+/* ---- gemm without allocation (no counterpart in the reference) ----------
+ * For callers that cannot allocate device memory: code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments, checks and result (bit for bit) as mbh_sgemm_dev, but the
+ * BF16x3 / FP8 pack planes live in a static device array of
+ * MBH_NOALLOC_SCRATCH_BYTES per GPU (held from the first call on that GPU
+ * until the process ends) instead of the stream's workspace; the entry calls
+ * no allocation routine and never synchronises.  A product whose planes need
+ * more (BF16x3 takes 4 bytes per element of op(A) and op(B)^T padded to
+ * whole tiles: square sizes up to about 2800 fit) returns MBH_E_ARG before
+ * queueing anything.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (several streams or threads) race, so the caller
+ * orders them.  The stream must belong to the current device. */
+#define MBH_NOALLOC_SCRATCH_BYTES (64ull * 1024ull * 1024ull)
+int mbh_sgemm_dev_noalloc(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                          const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
+                          float* c, int64_t ldc, int precision, void* stream);
+
 /* ---- 16-bit gemm: C := alpha*op(A)*op(B) + beta*C on half (h) or bfloat16
  * (b) data.  Replaces gemm[DType.float16] / gemm[DType.bfloat16]
  * (src/level3/gemm.mojo:56-124, whose non-f32 instantiations run the

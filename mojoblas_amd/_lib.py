@@ -35,6 +35,7 @@ SIGNATURES = {
     "mbh_dgemm_host": ([ch, ch, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64], i32),
     "mbh_sgemm_dev": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, i32, vp], i32),
     "mbh_dgemm_dev": ([ch, ch, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp], i32),
+    "mbh_sgemm_dev_noalloc": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, i32, vp], i32),
     "mbh_hgemm_host": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64], i32),
     "mbh_bgemm_host": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64], i32),
     "mbh_hgemm_dev": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, vp], i32),

@@ -13,9 +13,21 @@ namespace mbh {
 //             (square C);
 //  krange   : KRange, each tile visits only the k-tiles where a triangular
 //             op(A) / op(B) (starting at the operand's origin) is nonzero.
+//  scratch  : device buffer for the split-precision pack planes in place of
+//             the stream's workspace (nullptr = the workspace); a product
+//             that needs more than it holds returns hipErrorOutOfMemory
+//             before anything is queued.
+struct Scratch {
+    void* p;
+    size_t bytes;
+};
 hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
                  const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s,
-                 int tri = 0, int tri_grid = 0, int krange = 0);
+                 int tri = 0, int tri_grid = 0, int krange = 0, const Scratch* scratch = nullptr);
+// Bytes of pack-plane scratch that sgemm needs for a plain product (no
+// triangle grid or k-range) in precision `prec`: 0 for fp32, alpha == 0 or
+// k == 0.
+size_t sgemm_scratch_bytes(int64_t m, int64_t n, int64_t k, float alpha, int prec);
 hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
                  const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s, int tri = 0,
                  int tri_grid = 0, int krange = 0);

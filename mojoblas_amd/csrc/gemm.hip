@@ -108,8 +108,35 @@ hipError_t launch_dma_whole(bool at, bool bt, GemmParams<T> p, int tri_grid, hip
 // Split-precision path: pack op(A) and op(B)^T into bf16 planes, then run the
 // packed macro-tile kernel.  256x256 tiles (8 waves) when that still gives one
 // tile per CU, else 128x128 (4 waves).
+// Pack planes of the split-precision modes: bytes for BT-row tiles.
+template <int NPROD, int BT>
+size_t packed_bytes(int64_t m, int64_t n, int64_t k) {
+    constexpr int PK = 32;
+    const int64_t tm = (m + BT - 1) / BT, tn = (n + BT - 1) / BT, ktiles = (k + PK - 1) / PK;
+    return (size_t)(tm + tn) * BT * ktiles * PK * ((NPROD == 3) ? 2 : 1) * sizeof(uint16_t);
+}
+
+// e4m3 planes plus one int scale per 128x128 tile of each operand.
+template <int BT>
+size_t fp8_bytes(int64_t m, int64_t n, int64_t k) {
+    constexpr int PK = 64;
+    const int64_t tm = (m + BT - 1) / BT, tn = (n + BT - 1) / BT;
+    const int64_t kblocks = (k + 127) / 128, ktiles = 2 * kblocks;
+    return (size_t)(tm + tn) * BT * ktiles * PK + (size_t)(tm + tn) * (BT / 128) * kblocks * sizeof(int);
+}
+
+// Scratch for the pack planes: the caller's buffer when one is given, else
+// slot 0 of the stream's workspace.
+hipError_t pack_scratch(const Scratch* ext, hipStream_t s, size_t bytes, void** out) {
+    if (!ext) return workspace(s, 0, bytes, out);
+    if (bytes > ext->bytes) return hipErrorOutOfMemory;
+    *out = ext->p;
+    return hipSuccess;
+}
+
 template <int NPROD, int BT_, int WM_, int WN_>
-hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
+hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s,
+                      const Scratch* ext) {
     constexpr int BT = BT_, PK = 32;
     PackedGemmParams p{};
     const int64_t tm = (g.m + BT - 1) / BT, tn = (g.n + BT - 1) / BT;
@@ -119,7 +146,7 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     const int planes = (NPROD == 3) ? 2 : 1;
     const size_t a_el = (size_t)tm * BT * ktiles * PK, b_el = (size_t)tn * BT * ktiles * PK;
     void* ws = nullptr;
-    hipError_t e = workspace(s, 0, (a_el + b_el) * planes * sizeof(uint16_t), &ws);
+    hipError_t e = pack_scratch(ext, s, packed_bytes<NPROD, BT>(g.m, g.n, g.k), &ws);
     if (e != hipSuccess) return e;
     uint16_t* base = (uint16_t*)ws;
     PackParams pa{g.A, g.lda, g.m, g.k, (int)ktiles, base, planes == 2 ? base + a_el : nullptr, g.a_vec};
@@ -167,16 +194,16 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
 // FP8 path: per-128x128-tile scaled e4m3 pack, then the packed kernel in fp8
 // mode (64-deep k-tiles; K padded to a multiple of 128).
 template <int BT_, int WM_, int WN_>
-hipError_t run_fp8(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
+hipError_t run_fp8(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s, const Scratch* ext) {
     constexpr int BT = BT_, PK = 64;
     const int64_t tm = (g.m + BT - 1) / BT, tn = (g.n + BT - 1) / BT;
     const int64_t kblocks = (g.k + 127) / 128, ktiles = 2 * kblocks;
     const int64_t ntile = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
     if (ntile <= 0 || ntile > INT_MAX || ktiles > INT_MAX) return hipErrorInvalidValue;
     const size_t qa = (size_t)tm * BT * ktiles * PK, qb = (size_t)tn * BT * ktiles * PK;
-    const size_t na = (size_t)tm * (BT / 128) * kblocks, nb = (size_t)tn * (BT / 128) * kblocks;
+    const size_t na = (size_t)tm * (BT / 128) * kblocks;
     void* ws = nullptr;
-    hipError_t e = workspace(s, 0, qa + qb + (na + nb) * sizeof(int), &ws);
+    hipError_t e = pack_scratch(ext, s, fp8_bytes<BT>(g.m, g.n, g.k), &ws);
     if (e != hipSuccess) return e;
     uint8_t* base = (uint8_t*)ws;
     int* sa = (int*)(base + qa + qb);
@@ -228,18 +255,25 @@ int64_t tile_count(int64_t m, int64_t n, int BM, int BN, int tri_grid) {
 // A k-range launch (triangular operand) asks for four tiles per CU before it
 // takes the larger tile: its tiles differ in length, and with one tile per CU
 // the longest one would set the time.
-template <int NPROD>
-hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s) {
-    const int64_t t256 = tile_count(g.m, g.n, 256, 256, tri_grid);
-    const int64_t need256 = g.krange ? 1024 : 256;
-    if (NPROD == 8) {
-        if (t256 >= need256) return run_fp8<256, 2, 4>(at, bt, g, tri_grid, s);
-        return run_fp8<128, 2, 2>(at, bt, g, tri_grid, s);
-    }
-    if (t256 >= need256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s);
-    if (tile_count(g.m, g.n, 128, 128, tri_grid) >= 128) return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s);
-    return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s);
+int packed_tile(int nprod, int64_t m, int64_t n, int tri_grid, int krange) {
+    if (tile_count(m, n, 256, 256, tri_grid) >= (krange ? 1024 : 256)) return 256;
+    if (nprod == 8 || tile_count(m, n, 128, 128, tri_grid) >= 128) return 128;
+    return 64;
 }
+
+template <int NPROD>
+hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s,
+                         const Scratch* ext) {
+    const int bt_ = packed_tile(NPROD, g.m, g.n, tri_grid, g.krange);
+    if (NPROD == 8) {
+        if (bt_ == 256) return run_fp8<256, 2, 4>(at, bt, g, tri_grid, s, ext);
+        return run_fp8<128, 2, 2>(at, bt, g, tri_grid, s, ext);
+    }
+    if (bt_ == 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s, ext);
+    if (bt_ == 128) return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s, ext);
+    return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s, ext);
+}
+
 
 // Native 16-bit path.  op(A) (rows m) is k-contiguous in memory iff A is
 // transposed, op(B)^T (rows n) iff B is not; such an operand with K a
@@ -363,6 +397,19 @@ GemmParams<T> make_params(int64_t m, int64_t n, int64_t k, T alpha, const T* A, 
 
 }  // namespace
 
+size_t sgemm_scratch_bytes(int64_t m, int64_t n, int64_t k, float alpha, int prec) {
+    if (m <= 0 || n <= 0 || k <= 0 || alpha == 0.0f) return 0;
+    if (prec != PREC_BF16X3 && prec != PREC_BF16 && prec != PREC_FP8) return 0;
+    const int nprod = prec == PREC_FP8 ? 8 : (prec == PREC_BF16X3 ? 3 : 1);
+    const int t = packed_tile(nprod, m, n, 0, 0);
+    if (nprod == 8) return t == 256 ? fp8_bytes<256>(m, n, k) : fp8_bytes<128>(m, n, k);
+    if (nprod == 3)
+        return t == 256 ? packed_bytes<3, 256>(m, n, k) : t == 128 ? packed_bytes<3, 128>(m, n, k)
+                                                             : packed_bytes<3, 64>(m, n, k);
+    return t == 256 ? packed_bytes<1, 256>(m, n, k) : t == 128 ? packed_bytes<1, 128>(m, n, k)
+                                                         : packed_bytes<1, 64>(m, n, k);
+}
+
 template <typename T>
 hipError_t scale(int64_t m, int64_t n, T beta, T* C, int64_t ldc, hipStream_t s, int tri) {
     if (m <= 0 || n <= 0 || beta == T(1)) return hipSuccess;
@@ -376,7 +423,7 @@ template hipError_t scale<double>(int64_t, int64_t, double, double*, int64_t, hi
 
 hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
                  const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s, int tri,
-                 int tri_grid, int krange) {
+                 int tri_grid, int krange, const Scratch* scratch) {
     if (m <= 0 || n <= 0) return hipSuccess;
     if (alpha == 0.0f || k == 0) return scale<float>(m, n, beta, C, ldc, s, tri);
     GemmParams<float> p = make_params<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, krange);
@@ -392,11 +439,11 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
             }
             return launch_fp<float, float, float, float, 64, 64, 16>(at, bt, p, tri_grid, s);
         case PREC_BF16X3:
-            return launch_packed<3>(at, bt, p, tri_grid, s);
+            return launch_packed<3>(at, bt, p, tri_grid, s, scratch);
         case PREC_BF16:
-            return launch_packed<1>(at, bt, p, tri_grid, s);
+            return launch_packed<1>(at, bt, p, tri_grid, s, scratch);
         case PREC_FP8:
-            return launch_packed<8>(at, bt, p, tri_grid, s);
+            return launch_packed<8>(at, bt, p, tri_grid, s, scratch);
         default:
             return hipErrorInvalidValue;
     }

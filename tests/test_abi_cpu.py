@@ -289,3 +289,14 @@ def test_python_length_check_spares_unread_operands_without_gpu(capsys):
     assert mb.gemm("N", "N", 2, 2, 2, 1.0, np.zeros(1, np.float32), 1, c, 2, 0.0, c, 2) == 8
     assert "gemm: Info 8" in capsys.readouterr().out
     assert (c == 7).all()
+
+
+def test_noalloc_entry_validates_before_any_gpu_work():
+    L = _lib.lib()
+    p = ctypes.c_void_p(0x1000)  # never dereferenced: every case returns first
+    args = lambda m, n, k, lda, prec: (b"N", b"N", m, n, k, 1.0, p, lda, p, k, 0.0, p, max(m, 1), prec, None)  # noqa: E731
+    assert L.mbh_sgemm_dev_noalloc(*args(4, 4, 4, 3, 1)) == 8  # gemm info codes first
+    assert L.mbh_sgemm_dev_noalloc(*args(4, 4, 4, 4, 9)) == -3  # then the precision
+    assert L.mbh_sgemm_dev_noalloc(*args(0, 4, 4, 1, 1)) == 0  # quick return
+    assert L.mbh_sgemm_dev_noalloc(*args(4096, 4096, 4096, 4096, 1)) == E_ARG  # planes beyond the scratch
+    assert "pack scratch" in _lib.last_error()
