@@ -2,6 +2,11 @@
 each with its error against an FP64 gemm of the same inputs.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--variants NN,NT,TN,TT]
+  python bench.py --configs cfg1,cfg2,cfg2b,cfg3a,cfg3b,cfg4,l3
+
+The second form measures the other workloads (mojoblas_amd/bench_configs.py
+prints their GPU lines) and times the restated mojoBLAS CPU path of each
+beside it on the same host, one JSON line per CPU leg.
 
 A step is one SGEMM C := A*B (alpha=1, beta=0, column-major, m=n=k=8192,
 A/B iid N(0,1)) with operands resident in HBM.  The headline value is the
@@ -72,6 +77,148 @@ def errors(torch, C, C64, ABabs):
     }
 
 
+def host_info():
+    """CPU model, the logical CPUs this process may use, and the NUMA layout."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nodes = {}
+    base = "/sys/devices/system/node"
+    try:
+        for d in sorted(os.listdir(base)):
+            if d.startswith("node") and d[4:].isdigit():
+                with open(os.path.join(base, d, "cpulist")) as f:
+                    nodes[d] = f.read().strip()
+    except OSError:
+        pass
+    return {"cpu_model": model, "cpus_usable": len(os.sched_getaffinity(0)), "cpus_online": os.cpu_count(),
+            "numa_nodes": nodes}
+
+
+def _repeat(fn, seconds, min_reps=1):
+    """fn() repeated until `seconds` have passed (at least min_reps times);
+    returns (reps, elapsed seconds), mirroring the reference's harness
+    (benchmarks/benchmark_level3.mojo:29,38: warm-up, then repeat for >= 1 s)."""
+    fn()  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if reps >= min_reps and el >= seconds:
+            return reps, el
+
+
+def _cpu_leg(config, routine, flops_per_rep, reps, el, cores, sample):
+    rec = {"config": config, "routine": routine,
+           "cpu_baseline": {"value": round(reps * flops_per_rep / el / 1e12, 5), "unit": "TFLOP/s",
+                            "cores": cores, "kind": "port", "sample": sample, "seconds": round(el, 2)}}
+    rec["cpu_baseline"].update(host_info())
+    print(json.dumps(rec), flush=True)
+
+
+def cpu_configs(names, seconds):
+    """The restated mojoBLAS CPU path (oracle/baseline.c) of each workload,
+    timed on this host on a bounded sample of the workload (scaled to
+    TFLOP/s): the column sample keeps the per-column work of the full size,
+    and columns are independent in every restated loop nest."""
+    import oracle
+
+    rng = np.random.default_rng(11)
+    thr = oracle.base_threads()
+    f32 = np.float32
+    if "cfg1" in names:
+        n = 512
+        a, b = rng.uniform(-1, 1, n * n).astype(f32), rng.uniform(-1, 1, n * n).astype(f32)
+        c = rng.uniform(-1, 1, n * n).astype(f32)
+        reps, el = _repeat(lambda: oracle.base_sgemm("N", "N", n, n, n, 1.5, a, n, b, n, 0.5, c, n), seconds)
+        _cpu_leg("cfg1", "sgemm NN 512^3 alpha=1.5 beta=0.5 (gemm_v11 blocking)", 2.0 * n ** 3, reps, el, thr,
+                 "%d x the full workload" % reps)
+    if "cfg2" in names:
+        n, cols = 8192, 64
+        a, b = rng.standard_normal(n * n).astype(f32), rng.standard_normal(n * n).astype(f32)
+        c = np.zeros(n * cols, f32)
+        for t in ("NT", "TN", "TT"):
+            reps, el = _repeat(lambda: oracle.base_sgemm(t[0], t[1], n, cols, n, 1.0, a, n, b, n, 0.0, c, n),
+                               seconds / 3)
+            _cpu_leg("cfg2", "sgemm %s 8192^3 (_gemm_naive)" % t, 2.0 * n * n * cols, reps, el, thr,
+                     "%d x the first %d columns of C (m=k=8192): the column-parallel naive sweep does the same "
+                     "work per column as at n=8192" % (reps, cols))
+        del a, b, c
+    if "cfg2b" in names:
+        m, n, k = 4096, 4096, 16384
+        a, b = rng.standard_normal(m * k).astype(f32), rng.standard_normal(k * n).astype(f32)
+        c = np.zeros(m * n, f32)
+        reps, el = _repeat(lambda: oracle.base_sgemm("N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m), seconds)
+        _cpu_leg("cfg2b", "sgemm NN 4096x4096x16384 (gemm_v11 blocking)", 2.0 * m * n * k, reps, el, thr,
+                 "%d x the full workload" % reps)
+        del a, b, c
+    if "cfg3a" in names:
+        m, cols = 4096, 128
+        lo = np.tril(rng.uniform(-1, 1, (m, m)) / m, -1) + np.diag(rng.uniform(1, 2, m))
+        a = np.asfortranarray(lo.astype(f32)).ravel(order="F")
+        b0 = rng.uniform(-1, 1, m * cols).astype(f32)
+        b = b0.copy()
+
+        def run():
+            b[:] = b0
+            oracle.base_strsm_left("L", "N", "N", m, cols, 1.0, a, m, b, m)
+        reps, el = _repeat(run, seconds)
+        _cpu_leg("cfg3a", "strsm L,L,N,N m=4096 (trsm.mojo:110-148, serial)", float(m) * m * cols, reps, el, 1,
+                 "%d x the first %d of the 4096 right-hand sides (B reset each time): the reference's trsm "
+                 "solves column after column" % (reps, cols))
+        del a, b, b0
+    if "cfg3b" in names:
+        n, k = 8192, 2048
+        a = rng.standard_normal(n * k).astype(f32)
+        c = np.zeros(n * n, f32)
+        reps, el = _repeat(lambda: oracle.base_ssyrk_n("L", n, k, 1.0, a, n, 0.0, c, n), seconds)
+        _cpu_leg("cfg3b", "ssyrk L,N n=8192 k=2048 (syrk.mojo:136-166, column-parallel)", float(n) * (n + 1) * k,
+                 reps, el, thr, "%d x the full workload" % reps)
+        del a, c
+    if "cfg4" in names:
+        from mojoblas_amd import batched
+
+        m, n, k = batched.config4_shapes()
+        ops = [(int(mi), int(ni), int(ki), rng.uniform(-1, 1, mi * ki).astype(f32),
+                rng.uniform(-1, 1, ki * ni).astype(f32), np.zeros(mi * ni, f32)) for mi, ni, ki in zip(m, n, k)]
+
+        def run():
+            for mi, ni, ki, a, b, c in ops:
+                oracle.base_sgemm("N", "N", mi, ni, ki, 1.0, a, mi, b, ki, 0.0, c, mi)
+        reps, el = _repeat(run, seconds)
+        _cpu_leg("cfg4", "4096 variable-size NN sgemms, one gemm call each (gemm_v11 blocking)",
+                 float(np.sum(2.0 * m * n * k)), reps, el, thr, "%d x the whole batch" % reps)
+        del ops
+    if "l3" in names:
+        m, cols = 8192, 256
+        s = rng.uniform(-1, 1, (m, m)).astype(f32)
+        a = np.asfortranarray(np.tril(s)).ravel(order="F")
+        b = rng.standard_normal(m * cols).astype(f32)
+        c = np.zeros(m * cols, f32)
+        reps, el = _repeat(lambda: oracle.base_ssymm("L", "L", m, cols, 1.0, a, m, b, m, 0.0, c, m), seconds)
+        _cpu_leg("l3", "ssymm L,L m=8192 (symm.mojo:96-158, column-parallel)", 2.0 * m * m * cols, reps, el, thr,
+                 "%d x the first %d of the 8192 columns of B and C" % (reps, cols))
+        del s, a, b, c
+        m, cols = 4096, 128
+        a = np.asfortranarray(np.tril(rng.uniform(-1, 1, (m, m))).astype(f32)).ravel(order="F")
+        b0 = rng.standard_normal(m * cols).astype(f32)
+        b = b0.copy()
+
+        def run():
+            b[:] = b0
+            oracle.base_strmm("L", "L", "N", "N", m, cols, 1.0, a, m, b, m)
+        reps, el = _repeat(run, seconds)
+        _cpu_leg("l3", "strmm L,L,N,N m=4096 (trmm.mojo:128-147, serial)", float(m) * m * cols, reps, el, 1,
+                 "%d x the first %d of the 4096 columns of B (B reset each time)" % (reps, cols))
+
+
 def cpu_baseline(args):
     """Restated mojoBLAS gemm_v11 (oracle/baseline.c) on the host cores, on the
     same 8192^3 N,N workload, repeated until >= args.cpu_seconds."""
@@ -89,7 +236,7 @@ def cpu_baseline(args):
         if el >= args.cpu_seconds:
             break
     tf = reps * FLOPS / el / 1e12
-    return {
+    rec = {
         "value": round(tf, 4),
         "unit": "TFLOP/s",
         "cores": oracle.base_threads(),
@@ -97,6 +244,8 @@ def cpu_baseline(args):
         "sample": "%d x SGEMM N,N 8192^3 (alpha=1, beta=0) through the restated mojoBLAS gemm_v11 blocking "
                   "(TK=1024, MC=256, 16x16 SIMD outer-product micro-kernel), %.1f s" % (reps, el),
     }
+    rec.update(host_info())
+    return rec
 
 
 def main():
@@ -114,11 +263,32 @@ def main():
                     help="seconds of untimed gemms (every precision timed) before the first timed variant")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device copy (known HBM bytes) to calibrate PMC byte counters")
+    ap.add_argument("--configs", default="",
+                    help="instead of the headline: these workloads (GPU lines from mojoblas_amd.bench_configs, "
+                         "then the restated CPU path of each); cfg2 is CPU-only here (the GPU side of its "
+                         "transposes is in the headline's variants)")
+    ap.add_argument("--reps", type=int, default=10, help="GPU repetitions per workload with --configs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the level-3 path runs on a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.configs:
+        names = [c.strip() for c in args.configs.split(",") if c.strip()]
+        import torch
+
+        import mojoblas_amd as mb
+        from mojoblas_amd import bench_configs
+
+        torch.cuda.set_device(0)
+        for name in names:
+            if name in bench_configs.CONFIGS:
+                log("== %s (GPU)" % name)
+                bench_configs.CONFIGS[name](torch, mb, args.reps)
+                torch.cuda.empty_cache()
+        log("== CPU legs")
+        cpu_configs(set(names), max(1.0, args.cpu_seconds / 4))
+        return
 
     import torch
     import mojoblas_amd as mb
