@@ -64,10 +64,10 @@ using GemmParams = GemmArgs<T, T, T, T>;
 //    and B column-panels share one L2.
 //  * Grouped order inside that run: GROUP tile-rows are walked column by
 //    column, so concurrently resident tiles reuse the same A and B panels.
-__device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
-    int nwg = (int)gridDim.x;
-    int w = (int)blockIdx.x;
-    if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
+__device__ inline int xcd_remap(int w, int nwg) { return (nwg & 7) == 0 ? (w & 7) * (nwg >> 3) + (w >> 3) : w; }
+
+// Grouped order of logical tile id w (GROUP tile-rows walked column by column).
+__device__ inline void tile_coords_at(int w, int tiles_m, int tiles_n, int& tm, int& tn) {
     constexpr int GROUP = 8;
     int per_group = GROUP * tiles_n;
     int group = w / per_group;
@@ -78,14 +78,41 @@ __device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
     tn = in_group / gsize;
 }
 
-// Triangle tile enumeration for syrk-style launches: tile id t -> (r, c) with
-// c <= r, row-major over the lower block triangle.  mode 1 returns (tm, tn) =
-// (r, c) (lower triangle of C), mode 2 returns (c, r) (upper triangle).
-__device__ inline void tri_tile_coords(int mode, int t, int& tm, int& tn) {
-    int r = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-    while ((r + 1) * (r + 2) / 2 <= t) ++r;
-    while (r * (r + 1) / 2 > t) --r;
-    const int c = t - r * (r + 1) / 2;
+__device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+    tile_coords_at(xcd_remap((int)blockIdx.x, (int)gridDim.x), tiles_m, tiles_n, tm, tn);
+}
+
+// Triangle tile enumeration for syrk-style launches over T tile-rows: tile
+// id t -> (r, c) with c <= r over the lower block triangle, in groups of 8
+// tile-rows walked column by column (the tiles before group g are rows
+// 0 .. 8g-1, as in row-major order), so concurrently resident tiles share
+// few row and column panels.  Inside a group of gs rows, the columns left of
+// its diagonal block come first (gs tiles each), then the diagonal block
+// column by column.  mode 1 returns (tm, tn) = (r, c) (lower triangle of C),
+// mode 2 returns (c, r) (upper triangle).
+__host__ __device__ inline void tri_tile_coords(int mode, int t, int T, int& tm, int& tn) {
+    constexpr int G = 8;
+    // row of t in row-major order: rows 0 .. R-1 hold R(R+1)/2 tiles
+    int rr = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((rr + 1) * (rr + 2) / 2 <= t) ++rr;
+    while (rr * (rr + 1) / 2 > t) --rr;
+    const int g0 = (rr / G) * G;                // first row of t's group
+    const int gs = T - g0 < G ? T - g0 : G;     // rows in the group
+    int u = t - g0 * (g0 + 1) / 2;              // index inside the group
+    int r, c;
+    if (u < g0 * gs) {  // columns 0 .. g0-1, gs tiles each
+        c = u / gs;
+        r = g0 + u % gs;
+    } else {  // diagonal block, column-major: column cc has rows cc .. gs-1
+        u -= g0 * gs;
+        int cc = 0;
+        while (u >= gs - cc) {
+            u -= gs - cc;
+            ++cc;
+        }
+        c = g0 + cc;
+        r = g0 + cc + u;
+    }
     if (mode == 2) {
         tm = c;
         tn = r;
@@ -121,6 +148,13 @@ __device__ inline void tile_coords_kr(int krange, int tiles_m, int tiles_n, int&
         case KR_LE_COL: tn = tiles_n - 1 - w / tiles_m; tm = w % tiles_m; return;
         default: tile_coords(tiles_m, tiles_n, tm, tn); return;
     }
+}
+
+// Tile of logical id w of a launch that covers only part of the tile list
+// (no k-range order): grouped order, or the block triangle (tri_tiles 1/2).
+__device__ inline void tile_coords_logical(int tri_tiles, int w, int tiles_m, int tiles_n, int& tm, int& tn) {
+    if (tri_tiles) tri_tile_coords(tri_tiles, w, tiles_m, tm, tn);
+    else tile_coords_at(w, tiles_m, tiles_n, tm, tn);
 }
 
 __device__ inline void kt_range(int mode, int tm, int tn, int BM, int BN, int BK, int nk, int& kt0, int& kt1) {

@@ -48,7 +48,8 @@ hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double
 
 // Device scratch owned by the library: grow-only buffers per (device, stream,
 // slot); slot 0 belongs to the gemm pack pass, slot 1 to the level-3 drivers,
-// slots 2 and 3 to gemv partials and trsv.  The calling thread must hold the
+// slots 2 and 3 to gemv partials and trsv, slot 4 to the split-k partial sums
+// of a gemm's partial last round.  The calling thread must hold the
 // stream's lease (StreamLease) for the whole call; a buffer stays valid until
 // the next request on the same (stream, slot) that asks for more.  Returns
 // hipErrorStreamCaptureUnsupported when a slot would have to grow while the

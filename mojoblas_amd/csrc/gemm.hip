@@ -1,6 +1,9 @@
 // Launchers for the macro-tile gemm kernels and the beta-scale kernel.
 #include <climits>
 
+#include <atomic>
+#include <algorithm>
+
 #include "dispatch.hpp"
 #include "gemm_fp.hpp"
 #include "gemm_packed.hpp"
@@ -134,33 +137,88 @@ hipError_t pack_scratch(const Scratch* ext, hipStream_t s, size_t bytes, void** 
     return hipSuccess;
 }
 
+// Compute units of the current device (cached per device).
+int device_cus() {
+    static std::atomic<int> cache[16];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 0;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) return 0;
+        cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
+// Split-precision path: pack op(A) and op(B)^T into bf16 planes, then run the
+// packed macro-tile kernel.  256x256 tiles (8 waves) when that still gives one
+// tile per CU, else 128x128 (4 waves).
+//  * An operand pair that is one matrix (syrk: op(A) == op(B)^T, same
+//    pointer and ld) is packed once and both sides read the same planes.
+//  * Partial last round: when the tiles leave a last round of workgroups at
+//    most half full (syrk n = 8192: 528 tiles on 256 CUs = two rounds and 16
+//    tiles), the whole rounds run as usual and the remaining tiles are split
+//    along k into up to W / tail chunks of >= 8 k-tiles, one workgroup each,
+//    whose partial sums go to slot-4 workspace and are added in chunk order
+//    by splitk_reduce_kernel (deterministic).  Not for k-range launches or a
+//    caller's scratch.
 template <int NPROD, int BT_, int WM_, int WN_>
 hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s,
                       const Scratch* ext) {
-    constexpr int BT = BT_, PK = 32;
+    constexpr int BT = BT_, PK = 32, NT = 64 * WM_ * WN_;
     PackedGemmParams p{};
     const int64_t tm = (g.m + BT - 1) / BT, tn = (g.n + BT - 1) / BT;
     const int64_t ktiles = (g.k + PK - 1) / PK;
     const int64_t ntile = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
     if (ntile <= 0 || ntile > INT_MAX || ktiles > INT_MAX) return hipErrorInvalidValue;
     const int planes = (NPROD == 3) ? 2 : 1;
-    const size_t a_el = (size_t)tm * BT * ktiles * PK, b_el = (size_t)tn * BT * ktiles * PK;
+    const bool shared = g.A == g.B && g.lda == g.ldb && g.m == g.n && at != bt;
+    const size_t a_el = (size_t)tm * BT * ktiles * PK, b_el = shared ? 0 : (size_t)tn * BT * ktiles * PK;
     void* ws = nullptr;
-    hipError_t e = pack_scratch(ext, s, packed_bytes<NPROD, BT>(g.m, g.n, g.k), &ws);
+    hipError_t e = pack_scratch(ext, s, (a_el + b_el) * planes * sizeof(uint16_t), &ws);
     if (e != hipSuccess) return e;
+    // Partial-round split: resident workgroups per round of this kernel.
+    static std::atomic<int> occ{0};
+    if (occ.load(std::memory_order_relaxed) == 0) {
+        int r = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, gemm_packed_kernel<NPROD, BT, BT, WM_, WN_>, NT, 0) !=
+                hipSuccess || r <= 0)
+            r = -1;
+        occ.store(r, std::memory_order_relaxed);
+    }
+    const int64_t W = (int64_t)device_cus() * std::max(occ.load(std::memory_order_relaxed), 0);
+    int64_t nmain = ntile, tail = 0;
+    int ksplit = 1;
+    float* part = nullptr;
+    if (!ext && g.krange == KR_ALL && W > 0 && ntile > W) {
+        const int64_t t = ntile % W;
+        const int64_t ks = std::min<int64_t>(t > 0 ? W / t : 0, ktiles / 8);
+        if (t > 0 && 2 * t <= W && ks >= 2) {
+            tail = t;
+            ksplit = (int)ks;
+            nmain = ntile - tail;
+            void* pw = nullptr;
+            if ((e = workspace(s, 4, (size_t)tail * ksplit * BT * BT * sizeof(float), &pw)) != hipSuccess) return e;
+            part = (float*)pw;
+        }
+    }
     uint16_t* base = (uint16_t*)ws;
     PackParams pa{g.A, g.lda, g.m, g.k, (int)ktiles, base, planes == 2 ? base + a_el : nullptr, g.a_vec};
     uint16_t* bbase = base + a_el * planes;
     PackParams pb{g.B, g.ldb, g.n, g.k, (int)ktiles, bbase, planes == 2 ? bbase + b_el : nullptr, g.b_vec};
+    if (shared) {
+        pb.hi = pa.hi;
+        pb.lo = pa.lo;
+    }
     TimingMark tmk;
     tmk.active = timing_on();
     timing_mark(tmk, 0, s);
     // One launch packs both operands: 64-row x 128-k blocks, op(A) first.
     // op(A) rows = m: k-contiguous iff A is transposed; op(B)^T rows = n:
-    // k-contiguous iff B is NOT transposed.
+    // k-contiguous iff B is NOT transposed.  A shared pair: op(A)'s blocks only.
     const int64_t kblk = (ktiles + PACK_NKB - 1) / PACK_NKB;
     const int64_t gx_a = tm * BT / 64, gx_b = tn * BT / 64;
-    const int64_t nblk_a = gx_a * kblk, nblk = nblk_a + gx_b * kblk;
+    const int64_t nblk_a = gx_a * kblk, nblk = nblk_a + (shared ? 0 : gx_b * kblk);
     if (nblk > INT_MAX) return hipErrorInvalidValue;
     dim3 pgrid((unsigned)nblk), pblock(256);
     const int ga = (int)gx_a, na = (int)nblk_a, gb = (int)gx_b;
@@ -185,8 +243,20 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     p.tri = g.tri;
     p.tri_tiles = tri_grid;
     p.krange = g.krange;
+    p.tile0 = 0;
+    p.ksplit = 1;
     timing_mark(tmk, 1, s);
-    gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)ntile), dim3(64 * WM_ * WN_), 0, s>>>(p);
+    gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)nmain), dim3(NT), 0, s>>>(p);
+    if (tail > 0) {
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        PackedGemmParams q = p;
+        q.tile0 = (int)nmain;
+        q.ksplit = ksplit;
+        q.part = part;
+        gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)(tail * ksplit)), dim3(NT), 0, s>>>(q);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        splitk_reduce_kernel<BT, BT><<<dim3((unsigned)tail, BT * BT / 1024), dim3(256), 0, s>>>(q);
+    }
     timing_mark(tmk, 2, s);
     return hipGetLastError();
 }

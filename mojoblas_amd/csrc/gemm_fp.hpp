@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs<TA, TB, TC, T> p)
     typedef FpTileLds<T, TA, TB, AT, BT, BM, BN, BK> L;
     __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
     int tm, tn;
-    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, xcd_remap((int)blockIdx.x, (int)gridDim.x), p.tiles_m, tm, tn);
     else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
     gemm_fp_tile<T, TA, TB, TC, AT, BT, BM, BN, BK>(p, tm, tn, lds);
 }
@@ -483,7 +483,7 @@ __device__ inline void gemm_dma_tile(const GemmArgs<T, T, TC, TS>& p, void* lds)
     // Tile enumeration, k-range and store mask as in gemm_fp_kernel
     // (triangle grids for syrk / syr2k, triangular operands for trmm).
     int tm, tn;
-    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, xcd_remap((int)blockIdx.x, (int)gridDim.x), p.tiles_m, tm, tn);
     else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63;

@@ -60,6 +60,14 @@ struct PackedGemmParams {
     // kernels built for an MN-form operand, row-contiguous (k-row k at
     // Ah + k*a_ld, row count a multiple of 8).
     int64_t a_ld, b_ld;
+    // Partial-round split (bf16 modes, fp32 C): a launch with ksplit > 1
+    // covers logical tiles tile0 .. tile0 + gridDim.x / ksplit - 1, workgroup
+    // w taking chunk w % ksplit of tile tile0 + w / ksplit's k-tiles and
+    // storing its raw sum as a BM x BN column-major block at part + w*BM*BN
+    // (splitk_reduce_kernel finishes C).  ksplit == 1: tiles tile0 + (XCD
+    // order of) blockIdx.x, stored to C.
+    int tile0, ksplit;
+    float* part;
 };
 
 typedef int i32x8 __attribute__((ext_vector_type(8)));
@@ -97,7 +105,11 @@ __device__ inline void epilogue_32x32(const f32x16 (&acc)[TM][TN], void* Cgen, i
     typedef typename E::T CT;
     MBH_GLOBAL CT* const C = (MBH_GLOBAL CT*)Cgen;
     const int lrow = lane & 31;
-    const bool interior = tri == TRI_NONE && row0 + 32 * TM <= m && col0 + 32 * TN <= n;
+    // interior: the whole block inside C and, under a triangle mask, inside
+    // the kept triangle (every row >= every column, or <=)
+    const bool inside_tri = tri == TRI_NONE || (tri == TRI_LOWER ? row0 >= col0 + 32 * TN - 1
+                                                                : row0 + 32 * TM - 1 <= col0);
+    const bool interior = inside_tri && row0 + 32 * TM <= m && col0 + 32 * TN <= n;
     if (interior) {
         MBH_GLOBAL CT* cb = C + (row0 + lrow) + col0 * ldc;
         if (beta == 0.0f) {
@@ -622,7 +634,8 @@ __device__ inline void packed_frag_mfma(const PackedFrag<NPL, TM, TN>& f, f32x16
 // banks) over distinct 16-byte bank slots.
 template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N, int CK = 0, bool F16 = false, bool STRIDED = false,
           int BKT = 32, bool AMN = false, bool BMN = false>
-__device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, int tn, u32x4* lds0, u32x4* lds1) {
+__device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, int tn, u32x4* lds0, u32x4* lds1,
+                                             int chunk = 0) {
     static_assert(NPROD == 1 || NPROD == 3, "bf16 modes only");
     static_assert(!F16 || NPROD == 1, "f16 operands use one plane");
     static_assert(!STRIDED || NPROD == 1, "in-place operands hold one plane");
@@ -770,6 +783,14 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
 
     int kt0, kt1;
     kt_range(p.krange, tm, tn, BM, BN, BKT, p.ktiles, kt0, kt1);
+    if constexpr (CK == 0) {
+        if (p.ksplit > 1) {  // this workgroup's share of the tile's k-tiles
+            const int len = kt1 - kt0;
+            const int c0 = kt0 + (int)((int64_t)len * chunk / p.ksplit);
+            kt1 = kt0 + (int)((int64_t)len * (chunk + 1) / p.ksplit);
+            kt0 = c0;
+        }
+    }
     const int lrow = lane & 31, lgrp = lane >> 5, lsw = swz(lrow);
     // Chunks of the k-steps (16 k = two chunks; lanes 32-63 take the second).
     const int ch0 = lgrp ^ lsw, ch1 = (2 + lgrp) ^ lsw, ch2 = (4 + lgrp) ^ lsw, ch3 = (6 + lgrp) ^ lsw;
@@ -821,6 +842,13 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
 #undef MBH_DMA
 
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    if constexpr (CK == 0) {
+        if (p.ksplit > 1) {  // raw partial sum, the whole BM x BN block
+            epilogue_32x32<TM, TN, 0>(acc, p.part + (size_t)blockIdx.x * BM * BN, BM, BM, BN, wm0, wn0, lane, 1.0f,
+                                      0.0f, TRI_NONE);
+            return;
+        }
+    }
     epilogue_32x32<TM, TN, CK>(acc, p.C, p.ldc, p.m, p.n, m0 + wm0, n0 + wn0, lane, p.alpha, p.beta, p.tri);
 }
 
@@ -830,13 +858,52 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(Pac
     typedef PackedTileLds<NPROD, BM, BN> L;
     __shared__ u32x4 lds0[L::TOTAL / 2];
     __shared__ u32x4 lds1[L::TOTAL / 2];
-    int tm, tn;
-    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
-    else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
+    int tm, tn, chunk = 0;
+    if (p.ksplit > 1) {
+        chunk = (int)blockIdx.x % p.ksplit;
+        tile_coords_logical(p.tri_tiles, p.tile0 + (int)blockIdx.x / p.ksplit, p.tiles_m, p.tiles_n, tm, tn);
+    } else if (p.tile0 > 0 || p.tri_tiles) {
+        // part of the tile list, or the block triangle: logical ids in XCD order
+        tile_coords_logical(p.tri_tiles, p.tile0 + xcd_remap((int)blockIdx.x, (int)gridDim.x), p.tiles_m, p.tiles_n,
+                            tm, tn);
+    } else {
+        tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
+    }
     if constexpr (NPROD != 8)
-        gemm_packed_tile_pipe<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
+        gemm_packed_tile_pipe<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1, chunk);
     else
         gemm_packed_tile_dma<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1);
+}
+
+// Finishes the tiles of a split launch: C = alpha * (sum of the ksplit
+// partial blocks, in chunk order) + beta * C on the tile's entries inside C
+// and the triangle mask.  Block (t, y): tail tile t, CPB columns from
+// CPB*y; each thread one column's 4 consecutive rows, read from every
+// partial as one 16-byte load (rows are the contiguous index of a partial
+// block).  Grid (tail tiles, BN / CPB), 256 threads.
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(PackedGemmParams p) {
+    constexpr int TPC = BM / 4, CPB = 256 / TPC;  // threads per column, columns per block
+    static_assert(TPC * CPB == 256 && BN % CPB == 0, "block shape");
+    const int t = (int)blockIdx.x;
+    int tm, tn;
+    tile_coords_logical(p.tri_tiles, p.tile0 + t, p.tiles_m, p.tiles_n, tm, tn);
+    const int i = 4 * ((int)threadIdx.x % TPC), j = CPB * (int)blockIdx.y + (int)threadIdx.x / TPC;
+    const MBH_GLOBAL f32x4* part =
+        (const MBH_GLOBAL f32x4*)(p.part + (size_t)t * p.ksplit * BM * BN + (size_t)j * BM + i);
+    f32x4 sum = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < p.ksplit; ++c) sum += part[(size_t)c * (BM * BN / 4)];
+    MBH_GLOBAL float* C = (MBH_GLOBAL float*)p.C;
+    const int64_t col = (int64_t)tn * BN + j;
+    if (col >= p.n) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t row = (int64_t)tm * BM + i + e;
+        if (row >= p.m || !tri_keep(p.tri, row, col)) continue;
+        float v = p.alpha * sum[e];
+        if (p.beta != 0.0f) v += p.beta * C[row + col * p.ldc];
+        C[row + col * p.ldc] = v;
+    }
 }
 
 // k per packed row of the native 16-bit gemm (its pack pass uses the same).
@@ -853,7 +920,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed16_kernel(P
     __shared__ u32x4 lds0[CH];
     __shared__ u32x4 lds1[CH];
     int tm, tn;
-    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, (int)blockIdx.x, tm, tn);
+    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, xcd_remap((int)blockIdx.x, (int)gridDim.x), p.tiles_m, tm, tn);
     else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
     gemm_packed_tile_pipe<1, BM, BN, WAVES_M, WAVES_N, F16 ? 1 : 2, F16, true, GEMM16_BK, AMN, BMN>(p, tm, tn, lds0,
                                                                                                  lds1);

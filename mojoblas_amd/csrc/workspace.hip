@@ -22,7 +22,7 @@
 namespace mbh {
 
 namespace {
-constexpr int kSlots = 4;
+constexpr int kSlots = 5;
 
 struct Buf {
     void* p = nullptr;
