@@ -24,6 +24,12 @@ struct Scratch {
 hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
                  const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s,
                  int tri = 0, int tri_grid = 0, int krange = 0, const Scratch* scratch = nullptr);
+// syr2k's two products C := alpha*op(A)*op(B) + alpha*op(B)*op(A) + beta*C
+// (n x n, op() = (at, bt) of the first; tri / tri_grid as for sgemm): in the
+// split-precision modes both operands are packed once for the two products.
+hipError_t sgemm_syr2k(bool at, bool bt, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                       const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s, int tri,
+                       int tri_grid);
 // Bytes of pack-plane scratch that sgemm needs for a plain product (no
 // triangle grid or k-range) in precision `prec`: 0 for fp32, alpha == 0 or
 // k == 0.

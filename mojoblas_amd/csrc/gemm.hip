@@ -162,10 +162,75 @@ int device_cus() {
 //    whose partial sums go to slot-4 workspace and are added in chunk order
 //    by splitk_reduce_kernel (deterministic).  Not for k-range launches or a
 //    caller's scratch.
+// Resident workgroups per round of one packed-kernel instantiation on the
+// current device (0 if unknown).
+template <int NPROD, int BT, int WM_, int WN_>
+int64_t packed_round() {
+    static std::atomic<int> occ{0};
+    if (occ.load(std::memory_order_relaxed) == 0) {
+        int r = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, gemm_packed_kernel<NPROD, BT, BT, WM_, WN_>,
+                                                         64 * WM_ * WN_, 0) != hipSuccess || r <= 0)
+            r = -1;
+        occ.store(r, std::memory_order_relaxed);
+    }
+    return (int64_t)device_cus() * std::max(occ.load(std::memory_order_relaxed), 0);
+}
+
+// Launch of the packed main kernel over all tiles of p, with the partial
+// last round split along k (see run_packed) when `split` allows it.
+template <int NPROD, int BT, int WM_, int WN_>
+hipError_t packed_main(PackedGemmParams p, int64_t ntile, bool split, hipStream_t s) {
+    constexpr int NT = 64 * WM_ * WN_;
+    const int64_t W = packed_round<NPROD, BT, WM_, WN_>();
+    int64_t nmain = ntile, tail = 0;
+    int ksplit = 1;
+    hipError_t e;
+    if (split && p.krange == KR_ALL && W > 0 && ntile > W) {
+        const int64_t t = ntile % W;
+        const int64_t ks = std::min<int64_t>(t > 0 ? W / t : 0, p.ktiles / 8);
+        if (t > 0 && 2 * t <= W && ks >= 2) {
+            tail = t;
+            ksplit = (int)ks;
+            nmain = ntile - tail;
+            void* pw = nullptr;
+            if ((e = workspace(s, 4, (size_t)tail * ksplit * BT * BT * sizeof(float), &pw)) != hipSuccess) return e;
+            p.part = (float*)pw;
+        }
+    }
+    p.tile0 = 0;
+    p.ksplit = 1;
+    gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)nmain), dim3(NT), 0, s>>>(p);
+    if (tail > 0) {
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        p.tile0 = (int)nmain;
+        p.ksplit = ksplit;
+        gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)(tail * ksplit)), dim3(NT), 0, s>>>(p);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        splitk_reduce_kernel<BT, BT><<<dim3((unsigned)tail, BT * BT / 1024), dim3(256), 0, s>>>(p);
+    }
+    return hipGetLastError();
+}
+
+// Split-precision path: pack op(A) and op(B)^T into bf16 planes, then run the
+// packed macro-tile kernel.  256x256 tiles (8 waves) when that still gives one
+// tile per CU, else 128x128 (4 waves).
+//  * An operand pair that is one matrix (syrk: op(A) == op(B)^T, same
+//    pointer and ld) is packed once and both sides read the same planes.
+//  * syr2k (pair2 != nullptr): a second product op(B) op(A)^T -- the same
+//    two planes, swapped -- is added into C (beta 1) by a second launch, so
+//    both operands are packed once for the two products.
+//  * Partial last round: when the tiles leave a last round of workgroups at
+//    most half full (syrk n = 8192: 528 tiles on 256 CUs = two rounds and 16
+//    tiles), the whole rounds run as usual and the remaining tiles are split
+//    along k into up to W / tail chunks of >= 8 k-tiles, one workgroup each,
+//    whose partial sums go to slot-4 workspace and are added in chunk order
+//    by splitk_reduce_kernel (deterministic).  Not for k-range launches or a
+//    caller's scratch.
 template <int NPROD, int BT_, int WM_, int WN_>
 hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s,
-                      const Scratch* ext) {
-    constexpr int BT = BT_, PK = 32, NT = 64 * WM_ * WN_;
+                      const Scratch* ext, bool pair2 = false) {
+    constexpr int BT = BT_, PK = 32;
     PackedGemmParams p{};
     const int64_t tm = (g.m + BT - 1) / BT, tn = (g.n + BT - 1) / BT;
     const int64_t ktiles = (g.k + PK - 1) / PK;
@@ -177,31 +242,6 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     void* ws = nullptr;
     hipError_t e = pack_scratch(ext, s, (a_el + b_el) * planes * sizeof(uint16_t), &ws);
     if (e != hipSuccess) return e;
-    // Partial-round split: resident workgroups per round of this kernel.
-    static std::atomic<int> occ{0};
-    if (occ.load(std::memory_order_relaxed) == 0) {
-        int r = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, gemm_packed_kernel<NPROD, BT, BT, WM_, WN_>, NT, 0) !=
-                hipSuccess || r <= 0)
-            r = -1;
-        occ.store(r, std::memory_order_relaxed);
-    }
-    const int64_t W = (int64_t)device_cus() * std::max(occ.load(std::memory_order_relaxed), 0);
-    int64_t nmain = ntile, tail = 0;
-    int ksplit = 1;
-    float* part = nullptr;
-    if (!ext && g.krange == KR_ALL && W > 0 && ntile > W) {
-        const int64_t t = ntile % W;
-        const int64_t ks = std::min<int64_t>(t > 0 ? W / t : 0, ktiles / 8);
-        if (t > 0 && 2 * t <= W && ks >= 2) {
-            tail = t;
-            ksplit = (int)ks;
-            nmain = ntile - tail;
-            void* pw = nullptr;
-            if ((e = workspace(s, 4, (size_t)tail * ksplit * BT * BT * sizeof(float), &pw)) != hipSuccess) return e;
-            part = (float*)pw;
-        }
-    }
     uint16_t* base = (uint16_t*)ws;
     PackParams pa{g.A, g.lda, g.m, g.k, (int)ktiles, base, planes == 2 ? base + a_el : nullptr, g.a_vec};
     uint16_t* bbase = base + a_el * planes;
@@ -243,19 +283,13 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     p.tri = g.tri;
     p.tri_tiles = tri_grid;
     p.krange = g.krange;
-    p.tile0 = 0;
-    p.ksplit = 1;
     timing_mark(tmk, 1, s);
-    gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)nmain), dim3(NT), 0, s>>>(p);
-    if (tail > 0) {
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        PackedGemmParams q = p;
-        q.tile0 = (int)nmain;
-        q.ksplit = ksplit;
-        q.part = part;
-        gemm_packed_kernel<NPROD, BT, BT, WM_, WN_><<<dim3((unsigned)(tail * ksplit)), dim3(NT), 0, s>>>(q);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        splitk_reduce_kernel<BT, BT><<<dim3((unsigned)tail, BT * BT / 1024), dim3(256), 0, s>>>(q);
+    if ((e = packed_main<NPROD, BT, WM_, WN_>(p, ntile, !ext, s)) != hipSuccess) return e;
+    if (pair2) {  // + op(B) op(A)^T from the same planes, added to C
+        std::swap(p.Ah, p.Bh);
+        std::swap(p.Al, p.Bl);
+        p.beta = 1.0f;
+        if ((e = packed_main<NPROD, BT, WM_, WN_>(p, ntile, !ext, s)) != hipSuccess) return e;
     }
     timing_mark(tmk, 2, s);
     return hipGetLastError();
@@ -333,15 +367,16 @@ int packed_tile(int nprod, int64_t m, int64_t n, int tri_grid, int krange) {
 
 template <int NPROD>
 hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s,
-                         const Scratch* ext) {
+                         const Scratch* ext, bool pair2 = false) {
     const int bt_ = packed_tile(NPROD, g.m, g.n, tri_grid, g.krange);
-    if (NPROD == 8) {
+    if constexpr (NPROD == 8) {
         if (bt_ == 256) return run_fp8<256, 2, 4>(at, bt, g, tri_grid, s, ext);
         return run_fp8<128, 2, 2>(at, bt, g, tri_grid, s, ext);
+    } else {
+        if (bt_ == 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s, ext, pair2);
+        if (bt_ == 128) return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s, ext, pair2);
+        return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s, ext, pair2);
     }
-    if (bt_ == 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s, ext);
-    if (bt_ == 128) return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s, ext);
-    return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s, ext);
 }
 
 
@@ -517,6 +552,21 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
         default:
             return hipErrorInvalidValue;
     }
+}
+
+hipError_t sgemm_syr2k(bool at, bool bt, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                       const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s, int tri,
+                       int tri_grid) {
+    if (n <= 0) return hipSuccess;
+    if (alpha == 0.0f || k == 0) return scale<float>(n, n, beta, C, ldc, s, tri);
+    if (prec == PREC_BF16X3 || prec == PREC_BF16) {
+        GemmParams<float> p = make_params<float>(n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, 0);
+        return prec == PREC_BF16X3 ? launch_packed<3>(at, bt, p, tri_grid, s, nullptr, true)
+                                   : launch_packed<1>(at, bt, p, tri_grid, s, nullptr, true);
+    }
+    hipError_t e = sgemm(at, bt, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, s, tri, tri_grid);
+    if (e != hipSuccess) return e;
+    return sgemm(at, bt, n, n, k, alpha, B, ldb, A, lda, 1.0f, C, ldc, prec, s, tri, tri_grid);
 }
 
 hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,

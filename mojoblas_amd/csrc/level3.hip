@@ -488,6 +488,8 @@ hipError_t syr2k(char uplo, char trans, int64_t n, int64_t k, T alpha, const T* 
     const bool tr = !(trans == 'N' || trans == 'n');
     const int tri = upper ? TRI_UPPER : TRI_LOWER, grid = upper ? 2 : 1;
     if (alpha == T(0) || k == 0) return scale<T>(n, n, beta, C, ldc, s, tri);
+    if constexpr (sizeof(T) == 4)
+        return sgemm_syr2k(tr, !tr, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, s, tri, grid);
     MBH_TRY(xgemm<T>(tr, !tr, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, s, tri, grid));
     return xgemm<T>(tr, !tr, n, n, k, alpha, B, ldb, A, lda, T(1), C, ldc, prec, s, tri, grid);
 }
