@@ -78,6 +78,13 @@ int mbh_get_default_precision(void);
  * setting (-1 if none) or MBH_E_PRECISION. */
 int mbh_set_thread_precision(int precision);
 int mbh_device_sync(void);
+/* Test hook for the error paths (no counterpart in the reference): the
+ * (n+1)-th internal device allocation after this call -- workspace growth,
+ * host staging, batched plans -- fails once as if the device were out of
+ * memory (n = 0: the next one); n < 0 turns the hook off.  The failing call
+ * returns MBH_E_HIP or MBH_E_ALLOC before queueing anything and leaves its
+ * outputs untouched.  Returns 0. */
+int mbh_debug_fail_allocation(long n);
 /* Opt-in timing for measurement harnesses (no counterpart in the reference,
  * which has no GPU kernels to time): while enabled, every
  * split-precision (BF16x3 / bf16 / FP8) gemm records HIP events on its launch

@@ -29,6 +29,7 @@ SIGNATURES = {
     "mbh_get_default_precision": ([], i32),
     "mbh_set_thread_precision": ([i32], i32),
     "mbh_device_sync": ([], i32),
+    "mbh_debug_fail_allocation": ([ctypes.c_long], i32),
     "mbh_timing_enable": ([i32], i32),
     "mbh_timing_read": ([vp, i32], i32),
     "mbh_sgemm_host": ([ch, ch, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64], i32),

@@ -206,6 +206,7 @@ hipError_t slot_get(int i, size_t bytes, void** out) {
             sl.p = nullptr;
             sl.cap = 0;
         }
+        if (inject_alloc_failure()) return hipErrorOutOfMemory;
         if ((e = hipMalloc(&sl.p, bytes)) != hipSuccess) return e;
         sl.cap = bytes;
     }
@@ -338,6 +339,11 @@ int mbh_set_thread_precision(int precision) {
     const int old = t_prec;
     t_prec = precision;
     return old;
+}
+
+int mbh_debug_fail_allocation(long n) {
+    inject_alloc_failure_after(n);
+    return 0;
 }
 
 int mbh_device_sync(void) {

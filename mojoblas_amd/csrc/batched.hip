@@ -771,6 +771,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         mbh_vbatch_plan_destroy((mbh_vbatch_plan)plan);
         return hip_fail(err, what);
     };
+    if (inject_alloc_failure()) return fail(hipErrorOutOfMemory, "vbatch plan: counters");
     if ((e = hipMalloc(&plan->ctr, 2 * sizeof(int))) != hipSuccess) return fail(e, "vbatch plan: counters");
     if (count > 0) {
         if ((e = hipMalloc(&plan->probs, sizeof(VbProblem) * probs.size())) != hipSuccess)

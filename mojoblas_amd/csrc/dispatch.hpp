@@ -79,6 +79,13 @@ class StreamLease {
     hipError_t err_ = hipSuccess;
 };
 
+// Test hook for the error paths: the (n+1)-th internal device allocation
+// after this call (workspace growth, host staging, batched plans) fails with
+// hipErrorOutOfMemory, as a real exhausted device would, once; n < 0 turns
+// the hook off.  inject_alloc_failure() is consulted before each of them.
+void inject_alloc_failure_after(long n);
+bool inject_alloc_failure();
+
 // Frees every workspace buffer of a stream (after synchronising it),
 // including those retired while graphs captured on it may still use them.
 hipError_t workspace_release(hipStream_t s);

@@ -11,6 +11,7 @@
 // into a graph is never freed on growth (the graph keeps its address): it is
 // retired and freed by workspace_release.  Growth itself is refused during
 // capture (no allocation or synchronisation may be captured).
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -57,7 +58,15 @@ bool capturing(hipStream_t s) {
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
     return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
 }
+std::atomic<long> g_fail_countdown{-1};
 }  // namespace
+
+void inject_alloc_failure_after(long n) { g_fail_countdown.store(n < 0 ? -1 : n); }
+
+bool inject_alloc_failure() {
+    if (g_fail_countdown.load(std::memory_order_relaxed) < 0) return false;
+    return g_fail_countdown.fetch_sub(1) == 0;  // exactly one failure, then off
+}
 
 StreamLease::StreamLease(hipStream_t s) {
     int dev = 0;
@@ -99,6 +108,7 @@ hipError_t workspace(hipStream_t s, int slot, size_t bytes, void** out) {
             b.captured = false;
         }
         size_t want = bytes + bytes / 8;  // headroom against repeated small growth
+        if (inject_alloc_failure()) return hipErrorOutOfMemory;
         if ((e = hipMalloc(&b.p, want)) != hipSuccess) return e;
         b.cap = want;
     }
