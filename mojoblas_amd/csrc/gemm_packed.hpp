@@ -853,22 +853,27 @@ __device__ inline void gemm_packed_tile_pipe(const PackedGemmParams& p, int tm, 
 }
 
 
-template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
-__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(PackedGemmParams p) {
-    typedef PackedTileLds<NPROD, BM, BN> L;
-    __shared__ u32x4 lds0[L::TOTAL / 2];
-    __shared__ u32x4 lds1[L::TOTAL / 2];
-    int tm, tn, chunk = 0;
+// Tile of this workgroup for the packed launches (see PackedGemmParams).
+__device__ inline void packed_tile_of(const PackedGemmParams& p, int& tm, int& tn, int& chunk) {
+    chunk = 0;
     if (p.ksplit > 1) {
         chunk = (int)blockIdx.x % p.ksplit;
         tile_coords_logical(p.tri_tiles, p.tile0 + (int)blockIdx.x / p.ksplit, p.tiles_m, p.tiles_n, tm, tn);
     } else if (p.tile0 > 0 || p.tri_tiles) {
-        // part of the tile list, or the block triangle: logical ids in XCD order
         tile_coords_logical(p.tri_tiles, p.tile0 + xcd_remap((int)blockIdx.x, (int)gridDim.x), p.tiles_m, p.tiles_n,
                             tm, tn);
     } else {
         tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
     }
+}
+
+template <int NPROD, int BM, int BN, int WAVES_M, int WAVES_N>
+__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_packed_kernel(PackedGemmParams p) {
+    typedef PackedTileLds<NPROD, BM, BN> L;
+    __shared__ u32x4 lds0[L::TOTAL / 2];
+    __shared__ u32x4 lds1[L::TOTAL / 2];
+    int tm, tn, chunk;
+    packed_tile_of(p, tm, tn, chunk);
     if constexpr (NPROD != 8)
         gemm_packed_tile_pipe<NPROD, BM, BN, WAVES_M, WAVES_N>(p, tm, tn, lds0, lds1, chunk);
     else
