@@ -426,12 +426,22 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default=",".join(CONFIGS))
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device twice (the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
     import torch
 
     import mojoblas_amd as mb
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for name in [c.strip() for c in args.only.split(",") if c.strip()]:
         log("== %s" % name)
         CONFIGS[name](torch, mb, args.reps)

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mojoblas_hip.h"
@@ -44,8 +45,15 @@ __host__ __device__ constexpr int vb_tile(int cls) { return cls == 0 ? 64 : 128;
 
 namespace {
 
+// Work queues of the lean path: VB_QUEUES per tile class, one per XCD, each
+// with its own counter on its own 256-byte line (VB_CTR_STRIDE ints apart),
+// so the claims of a class spread over VB_QUEUES addresses instead of
+// serialising on one.  The two-class kernels use counter 0 alone.
+constexpr int VB_QUEUES = 8, VB_CTR_STRIDE = 64;
+constexpr int VB_COUNTERS = 2 * VB_QUEUES;
+
 __global__ void vb_reset_kernel(int* ctr) {
-    if (blockIdx.x == 0 && threadIdx.x < 2) ctr[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < VB_COUNTERS) ctr[threadIdx.x * VB_CTR_STRIDE] = 0;
 }
 
 // fp32 batched tiles: the 64x64 class runs 32-deep k-tiles with prefetch
@@ -83,37 +91,54 @@ __device__ inline void vb_walk(int* ctr, int nitems, int* slot, F&& body) {
 // registers.  The batched operands are column-major with ld = rows, so
 // columns start at any 4-byte offset; the loads are 16-byte loads at dword
 // alignment (which the hardware serves as one vector load) at 32-bit byte
-// offsets from the problem's (uniform) base.  A 16-byte load moves four
-// times the bytes of a dword load per address-unit slot, and T = 128 halves
-// the bytes per flop of T = 64.  The plan takes this path only when every
-// problem's offsets fit in int32 (vb_lean_ok).
+// offsets from the problem's (uniform) base.  The plan takes this path only
+// when every problem's offsets fit in int32 (vb_lean_ok).
 //
-//   A staging: thread t holds rows m0 + 4(t % (T/4)) + e, e < 4 (one 16-byte
-//              load), at k = k0 + t/(T/4) + (1024/T) q, q < T/32;
-//   B staging: thread t holds k = k0 + 4(t%8) + e, e < 4 (one 16-byte load),
-//              in columns n0 + t/8 + 32q, q < T/32.
+//   A staging: load q (< T/32) of a wave covers rows m0 + 32q .. + 31: lane
+//              l holds rows m0 + 32q + 4(l % 8) + e, e < 4 (one 16-byte
+//              load), at k = k0 + 8 wave + kperm(l / 8);
+//   B staging: load q covers columns n0 + 32q .. + 31: lane l holds
+//              k = k0 + 4(l % 8) + e, e < 4, in column n0 + 32q + 8 wave + l / 8.
+// A load whose 32 rows (A) or columns (B) all lie past the problem's edge is
+// not issued and its LDS rows keep stale data; so are the MFMAs and the
+// stores of a wave's 32 x 32 blocks past the edge.  Stale rows and columns,
+// like rows >= m of A and columns >= n of B inside an issued load (which
+// hold other data of the operand), only reach C entries that are never
+// stored.  k >= K is zeroed in both operands.
 // Edges: a group of four that would run past the operand's last element is
 // loaded from its last four instead and shifted back, so every load stays
-// inside the operand.  Rows >= m of A and columns >= n of B only reach C
-// entries that are never stored, so they may hold any data of the operand;
-// k >= K is zeroed in both operands.  Operands of fewer than four elements
-// load single elements.
-// fp32 mode: LDS [k][T + 1] per operand, v_mfma_f32_32x32x2_f32.
-// bf16 mode: the packed swizzled 64-byte-row layout of the split-precision
-//            kernels (2-byte writes for A, 8-byte writes for B), fragments and
-//            MFMAs of packed_ktile_mfma.
+// inside the operand.  Operands of fewer than four elements load single
+// elements.
+//
+// LDS, per buffer (every write and fragment read is free of bank conflicts):
+//   fp32: A [k][T + 4] floats (one 16-byte write of four rows per load), B
+//         [n][36] floats (one 16-byte write of four k per load).  The 2-deep
+//         steps of v_mfma_f32_32x32x2_f32 visit k in the order 8c + 4h + e
+//         (step (c, e), lane half h): one float of A per step and one
+//         16-byte read of B per four steps; A and B use the same order, so
+//         the sum runs over the same products.
+//   bf16: A [k][T] bf16 with 2T + 32 bytes per k-row (one 8-byte write of
+//         four rows per load; the k-rows of lanes 0-7 and 8-15 of a load sit
+//         two rows apart, 64 bytes apart in the banks), fragments by two
+//         ds_read_b64_tr_b16 per 32-row block; B the swizzled 64-byte rows of
+//         the split-precision kernels (8-byte writes of four k, one 16-byte
+//         read per fragment); v_mfma_f32_32x32x16_bf16.
 template <bool BF16, int T>
 struct VbLean {
     static_assert(T == 64 || T == 128, "lean tiles are 64 or 128 wide");
-    static constexpr int S = T + 1;                                  // fp32 row stride (floats)
-    static constexpr int BUF = BF16 ? 2 * T * 64 : 2 * 32 * S * 4;  // bytes per buffer, A + B
+    static constexpr int NQ = T / 32;  // loads per operand per thread
+    static constexpr int TW = T / 64;  // 32x32 blocks per wave and dimension
+    // fp32: row strides in floats; bf16: A k-row stride in bytes.
+    static constexpr int SAF = T + 4, SBF = 36, SAH = 2 * T + 32;
+    static constexpr int A_BYTES = BF16 ? 32 * SAH : 32 * SAF * 4;
+    static constexpr int B_BYTES = BF16 ? T * 64 : T * SBF * 4;
+    static constexpr int BUF = A_BYTES + B_BYTES;  // bytes per buffer
     static constexpr int BYTES = 2 * BUF;
-    static constexpr int NQ = T / 32;          // 16-byte loads per operand per thread
-    static constexpr int TW = T / 64;          // 32x32 blocks per wave and dimension
-    static constexpr int AG = T / 4;           // A row groups
-    static constexpr int KS = 1024 / T;        // k stride between a thread's A loads
     static constexpr int MINWG = T == 64 ? 4 : 2;  // resident workgroups per CU (register bound)
 };
+
+// k-row of a lane group's A load: groups 2p and 2p + 1 two rows apart.
+__device__ inline int vb_kperm(int j) { return (j & 4) | ((j & 1) << 1) | ((j >> 1) & 1); }
 
 // One work item of the lean path (a tile of one problem), read once from the
 // plan; the item index is wave-uniform, so this sits in scalar registers.
@@ -164,49 +189,64 @@ __device__ inline void vb_shift4(const float4& v, int s, float* out) {
     out[3] = v.w;
 }
 
-// Persistent walk over a work queue of T x T items with the k-loop pipelined
-// across items: the last k-step of an item loads the first k-tile of the
-// next item (claimed one item ahead), so a workgroup moves from tile to tile
-// without an exposed load.  Two slots in LDS alternate between items: the
-// slot an item reads was written before the previous item's last barrier,
-// and the other slot is rewritten only after every thread has passed a
-// barrier since reading it (each item passes at least one).  Thread 0 issues
-// the claim when an item starts and publishes it before the item's last
-// barrier.
+// Persistent walk over the work queues of T x T items with the k-loop
+// pipelined across items: the last k-step of an item loads the first k-tile
+// of the next item (claimed one item ahead), so a workgroup moves from tile
+// to tile without an exposed load.  Two slots in LDS alternate between
+// items: the slot an item reads was written before the previous item's last
+// barrier, and the other slot is rewritten only after every thread has
+// passed a barrier since reading it (each item passes at least one).  Thread
+// 0 issues the claim when an item starts and resolves and publishes it
+// before the item's last barrier.
+// Queue x holds items [qoff[x], qoff[x + 1]) (the plan puts all tiles of a
+// problem in one queue, so its operands are fetched into one XCD's L2).  A
+// workgroup serves the queue of its XCD first (blockIdx % 8: workgroups are
+// dispatched to the XCDs round-robin; the mapping only affects locality),
+// then the others in turn once its own is drained, and leaves when all are.
 template <bool BF16, int T>
 __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_kernel(
-    const VbProblem* __restrict__ probs, const VbItem* __restrict__ items, int nitems, int* ctr, float alpha,
-    float beta, const float* const* A, const float* const* B, float* const* C) {
+    const VbProblem* __restrict__ probs, const VbItem* __restrict__ items, int nitems, int* ctr,
+    const int* __restrict__ qoff, float alpha, float beta, const float* const* A, const float* const* B,
+    float* const* C) {
     typedef VbLean<BF16, T> L;
-    constexpr int S = L::S, NQ = L::NQ, TW = L::TW, KS = L::KS;
+    constexpr int NQ = L::NQ, TW = L::TW, SAF = L::SAF, SBF = L::SBF, SAH = L::SAH;
     __shared__ __attribute__((aligned(16))) unsigned char lds[L::BYTES];
     __shared__ int slot[2];
+    __shared__ int qo[VB_QUEUES + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ga = tid % L::AG, ka = tid / L::AG;  // A: row group, k
-    const int gb = tid & 7, cb = tid >> 3;         // B: k group, column
+    const int g8 = lane & 7;                          // A: row group; B: k group
+    const int ka = 8 * wave + vb_kperm(lane >> 3);    // A: k-row of the lane's loads
+    const int cb = tid >> 3;                          // B: column within a 32-column load
     const int wm0 = (wave & 1) * (T / 2), wn0 = (wave >> 1) * (T / 2);
+    const int lr = lane & 31, lh = lane >> 5;
     // Staged k-tile: va[q] holds the four elements of A from linear index
-    // r + k lda - sa[q] (r = m0 + 4ga, k = k0 + ka + KS q), vb[q] those of B
-    // from kk + col ldb - sb[q] (kk = k0 + 4gb, col = n0 + cb + 32q); the
+    // r + k lda - sa[q] (r = m0 + 32q + 4 g8, k = k0 + ka), vb[q] those of B
+    // from kk + col ldb - sb[q] (kk = k0 + 4 g8, col = n0 + cb + 32q); the
     // store shifts them back by sa / sb and zeroes k >= K.  The loads only
     // issue here and are consumed by the store after the k-tile's MFMAs.
+    // Loads q >= qa (A) / qb (B) are not issued (past the tile's edge).
     float4 va[NQ], vb[NQ];
     int sa[NQ], sb[NQ];
+    auto nload = [&](int extent) { return min(NQ, (extent + 31) >> 5); };
     auto load = [&](const VbLeanItem& d, int k0) {
-        const int r = d.m0 + 4 * ga;
-        const int kk = k0 + 4 * gb;
+        const int qa = nload(d.m - d.m0), qb = nload(d.n - d.n0);
+        const int kk = k0 + 4 * g8;
         // Last element of each operand (column-major, the final column
         // holds m (A) or k (B) elements).
         const int alast = (d.K - 1) * d.lda + d.m - 1, blast = (d.n - 1) * d.ldb + d.K - 1;
         if (alast >= 3 && blast >= 3) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                const int ia = r + (k0 + ka + KS * q) * d.lda, ja = min(ia, alast - 3);
-                sa[q] = ia - ja;
-                va[q] = vb_load4(d.A, 4u * (unsigned)ja);
-                const int ib = kk + min(d.n0 + cb + 32 * q, d.n - 1) * d.ldb, jb = min(ib, blast - 3);
-                sb[q] = ib - jb;
-                vb[q] = vb_load4(d.B, 4u * (unsigned)jb);
+                if (q < qa) {
+                    const int ia = d.m0 + 32 * q + 4 * g8 + (k0 + ka) * d.lda, ja = min(ia, alast - 3);
+                    sa[q] = ia - ja;
+                    va[q] = vb_load4(d.A, 4u * (unsigned)ja);
+                }
+                if (q < qb) {
+                    const int ib = kk + min(d.n0 + cb + 32 * q, d.n - 1) * d.ldb, jb = min(ib, blast - 3);
+                    sb[q] = ib - jb;
+                    vb[q] = vb_load4(d.B, 4u * (unsigned)jb);
+                }
             }
         } else {
             // Operands of fewer than four elements: single elements, relaxed
@@ -215,7 +255,7 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
             for (int q = 0; q < NQ; ++q) {
                 sa[q] = 0;
                 sb[q] = 0;
-                const int k = k0 + ka + KS * q, col = d.n0 + cb + 32 * q;
+                const int r = d.m0 + 32 * q + 4 * g8, k = k0 + ka, col = d.n0 + cb + 32 * q;
                 float a4[4], b4[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -229,59 +269,59 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
             }
         }
     };
-    // krem = K - k0 of the staged k-tile.  A k-tile with krem > 32 holds
-    // neither operand's last element, so its groups need no shift and no
-    // k mask.
-    auto store = [&](MBH_LDS unsigned char* buf, int krem) {
+    // krem = K - k0 of the staged k-tile.  In a k-tile with krem > 32 every
+    // k lies below K - 1, so a group of B (four k of one column) or a group
+    // of A holding a row < m (four rows of one column, lda >= 3) ends at or
+    // before the operand's last element: no shift and no k mask.  (With
+    // lda <= 2, four rows from a valid one can run past the last element.)
+    auto store = [&](MBH_LDS unsigned char* buf, const VbLeanItem& d, int krem) {
+        const int qa = nload(d.m - d.m0), qb = nload(d.n - d.n0);
         float ra[4 * NQ], rb[4 * NQ];
-        if (krem > 32) {
+        if (krem > 32 && d.lda >= 3) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 ra[4 * q] = va[q].x, ra[4 * q + 1] = va[q].y, ra[4 * q + 2] = va[q].z, ra[4 * q + 3] = va[q].w;
                 rb[4 * q] = vb[q].x, rb[4 * q + 1] = vb[q].y, rb[4 * q + 2] = vb[q].z, rb[4 * q + 3] = vb[q].w;
             }
         } else {
+            const bool kva = ka < krem;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 float a4[4], b4[4];
                 vb_shift4(va[q], sa[q], a4);
                 vb_shift4(vb[q], sb[q], b4);
-                const bool kva = ka + KS * q < krem;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     ra[4 * q + e] = kva ? a4[e] : 0.0f;
-                    rb[4 * q + e] = 4 * gb + e < krem ? b4[e] : 0.0f;
+                    rb[4 * q + e] = 4 * g8 + e < krem ? b4[e] : 0.0f;
                 }
             }
         }
         if constexpr (BF16) {
-            MBH_LDS uint16_t* a16 = (MBH_LDS uint16_t*)buf;
-            MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + T * 64);
             typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+            MBH_LDS unsigned char* arow = buf + ka * SAH + 8 * g8;
+            MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + L::A_BYTES);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                const int k = ka + KS * q;
-#pragma unroll
-                for (int e = 0; e < 4; e += 2) {
-                    const uint32_t w = bf16x2_rn_bits(ra[4 * q + e], ra[4 * q + e + 1]);
-                    const int r0 = 4 * ga + e, r1 = r0 + 1;
-                    a16[r0 * 32 + (((k >> 3) ^ lds_swz(r0)) << 3) + (k & 7)] = (uint16_t)w;
-                    a16[r1 * 32 + (((k >> 3) ^ lds_swz(r1)) << 3) + (k & 7)] = (uint16_t)(w >> 16);
+                if (q < qa)
+                    *(MBH_LDS u32x2*)(arow + 64 * q) = u32x2{bf16x2_rn_bits(ra[4 * q], ra[4 * q + 1]),
+                                                             bf16x2_rn_bits(ra[4 * q + 2], ra[4 * q + 3])};
+                if (q < qb) {
+                    const int j = cb + 32 * q, k4 = 4 * g8;
+                    *(MBH_LDS u32x2*)(b16 + j * 32 + (((k4 >> 3) ^ lds_swz(j)) << 3) + (k4 & 7)) =
+                        u32x2{bf16x2_rn_bits(rb[4 * q], rb[4 * q + 1]), bf16x2_rn_bits(rb[4 * q + 2], rb[4 * q + 3])};
                 }
-                const int j = cb + 32 * q, k4 = 4 * gb;
-                *(MBH_LDS u32x2*)(b16 + j * 32 + (((k4 >> 3) ^ lds_swz(j)) << 3) + (k4 & 7)) =
-                    u32x2{bf16x2_rn_bits(rb[4 * q], rb[4 * q + 1]), bf16x2_rn_bits(rb[4 * q + 2], rb[4 * q + 3])};
             }
         } else {
-            MBH_LDS float* as = (MBH_LDS float*)buf;
-            MBH_LDS float* bs = as + 32 * S;
+            MBH_LDS float* as = (MBH_LDS float*)buf + ka * SAF + 4 * g8;
+            MBH_LDS float* bs = (MBH_LDS float*)(buf + L::A_BYTES) + cb * SBF + 4 * g8;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    as[(ka + KS * q) * S + 4 * ga + e] = ra[4 * q + e];
-                    bs[(4 * gb + e) * S + cb + 32 * q] = rb[4 * q + e];
-                }
+            for (int q = 0; q < NQ; ++q) {
+                if (q < qa)
+                    *(MBH_LDS f32x4*)(as + 32 * q) = f32x4{ra[4 * q], ra[4 * q + 1], ra[4 * q + 2], ra[4 * q + 3]};
+                if (q < qb)
+                    *(MBH_LDS f32x4*)(bs + 32 * q * SBF) = f32x4{rb[4 * q], rb[4 * q + 1], rb[4 * q + 2], rb[4 * q + 3]};
+            }
         }
     };
     f32x16 acc[TW][TW];
@@ -291,63 +331,117 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
         for (int j = 0; j < TW; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    // nk: k values of the tile below K.  fp32 mode runs only the 2-deep
-    // MFMA steps that hold such k (the rows past K are zero in both
-    // operands, so the sum is unchanged); bf16 mode runs the whole tile.
+    // MFMAs of one k-tile over the wave's blocks below the tile's edge
+    // (bi block-rows, bj block-columns).  nk: k values of the tile below K;
+    // only the MFMA steps that hold such k run (the rows past K are zero in
+    // both operands, so the sum is unchanged).  FULL: every block inside.
     // The scheduling barrier keeps the staging of the next k-tile (which
     // waits for its loads) behind this k-tile's MFMAs.
-    auto compute = [&](const MBH_LDS unsigned char* buf, int nk) {
+    auto compute = [&](const MBH_LDS unsigned char* buf, int nk, auto full, int bi, int bj) {
+        constexpr bool FULL = decltype(full)::value;
+        auto on = [&](int i, int j) { return FULL || (i < bi && j < bj); };
         if constexpr (BF16) {
-            const MBH_LDS u32x4* a4 = (const MBH_LDS u32x4*)buf;
-            const MBH_LDS u32x4* b4 = (const MBH_LDS u32x4*)(buf + T * 64);
-            packed_ktile_mfma<1, TW, TW>(a4, a4, b4, b4, wm0, wn0, lane, 0, 0, acc);
-        } else {
-            const MBH_LDS float* as = (const MBH_LDS float*)buf + (lane >> 5) * S + wm0 + (lane & 31);
-            const MBH_LDS float* bs = (const MBH_LDS float*)buf + 32 * S + (lane >> 5) * S + wn0 + (lane & 31);
-            auto kstep = [&](int kk) {
-                float af[TW], bf[TW];
+            typedef short v4i16 __attribute__((ext_vector_type(4)));
+            // transposing read: lane t of a 16-lane group G reads rows
+            // 16(G & 1) + 4(t & 3) .. + 3 at k-row 8(G >> 1) + (t >> 2)
+            const int G = lane >> 4, t = lane & 15;
+            const MBH_LDS unsigned char* ab =
+                buf + (8 * (G >> 1) + (t >> 2)) * SAH + 2 * (wm0 + 16 * (G & 1) + 4 * (t & 3));
+            const MBH_LDS u32x4* b4 = (const MBH_LDS u32x4*)(buf + L::A_BYTES);
+            const int nks = nk >= 32 ? 2 : (nk + 15) >> 4;
 #pragma unroll
-                for (int i = 0; i < TW; ++i) af[i] = as[kk * S + 32 * i];
+            for (int ks = 0; ks < 2; ++ks) {
+                if (ks >= nks) break;
+                bf16x8 ah[TW], bh[TW];
 #pragma unroll
-                for (int j = 0; j < TW; ++j) bf[j] = bs[kk * S + 32 * j];
+                for (int i = 0; i < TW; ++i)
+                    if (FULL || i < bi) {
+                        const MBH_LDS unsigned char* p = ab + 16 * ks * SAH + 64 * i;
+                        const v4i16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((MBH_LDS v4i16*)p);
+                        const v4i16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((MBH_LDS v4i16*)(p + 4 * SAH));
+                        ah[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+                    }
+#pragma unroll
+                for (int j = 0; j < TW; ++j)
+                    if (FULL || j < bj) {
+                        const int row = wn0 + 32 * j + lr;
+                        bh[j] = __builtin_bit_cast(bf16x8, (u32x4)b4[row * 4 + ((2 * ks + lh) ^ lds_swz(row))]);
+                    }
 #pragma unroll
                 for (int i = 0; i < TW; ++i)
 #pragma unroll
                     for (int j = 0; j < TW; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+                        if (on(i, j)) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+            }
+        } else {
+            const MBH_LDS float* as = (const MBH_LDS float*)buf + 4 * lh * SAF + wm0 + lr;
+            const MBH_LDS float* bs = (const MBH_LDS float*)(buf + L::A_BYTES) + (wn0 + lr) * SBF + 4 * lh;
+            auto kblock = [&](int c) {
+                f32x4 bq[TW];
+#pragma unroll
+                for (int j = 0; j < TW; ++j)
+                    if (FULL || j < bj) bq[j] = *(const MBH_LDS f32x4*)(bs + 32 * j * SBF + 8 * c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float af[TW];
+#pragma unroll
+                    for (int i = 0; i < TW; ++i)
+                        if (FULL || i < bi) af[i] = as[(8 * c + e) * SAF + 32 * i];
+#pragma unroll
+                    for (int i = 0; i < TW; ++i)
+#pragma unroll
+                        for (int j = 0; j < TW; ++j)
+                            if (on(i, j)) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(bq[j][e], af[i], acc[i][j], 0, 0, 0);
+                }
             };
             if (nk >= 32) {
 #pragma unroll
-                for (int kk = 0; kk < 32; kk += 2) kstep(kk);
+                for (int c = 0; c < 4; ++c) kblock(c);
             } else {
-                for (int kk = 0; kk < nk; kk += 2) kstep(kk);
+                for (int c = 0; 8 * c < nk; ++c) kblock(c);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
     };
     // C^T accumulators: register r of block (i, j), lane l is
-    // C(row0 + 32i + l%32, col0 + 32j + acc_col(r, l)).
+    // C(row0 + 32i + l%32, col0 + 32j + acc_col(r, l)).  Blocks past the
+    // tile's edge store nothing; blocks inside it store without masks.
     auto epilogue = [&](const VbLeanItem& d) {
+        const int mr = d.m - d.m0 - wm0, nr = d.n - d.n0 - wn0;  // rows / columns left for the wave
 #pragma unroll
         for (int i = 0; i < TW; ++i) {
-            const int row = d.m0 + wm0 + 32 * i + (lane & 31);
-            const bool rok = row < d.m;
+            if (32 * i >= mr) continue;
+            const int row = d.m0 + wm0 + 32 * i + lr;
+            const bool rok = lr < mr - 32 * i;
 #pragma unroll
             for (int j = 0; j < TW; ++j) {
+                if (32 * j >= nr) continue;
                 const int col0 = d.n0 + wn0 + 32 * j;
-                if (beta == 0.0f) {
+                MBH_GLOBAL float* cp = d.C + (row + col0 * d.ldc);
+                if (32 * i + 32 <= mr && 32 * j + 32 <= nr) {
+                    if (beta == 0.0f) {
+#pragma unroll
+                        for (int r = 0; r < 16; ++r) cp[acc_col<32>(r, lane) * d.ldc] = alpha * acc[i][j][r];
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 16; ++r) {
+                            MBH_GLOBAL float* c = cp + acc_col<32>(r, lane) * d.ldc;
+                            *c = alpha * acc[i][j][r] + beta * *c;
+                        }
+                    }
+                } else if (beta == 0.0f) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const int col = col0 + acc_col<32>(r, lane);
-                        if (rok && col < d.n) d.C[row + col * d.ldc] = alpha * acc[i][j][r];
+                        const int cc = acc_col<32>(r, lane);
+                        if (rok && col0 + cc < d.n) cp[cc * d.ldc] = alpha * acc[i][j][r];
                     }
                 } else {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const int col = col0 + acc_col<32>(r, lane);
-                        if (rok && col < d.n) {
-                            MBH_GLOBAL float* cp = d.C + (row + col * d.ldc);
-                            *cp = alpha * acc[i][j][r] + beta * *cp;
+                        const int cc = acc_col<32>(r, lane);
+                        if (rok && col0 + cc < d.n) {
+                            MBH_GLOBAL float* c = cp + cc * d.ldc;
+                            *c = alpha * acc[i][j][r] + beta * *c;
                         }
                     }
                 }
@@ -356,10 +450,35 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
     };
     MBH_LDS unsigned char* const L0 = (MBH_LDS unsigned char*)lds;
     auto lbuf = [&](int b) { return L0 + b * L::BUF; };
+    // Blocks of the wave inside the current item's edge, and the k-tile's
+    // MFMAs with the right form.
+    auto run = [&](const VbLeanItem& d, const MBH_LDS unsigned char* buf, int nk) {
+        const int bi = min(TW, max(0, (d.m - d.m0 - wm0 + 31) >> 5));
+        const int bj = min(TW, max(0, (d.n - d.n0 - wn0 + 31) >> 5));
+        if (bi == TW && bj == TW)
+            compute(buf, nk, std::true_type(), bi, bj);
+        else if (bi > 0 && bj > 0)
+            compute(buf, nk, std::false_type(), bi, bj);
+    };
 
+    // Claims (thread 0): qs = queues drained so far, counting from the own.
+    const int home = blockIdx.x % VB_QUEUES;
+    int qs = 0;
+    auto issue = [&]() { return qs < VB_QUEUES ? atomicAdd(ctr + ((home + qs) % VB_QUEUES) * VB_CTR_STRIDE, 1) : 0; };
+    auto resolve = [&](int raw) {
+        for (;;) {
+            if (qs >= VB_QUEUES) return nitems;
+            const int x = (home + qs) % VB_QUEUES;
+            if (raw < qo[x + 1] - qo[x]) return qo[x] + raw;
+            ++qs;
+            raw = issue();
+        }
+    };
+    if (tid <= VB_QUEUES) qo[tid] = qoff[tid];
+    __syncthreads();
     if (tid == 0) {
-        slot[0] = atomicAdd(ctr, 1);
-        slot[1] = atomicAdd(ctr, 1);
+        slot[0] = resolve(issue());
+        slot[1] = resolve(issue());
     }
     __syncthreads();
     const int it0 = __builtin_amdgcn_readfirstlane(slot[0]);
@@ -368,7 +487,7 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
     int sel = 1, buf = 0;
     if (cur.nkt > 0) {
         load(cur, 0);
-        store(lbuf(0), cur.K);
+        store(lbuf(0), cur, cur.K);
     }
     __syncthreads();
     for (;;) {
@@ -376,7 +495,7 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
         const int itn = __builtin_amdgcn_readfirstlane(slot[sel]);
         const bool has_next = itn < nitems;
         int claim = 0;
-        if (tid == 0 && has_next) claim = atomicAdd(ctr, 1);
+        if (tid == 0 && has_next) claim = issue();
         VbLeanItem nx = cur;
         if (has_next) nx = vb_lean_item<T>(probs, items, itn, A, B, C);
         const bool pre = has_next && nx.nkt > 0;
@@ -384,16 +503,16 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
         // item's first k-tile instead (k = 0: C := beta C, no k-steps).
         for (int kt = 0; kt + 1 < cur.nkt; ++kt) {
             load(cur, (kt + 1) * 32);
-            compute(lbuf(buf), 32);
-            store(lbuf(buf ^ 1), cur.K - (kt + 1) * 32);
+            run(cur, lbuf(buf), 32);
+            store(lbuf(buf ^ 1), cur, cur.K - (kt + 1) * 32);
             __syncthreads();
             buf ^= 1;
         }
         if (pre) load(nx, 0);
-        if (cur.nkt > 0) compute(lbuf(buf), cur.K - (cur.nkt - 1) * 32);
+        if (cur.nkt > 0) run(cur, lbuf(buf), cur.K - (cur.nkt - 1) * 32);
         const int nbuf = cur.nkt > 0 ? buf ^ 1 : buf;
-        if (pre) store(lbuf(nbuf), nx.K);
-        if (tid == 0 && has_next) slot[sel ^ 1] = claim;
+        if (pre) store(lbuf(nbuf), nx, nx.K);
+        if (tid == 0 && has_next) slot[sel ^ 1] = resolve(claim);
         __syncthreads();
         buf = nbuf;
         epilogue(cur);
@@ -606,10 +725,11 @@ struct VbPlan {
     VbProblem* probs = nullptr;
     VbItem* items = nullptr;
     int* ctr = nullptr;
+    int* qoff = nullptr;  // lean path: per class, VB_QUEUES + 1 queue offsets
     int64_t class_count[2] = {0, 0};
     bool lean = true;  // every problem fits the lean tiles' int32 offsets
-    // Lean path: items [0, nitems1) are 128x128 tiles (queue ctr[0], grid
-    // grid1), the rest 64x64 tiles (queue ctr[1], grid).
+    // Lean path: items [0, nitems1) are 128x128 tiles (counters and queue
+    // offsets of class 0, grid grid1), the rest 64x64 tiles (class 1, grid).
     int nitems1 = 0, grid1 = 0;
 };
 
@@ -729,13 +849,52 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         if (probs[(size_t)p].m > 0 && probs[(size_t)p].n > 0)
             for (int64_t t = 0; t < tiles_of(p); ++t)
                 queues[use_lean && probs[(size_t)p].cls == 0 ? 1 : 0].push_back({(int)p, (int)t, tile_cost(p)});
-    // Longest-first (the tiles of a problem stay adjacent): the dynamic
-    // queue then finishes with the short items.
+    // Longest-first (the tiles of a problem stay adjacent): a dynamic queue
+    // then finishes with the short items.  The lean path splits each class
+    // into VB_QUEUES queues of whole problems, longest-processing-time first
+    // onto the least loaded queue (the cost of a tile counts about two
+    // k-tiles of fixed work: claim, descriptor, epilogue); plan->qoff holds
+    // each queue's first item, relative to its class.
+    auto longest_first = [](std::vector<Item>& v) {
+        std::stable_sort(v.begin(), v.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+    };
     std::vector<VbItem> ditems;
     ditems.reserve((size_t)ntiles);
-    for (int q = 0; q < 2; ++q) {
-        std::stable_sort(queues[q].begin(), queues[q].end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-        for (const Item& it : queues[q]) ditems.push_back({it.prob, it.tile});
+    std::vector<int> qoff(2 * (VB_QUEUES + 1), 0);
+    for (int c = 0; c < 2; ++c) {
+        if (!use_lean) {
+            longest_first(queues[c]);
+            for (const Item& it : queues[c]) ditems.push_back({it.prob, it.tile});
+            continue;
+        }
+        struct Job {
+            int prob;
+            int64_t tiles;
+            double cost;
+        };
+        std::vector<Job> jobs;
+        for (size_t i = 0; i < queues[c].size();) {  // the tiles of a problem are consecutive
+            const int p = queues[c][i].prob;
+            const int T = vb_tile(probs[(size_t)p].cls);
+            const int64_t nt = tiles_of(p);
+            jobs.push_back({p, nt, (double)nt * (queues[c][i].cost + 64.0 * T * T)});
+            i += (size_t)nt;
+        }
+        std::stable_sort(jobs.begin(), jobs.end(), [](const Job& a, const Job& b) { return a.cost > b.cost; });
+        std::vector<Item> sub[VB_QUEUES];
+        double load[VB_QUEUES] = {};
+        for (const Job& j : jobs) {
+            const int q = (int)(std::min_element(load, load + VB_QUEUES) - load);
+            load[q] += j.cost;
+            for (int64_t t = 0; t < j.tiles; ++t) sub[q].push_back({j.prob, (int)t, tile_cost(j.prob)});
+        }
+        const int base = (int)ditems.size();
+        for (int q = 0; q < VB_QUEUES; ++q) {
+            qoff[c * (VB_QUEUES + 1) + q] = (int)ditems.size() - base;
+            longest_first(sub[q]);
+            for (const Item& it : sub[q]) ditems.push_back({it.prob, it.tile});
+        }
+        qoff[c * (VB_QUEUES + 1) + VB_QUEUES] = (int)ditems.size() - base;
     }
     plan->nitems = (int)ditems.size();
     plan->nitems1 = use_lean ? (int)queues[0].size() : 0;
@@ -772,7 +931,11 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         return hip_fail(err, what);
     };
     if (inject_alloc_failure()) return fail(hipErrorOutOfMemory, "vbatch plan: counters");
-    if ((e = hipMalloc(&plan->ctr, 2 * sizeof(int))) != hipSuccess) return fail(e, "vbatch plan: counters");
+    if ((e = hipMalloc(&plan->ctr, VB_COUNTERS * VB_CTR_STRIDE * sizeof(int))) != hipSuccess)
+        return fail(e, "vbatch plan: counters");
+    if ((e = hipMalloc(&plan->qoff, qoff.size() * sizeof(int))) != hipSuccess) return fail(e, "vbatch plan: queues");
+    if ((e = hipMemcpy(plan->qoff, qoff.data(), qoff.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "vbatch plan: queues copy");
     if (count > 0) {
         if ((e = hipMalloc(&plan->probs, sizeof(VbProblem) * probs.size())) != hipSuccess)
             return fail(e, "vbatch plan: problems");
@@ -832,20 +995,22 @@ int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* co
         // 128x128 items first (the longer ones), then the 64x64 queue.
         const bool bf = plan->prec == MBH_PREC_BF16;
         const int n1 = plan->nitems1, n0 = plan->nitems - plan->nitems1;
+        int* ctr1 = plan->ctr + VB_QUEUES * VB_CTR_STRIDE;
+        const int* qoff1 = plan->qoff + VB_QUEUES + 1;
         if (n1 > 0) {
             if (bf)
                 vb_gemm_lean_kernel<true, 128><<<dim3((unsigned)plan->grid1), block, 0, s>>>(
-                    plan->probs, plan->items, n1, plan->ctr, alpha, beta, a, b, c);
+                    plan->probs, plan->items, n1, plan->ctr, plan->qoff, alpha, beta, a, b, c);
             else
                 vb_gemm_lean_kernel<false, 128><<<dim3((unsigned)plan->grid1), block, 0, s>>>(
-                    plan->probs, plan->items, n1, plan->ctr, alpha, beta, a, b, c);
+                    plan->probs, plan->items, n1, plan->ctr, plan->qoff, alpha, beta, a, b, c);
         }
         if (n0 > 0) {
             if (bf)
-                vb_gemm_lean_kernel<true, 64><<<grid, block, 0, s>>>(plan->probs, plan->items + n1, n0, plan->ctr + 1,
+                vb_gemm_lean_kernel<true, 64><<<grid, block, 0, s>>>(plan->probs, plan->items + n1, n0, ctr1, qoff1,
                                                                      alpha, beta, a, b, c);
             else
-                vb_gemm_lean_kernel<false, 64><<<grid, block, 0, s>>>(plan->probs, plan->items + n1, n0, plan->ctr + 1,
+                vb_gemm_lean_kernel<false, 64><<<grid, block, 0, s>>>(plan->probs, plan->items + n1, n0, ctr1, qoff1,
                                                                       alpha, beta, a, b, c);
         }
     } else if (plan->prec == MBH_PREC_BF16) {
@@ -875,6 +1040,7 @@ int mbh_vbatch_plan_destroy(mbh_vbatch_plan handle) {
     if (plan->probs) (void)hipFree(plan->probs);
     if (plan->items) (void)hipFree(plan->items);
     if (plan->ctr) (void)hipFree(plan->ctr);
+    if (plan->qoff) (void)hipFree(plan->qoff);
     delete plan;
     return e == hipSuccess ? 0 : hip_fail(e, "mbh_vbatch_plan_destroy");
 }

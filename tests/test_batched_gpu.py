@@ -96,6 +96,35 @@ def test_ragged_batch_vs_oracle(prec, table):
     plan.close()
 
 
+@pytest.mark.parametrize("prec,table", _cases(("all64", "all128")), indirect=["table"])
+def test_tile_edges(prec, table):
+    """Every m and n around the 32-row blocks and the 64 / 128 tiles (a load,
+    MFMA block or store is skipped when it lies wholly past the edge), k
+    around the 8- and 16-deep steps and the 32-deep k-tiles, operands of
+    fewer than four elements; each problem checked in full."""
+    edges = [1, 2, 3, 4, 5, 31, 32, 33, 63, 64, 65, 96, 97, 127, 128, 129, 160, 191, 192, 255, 257]
+    ks = [0, 1, 2, 3, 7, 8, 9, 15, 16, 17, 31, 32, 33, 40, 63, 64, 65, 100]
+    rng = np.random.default_rng(44)
+    m = np.array([edges[i % len(edges)] for i in range(160)], np.int64)
+    n = np.array([edges[(7 * i + 3) % len(edges)] for i in range(160)], np.int64)
+    k = np.array([ks[(5 * i + 1) % len(ks)] for i in range(160)], np.int64)
+    rng.shuffle(k)
+    g = torch.Generator(device="cuda").manual_seed(45)
+    bufs, offs, ptrs = batched.pack_problems(torch, m, n, k, gen=g, fill="normal")
+    bufs[2].uniform_(-1, 1, generator=g)
+    c0 = bufs[2].cpu().numpy().copy()
+    plan = batched.VBatchPlan(m, n, k, precision=prec)
+    plan.exec(0.75, ptrs[0], ptrs[1], 1.5, ptrs[2])
+    torch.cuda.synchronize()
+    _check_problems(prec, m, n, k, bufs, offs, 0.75, 1.5, c0, range(len(m)))
+    # beta == 0 never reads C (NaN stays out of the result)
+    bufs[2].fill_(float("nan"))
+    plan.exec(0.75, ptrs[0], ptrs[1], 0.0, ptrs[2])
+    torch.cuda.synchronize()
+    _check_problems(prec, m, n, k, bufs, offs, 0.75, 0.0, np.zeros_like(c0), range(len(m)))
+    plan.close()
+
+
 @pytest.mark.parametrize("prec,table", _cases(("default", "all64", "mixed")), indirect=["table"])
 def test_config4_4096_problems(prec, table):
     """Config size: 4096 N,N problems, m, n, k log-uniform in [16, 512],
