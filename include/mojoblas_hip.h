@@ -266,12 +266,15 @@ int mbh_trsv_status(void* stream, int* failed);
 /* ---- variable-size batched gemm (no counterpart in the reference) --------
  * count independent problems C_p := alpha*A_p*B_p + beta*C_p (each with the
  * semantics of gemm("N","N", m_p, n_p, k_p, ...), gemm.mojo:56-124), run as
- * persistent kernels that pull C tiles from a work queue per tile class
- * (128x128 tiles first, then 64x64).
+ * persistent kernels that pull C tiles from work queues (128x128 tiles
+ * first, then 64x64; per tile class one queue per XCD holding whole
+ * problems, drained by that XCD's workgroups first, then by any).
  *  plan_create : host arrays of shapes and leading dimensions; validates every
  *                problem (MBH_E_ARG names the first bad one), assigns tile
- *                classes from the tuning table, orders the tiles longest-first
- *                and stores the plan in HBM (allocates; once per shape list).
+ *                classes from the tuning table, splits the problems over the
+ *                queues by longest-processing-time, orders each queue's tiles
+ *                longest-first and stores the plan in HBM (allocates; once
+ *                per shape list).
  *  exec        : device arrays of count device pointers; allocates nothing and
  *                never synchronises the host, so it can be captured in a
  *                hipGraph and replayed.  precision MBH_PREC_FP32 (fp32 MFMA) or

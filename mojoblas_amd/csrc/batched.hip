@@ -418,31 +418,34 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
                 if (32 * j >= nr) continue;
                 const int col0 = d.n0 + wn0 + 32 * j;
                 MBH_GLOBAL float* cp = d.C + (row + col0 * d.ldc);
-                if (32 * i + 32 <= mr && 32 * j + 32 <= nr) {
-                    if (beta == 0.0f) {
+                const bool inside = 32 * i + 32 <= mr && 32 * j + 32 <= nr;
+                if (beta == 0.0f) {
+                    if (inside) {
 #pragma unroll
                         for (int r = 0; r < 16; ++r) cp[acc_col<32>(r, lane) * d.ldc] = alpha * acc[i][j][r];
                     } else {
 #pragma unroll
                         for (int r = 0; r < 16; ++r) {
-                            MBH_GLOBAL float* c = cp + acc_col<32>(r, lane) * d.ldc;
-                            *c = alpha * acc[i][j][r] + beta * *c;
+                            const int cc = acc_col<32>(r, lane);
+                            if (rok && col0 + cc < d.n) cp[cc * d.ldc] = alpha * acc[i][j][r];
                         }
-                    }
-                } else if (beta == 0.0f) {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int cc = acc_col<32>(r, lane);
-                        if (rok && col0 + cc < d.n) cp[cc * d.ldc] = alpha * acc[i][j][r];
                     }
                 } else {
+                    // C in quarters of four, each loaded whole (lanes past
+                    // the edge read C(m0, n0) instead) before its results are
+                    // formed and stored: one wait per quarter, not per element
+                    auto ok = [&](int r) { return inside || (rok && col0 + acc_col<32>(r, lane) < d.n); };
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int cc = acc_col<32>(r, lane);
-                        if (rok && col0 + cc < d.n) {
-                            MBH_GLOBAL float* c = cp + cc * d.ldc;
-                            *c = alpha * acc[i][j][r] + beta * *c;
-                        }
+                    for (int h = 0; h < 16; h += 4) {
+                        float cv[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            cv[r] = *(ok(h + r) ? cp + acc_col<32>(h + r, lane) * d.ldc : d.C + (d.m0 + d.n0 * d.ldc));
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) cv[r] = alpha * acc[i][j][h + r] + beta * cv[r];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (ok(h + r)) cp[acc_col<32>(h + r, lane) * d.ldc] = cv[r];
                     }
                 }
             }
