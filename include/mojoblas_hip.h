@@ -156,11 +156,12 @@ int mbh_bgemm_dev(char transa, char transb, int64_t m, int64_t n, int64_t k, flo
  * Replaces trsm[dtype] (src/level3/trsm.mojo:20-275; info codes :62-89:
  * 1,2,3,4,5,6,9,11).  BLAS semantics, including side R with trans T (the
  * reference solves the untransposed system there, trsm.mojo:223-273).
- * fp32: `precision` selects the gemm of the off-diagonal updates; the 128x128
- * diagonal blocks are inverted and applied in fp64; refine = 1 adds one step
- * of iterative refinement with an fp64 residual (0 = none), whose correction
- * solve runs its updates as single bf16 products.  The _host entry uses the
- * default precision and refine = 1. */
+ * fp32: the 128x128 diagonal blocks are inverted and applied in fp64.
+ * refine = 0: B is solved in place in fp32, `precision` selecting the gemm of
+ * the off-diagonal updates.  refine = 1 (fp32 data to fp32 accuracy): the
+ * solve runs on fp64 copies of alpha*B and of A's triangle with fp64 updates
+ * and X is rounded to fp32 once (`precision` has no effect there).  The _host
+ * entry uses refine = 1. */
 int mbh_strsm_host(char side, char uplo, char transa, char diag, int64_t m, int64_t n, float alpha,
                    const float* a, int64_t lda, float* b, int64_t ldb);
 int mbh_dtrsm_host(char side, char uplo, char transa, char diag, int64_t m, int64_t n, double alpha,
@@ -315,8 +316,8 @@ int mbh_sgemm_vbatched_dev(int64_t count, const int64_t* m, const int64_t* n, co
  *    (side R) and upstream dX: dB = alpha*G with G = op(A)^-T dX (L) or
  *    dX op(A)^-T (R); dA (optional) = -G X^T (L) or -X^T G (R) as a gradient
  *    of op(A), returned for A as stored: only its `uplo` triangle is nonzero
- *    and its diagonal is zero when diag is U.  The solve for G runs with one
- *    fp64-residual refinement step. */
+ *    and its diagonal is zero when diag is U.  The solve for G runs in fp64
+ *    (refine = 1 above). */
 int mbh_sgemm_backward(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
                        const float* a, int64_t lda, const float* b, int64_t ldb, const float* dc,
                        int64_t lddc, float* da, int64_t ldda, float* db, int64_t lddb, int precision,

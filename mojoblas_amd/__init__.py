@@ -251,10 +251,11 @@ def trsm(side, uplo, trans_a, diag, m, n, alpha, a, lda, b, ldb, precision=None,
     """Solve op(A) X = alpha B (side L) or X op(A) = alpha B (side R); X
     overwrites B (src/level3/trsm.mojo:20-275, BLAS semantics).
 
-    float32: `precision` ("fp32" default, "bf16x3") selects the gemm of the
-    off-diagonal updates; diagonal 128x128 blocks are solved in fp64;
-    refine=1 adds one fp64-residual refinement step (device entry; the host
-    entry always refines once).
+    float32: diagonal 128x128 blocks are solved in fp64.  refine=1 (default;
+    the host entry always uses it) runs the whole solve on fp64 copies of
+    alpha*B and A's triangle and rounds X to fp32 once; refine=0 solves in
+    fp32 in place, `precision` ("fp32" default, "bf16x3") selecting the gemm
+    of the off-diagonal updates.
     """
     ptrs = [_p(a), lda, _p(b), ldb]
     return _call("trsm", [_flag(side), _flag(uplo), _flag(trans_a), _flag(diag), m, n, alpha],

@@ -44,14 +44,6 @@ hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha
 hipError_t gemm16(bool f16, bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* A,
                   int64_t lda, const uint16_t* B, int64_t ldb, float beta, uint16_t* C, int64_t ldc, hipStream_t s);
 
-// fp64-MFMA gemm over mixed storage types (fp32/fp64 operands, fp32/fp64 C),
-// all arithmetic in fp64.  shape 0: 64x64 tiles; 1: 128x64 (a <=128-row C is
-// one tile row, so C may alias B); 2: 64x128 (C may alias A when n <= 128).
-template <typename TA, typename TB, typename TC>
-hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const TA* A, int64_t lda,
-                       const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, hipStream_t s, int shape,
-                       int krange = 0);
-
 // Device scratch owned by the library: grow-only buffers per (device, stream,
 // slot); slot 0 belongs to the gemm pack pass, slot 1 to the level-3 drivers,
 // slots 2 and 3 to gemv partials and trsv, slot 4 to the split-k partial sums

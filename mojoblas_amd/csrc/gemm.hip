@@ -586,52 +586,6 @@ hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha
     return launch_fp<double, double, double, double, 64, 64, 16>(at, bt, p, tri_grid, s);
 }
 
-template <typename TA, typename TB, typename TC>
-hipError_t gemm_f64acc(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const TA* A, int64_t lda,
-                       const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, hipStream_t s, int shape,
-                       int krange) {
-    if (m <= 0 || n <= 0) return hipSuccess;
-    if (alpha == 0.0 || k == 0) return hipErrorInvalidValue;  // callers never ask for a pure scale here
-    GemmArgs<TA, TB, TC, double> p{};
-    p.m = m;
-    p.n = n;
-    p.k = k;
-    p.A = A;
-    p.lda = lda;
-    p.B = B;
-    p.ldb = ldb;
-    p.C = C;
-    p.ldc = ldc;
-    p.alpha = alpha;
-    p.beta = beta;
-    p.krange = krange;
-    p.a_vec = vec_ok(A, lda);
-    p.b_vec = vec_ok(B, ldb);
-    // 1 / 2: one 128-row tile row / 128-column tile column (in-place leaf
-    // products of the triangular solves), 32-deep k-tiles.  0: 64 x 64 with
-    // 32-deep k-tiles, which measured faster than 64 x 64 x 16, 128 x 64 and
-    // 128 x 128 for the 4096^2 refinement residual (the fp64 dgemm keeps
-    // 64 x 64 x 16, faster there).
-    if (shape == 1) return launch_fp<double, TA, TB, TC, 128, 32, 32>(at, bt, p, 0, s);
-    if (shape == 2) return launch_fp<double, TA, TB, TC, 32, 128, 32>(at, bt, p, 0, s);
-    // fp32 operands and fp64 C in whole 128 x 128 tiles, enough of them:
-    // the direct-to-LDS tile with fp64 products.
-    if constexpr (sizeof(TA) == 4 && sizeof(TB) == 4 && sizeof(TC) == 8) {
-        if (m % 128 == 0 && n % 128 == 0 && k % 32 == 0 && p.a_vec && p.b_vec &&
-            tile_count(m, n, 128, 128, 0) >= (krange ? 1024 : 256)) {
-            dim3 grid;
-            if (!grid_for<GemmArgs<TA, TB, TC, double>, 128, 128>(p, 0, grid)) return hipErrorInvalidValue;
-            const dim3 block(256);
-            if (!at && !bt) gemm_dma_f64acc_kernel<false, false><<<grid, block, 0, s>>>(p);
-            else if (!at && bt) gemm_dma_f64acc_kernel<false, true><<<grid, block, 0, s>>>(p);
-            else if (at && !bt) gemm_dma_f64acc_kernel<true, false><<<grid, block, 0, s>>>(p);
-            else gemm_dma_f64acc_kernel<true, true><<<grid, block, 0, s>>>(p);
-            return hipGetLastError();
-        }
-    }
-    return launch_fp<double, TA, TB, TC, 64, 64, 32>(at, bt, p, 0, s);
-}
-
 template <bool F16>
 static hipError_t gemm16_t(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t* A,
                            int64_t lda, const uint16_t* B, int64_t ldb, float beta, uint16_t* C, int64_t ldc,
@@ -657,14 +611,5 @@ hipError_t gemm16(bool f16, bool at, bool bt, int64_t m, int64_t n, int64_t k, f
     if (f16) return gemm16_t<true>(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
     return gemm16_t<false>(at, bt, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
-
-#define MBH_F64ACC(TA, TB, TC)                                                                                     \
-    template hipError_t gemm_f64acc<TA, TB, TC>(bool, bool, int64_t, int64_t, int64_t, double, const TA*, int64_t, \
-                                                const TB*, int64_t, double, TC*, int64_t, hipStream_t, int, int);
-MBH_F64ACC(double, float, float)
-MBH_F64ACC(float, double, float)
-MBH_F64ACC(float, float, double)
-MBH_F64ACC(double, double, double)
-#undef MBH_F64ACC
 
 }  // namespace mbh

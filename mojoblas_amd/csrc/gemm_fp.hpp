@@ -614,13 +614,4 @@ __global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
     gemm_dma_tile<T, T, T, AT, BT>(p, lds);
 }
 
-// fp32 operands, fp64 products and accumulation, fp64 C (the refinement
-// residual of the triangular solve): fp32 panels in LDS, each element
-// widened at the fragment read, v_mfma_f64_16x16x4f64 on 32-deep k-tiles.
-template <bool AT, bool BT>
-__global__ __launch_bounds__(256) void gemm_dma_f64acc_kernel(GemmArgs<float, float, double, double> p) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds[DmaTileLds<float, double, double, AT, BT>::BYTES];
-    gemm_dma_tile<float, double, double, AT, BT>(p, lds);
-}
-
 }  // namespace mbh

@@ -4,10 +4,10 @@
 //         multiples of NB = 128) until a leaf is one 128x128 diagonal block;
 //         a leaf applies the fp64 inverse of that block with an fp64-MFMA
 //         product, every off-diagonal update is one large gemm on the fp32
-//         (or BF16x3) matrix cores.  For fp32 data one step of iterative
-//         refinement follows: R = alpha*B - op(A)*X formed in fp64 from the
-//         fp32 operands, op(A)*D = R solved the same way but with bf16
-//         trailing updates (D needs only a few digits), X += D.
+//         (or BF16x3) matrix cores; left solves of up to 4 blocks run as one
+//         panel kernel.  For fp32 data with refinement asked for, the same
+//         solve runs on fp64 copies of alpha*B and of A's triangle with fp64
+//         products throughout, and X is rounded to fp32 once.
 //   trmm  op(A) materialized as a full triangle (zeros, unit diagonal), one
 //         gemm whose tiles visit only the k-tiles where op(A) is nonzero, into
 //         scratch, then copied back (trmm is in place).
@@ -217,7 +217,13 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int64_t len, int64_t oth
             const int q = 32 * wave + 16 * i + l16, k = 4 * kk + lg;
             af[i][kk] = LEFT ? inv[q + (size_t)k * NB] : inv[k + (size_t)q * NB];
         }
-    for (int e = tid; e < NB * LEAF_W; e += 256) {
+    // B_b into LDS, every load issued before the first is used (indices
+    // clamped into B, zero outside it).
+    constexpr int PER = NB * LEAF_W / 256;
+    T bv_[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * 256;
         int k, c;
         if (LEFT) {  // consecutive threads: consecutive rows k of one column
             k = e % NB;
@@ -226,10 +232,14 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int64_t len, int64_t oth
             c = e % LEAF_W;
             k = e / LEAF_W;
         }
-        double v = 0.0;
-        if (k < len && c0 + c < other)
-            v = LEFT ? (double)B[k + (c0 + c) * ldb] : (double)B[(c0 + c) + k * ldb];
-        bs[k][c] = v;
+        const int64_t kk = min((int64_t)k, len - 1), cc = min(c0 + c, other - 1);
+        bv_[q] = LEFT ? B[kk + cc * ldb] : B[cc + kk * ldb];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * 256;
+        const int k = LEFT ? e % NB : e / LEAF_W, c = LEFT ? e / NB : e % LEAF_W;
+        bs[k][c] = k < len && c0 + c < other ? (double)bv_[q] : 0.0;
     }
     __syncthreads();
     f64x4 acc[2];
@@ -259,6 +269,153 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int64_t len, int64_t oth
                 if (row < other && col < len) B[row + col * ldb] = (T)acc[i][r];
             }
         }
+}
+
+// Left-side panel of up to PANEL_BLK diagonal blocks (PANEL_BLK * NB rows)
+// solved in one kernel: for each block j in solve order, X_j = inv_j * B_j,
+// then B_i -= op(A)_ij * X_j for the panel's blocks i still to come, all in
+// fp64 with the workgroup's strip of B (LEAF_W columns, every panel row)
+// resident in LDS.  This replaces PANEL_BLK leaves and the PANEL_BLK - 1
+// small trailing products between them (k = 128 and 256, the least
+// efficient gemms of the recursion) by one launch.
+//
+// 8 waves; wave w owns rows 16w .. 16w + 15 of a block and the strip's 16
+// columns (one 16x16 v_mfma_f64_16x16x4f64 block: register r of lane l is
+// X(16w + l % 16, 4r + l / 16)).  Operand fragments (inv_j rows, or op(A)_ij
+// rows) come from global memory: lane l holds row 16w + l % 16 at
+// k = 4kk + l / 16.  The fragments of the next product are loaded while the
+// current one runs.  forward: blocks 0, 1, ... (op(A) lower), else from the
+// last block down.  Rows past len are zero in LDS; inv is zero-padded past
+// a short last block and op(A) is only read inside the panel.
+constexpr int PANEL_BLK = 4, PANEL_NT = 512;
+
+template <typename T>
+__global__ __launch_bounds__(PANEL_NT) void trsm_panel_kernel(int64_t len, int64_t other, const double* __restrict__ inv,
+                                                            const T* __restrict__ A, int64_t lda, int trans, int forward,
+                                                            T* B, int64_t ldb) {
+    __shared__ double bs[PANEL_BLK * NB][LEAF_W + 1];  // bs[k][c] = B(k, c0 + c), then X
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int l16 = lane & 15, lg = lane >> 4;
+    const int64_t c0 = (int64_t)blockIdx.x * LEAF_W;
+    const int nbk = (int)((len + NB - 1) / NB);
+    {  // the strip into LDS: every load issued before the first is used
+        constexpr int PER = PANEL_BLK * NB * LEAF_W / PANEL_NT;
+        T v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + q * PANEL_NT, k = e % (PANEL_BLK * NB), c = e / (PANEL_BLK * NB);
+            const int64_t kk = min((int64_t)k, len - 1), cc = min(c0 + c, other - 1);  // consecutive threads: rows
+            v[q] = B[kk + cc * ldb];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + q * PANEL_NT, k = e % (PANEL_BLK * NB), c = e / (PANEL_BLK * NB);
+            bs[k][c] = k < len && c0 + c < other ? (double)v[q] : 0.0;
+        }
+    }
+    // Product p of the panel's sequence: step j = p's block in solve order;
+    // p is the inverse (i = j) or the update of block i by X_j.
+    const int row = 16 * wave + l16;
+    // Fragments of an update hold op(A)'s element (fp32 data: its bits in
+    // the low word), every load unconditional with indices clamped into the
+    // panel; converted and masked where the MFMA uses them, so no load waits
+    // where it is issued.
+    auto load = [&](double (&f)[NB / 4], int i, int j, int k0, int k1) {  // k-steps [k0, k1)
+        if (i == j) {
+            const double* iv = inv + (size_t)j * NB * NB;
+#pragma unroll
+            for (int kk = k0; kk < k1; ++kk) f[kk] = iv[row + (size_t)(4 * kk + lg) * NB];
+        } else {
+            const int64_t gi = min((int64_t)i * NB + row, len - 1);
+#pragma unroll
+            for (int kk = k0; kk < k1; ++kk) {
+                const int64_t gk = min((int64_t)j * NB + 4 * kk + lg, len - 1);
+                const T x = trans ? A[gk + gi * lda] : A[gi + gk * lda];
+                if constexpr (sizeof(T) == 4)
+                    f[kk] = __builtin_bit_cast(double, (unsigned long long)__builtin_bit_cast(unsigned, x));
+                else
+                    f[kk] = x;
+            }
+        }
+    };
+    auto blk = [&](int step) { return forward ? step : nbk - 1 - step; };
+    // The ordered list of products: for step s, (j, j), then (i, j) for the
+    // blocks i after j in solve order.
+    auto next_prod = [&](int& s, int& t) {  // t = 0: inverse; t > 0: t-th update of step s
+        if (s + t + 1 < nbk) {
+            ++t;
+        } else {
+            ++s;
+            t = 0;
+        }
+    };
+    double fa[NB / 4], fb[NB / 4];
+    int s = 0, t = 0;
+    load(fa, blk(0), blk(0), 0, NB / 4);
+    __syncthreads();
+    // One product with fragments fu (loaded earlier); loads the next
+    // product's into fn.  Returns false after the last product.
+    auto product = [&](double (&fu)[NB / 4], double (&fn)[NB / 4]) {
+        const int j = blk(s), i = blk(s + t);
+        int s2 = s, t2 = t;
+        next_prod(s2, t2);
+        // The next product's fragments (after the last product, a harmless
+        // reload, so that every product issues the same loads), issued a
+        // quarter at a time between the MFMAs of this one so that the load
+        // unit's queue drains while the matrix cores run.
+        const int ni = s2 < nbk ? blk(s2 + t2) : j, nj = s2 < nbk ? blk(s2) : j;
+        const int bj = j * NB;
+        const bool rok = i * NB + row < len;
+        auto mfma = [&](double b, double a, f64x4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, c, 0, 0, 0); };
+        // two accumulators (even and odd k-steps): two independent chains
+        f64x4 acc0 = f64x4{0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+        constexpr int Q = NB / 16;  // k-steps per quarter
+#pragma unroll
+        for (int h = 0; h < NB / 4; h += Q) {
+            load(fn, ni, nj, h, h + Q);
+            double bv[Q];
+#pragma unroll
+            for (int kk = 0; kk < Q; ++kk) bv[kk] = bs[bj + 4 * (h + kk) + lg][l16];
+            if (t != 0) {  // op(A) element (converted in place), zero outside the panel
+#pragma unroll
+                for (int kk = h; kk < h + Q; ++kk) {
+                    double x = fu[kk];
+                    if constexpr (sizeof(T) == 4)
+                        x = (double)__builtin_bit_cast(float, (unsigned)__builtin_bit_cast(unsigned long long, x));
+                    fu[kk] = rok && bj + 4 * kk + lg < len ? x : 0.0;
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < Q; kk += 2) {
+                acc0 = mfma(bv[kk], fu[h + kk], acc0);
+                acc1 = mfma(bv[kk + 1], fu[h + kk + 1], acc1);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const f64x4 acc = acc0 + acc1;
+        const int bi = i * NB + row;  // this lane's row of the result
+        auto col = [&](int r) { return 4 * r + lg; };
+        if (t == 0) {
+            __syncthreads();  // every wave has read B_j
+#pragma unroll
+            for (int r = 0; r < 4; ++r) bs[bi][col(r)] = acc[r];
+            __syncthreads();  // X_j complete
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) bs[bi][col(r)] -= acc[r];
+            if (s2 != s) __syncthreads();  // every update of step s done
+        }
+        s = s2;
+        t = t2;
+        return s < nbk;
+    };
+    while (product(fa, fb) && product(fb, fa)) {
+    }
+    for (int e = tid; e < PANEL_BLK * NB * LEAF_W; e += PANEL_NT) {
+        const int k = e % (PANEL_BLK * NB), c = e / (PANEL_BLK * NB);
+        if (k < len && c0 + c < other) B[k + (c0 + c) * ldb] = (T)bs[k][c];
+    }
 }
 
 template <typename T>
@@ -317,7 +474,17 @@ struct TriSolver {
         return hipGetLastError();
     }
 
+    // Left solves of up to PANEL_BLK diagonal blocks run as one panel kernel.
+    hipError_t panel(T* B, int64_t ldb, int64_t r0, int64_t len) const {
+        const double* iv = inv + (size_t)(r0 / NB) * NB * NB;
+        if (n / LEAF_W >= INT_MAX) return hipErrorInvalidValue;
+        trsm_panel_kernel<T><<<dim3((unsigned)((n + LEAF_W - 1) / LEAF_W)), PANEL_NT, 0, s>>>(
+            len, n, iv, opA(r0, r0), lda, trans ? 1 : 0, eff_lower ? 1 : 0, B + r0, ldb);
+        return hipGetLastError();
+    }
+
     hipError_t solve(T* B, int64_t ldb, int64_t r0, int64_t len) const {
+        if (left && len > NB && len <= PANEL_BLK * NB) return panel(B, ldb, r0, len);
         if (len <= NB) return leaf(B, ldb, r0, len);
         const int64_t h = split(len), r1 = r0 + h, len2 = len - h;
         if (left) {
@@ -384,54 +551,37 @@ hipError_t trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t
     const int64_t nblk = (na + NB - 1) / NB;
     const bool do_ir = (sizeof(T) == 4) && refine > 0;
 
-    // scratch: inverses | (IR) fp64 alpha*B0 | fp32 triangle | fp32 correction
+    // scratch: inverses | (refine) fp64 X | fp64 triangle
     size_t inv_b = (size_t)nblk * NB * NB * sizeof(double);
-    size_t b0_b = do_ir ? (size_t)m * n * sizeof(double) : 0;
-    size_t tri_b = do_ir ? (size_t)na * na * sizeof(T) : 0;
-    size_t d_b = do_ir ? (size_t)m * n * sizeof(T) : 0;
+    size_t x_b = do_ir ? (size_t)m * n * sizeof(double) : 0;
+    size_t tri_b = do_ir ? (size_t)na * na * sizeof(double) : 0;
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     void* ws = nullptr;
-    MBH_TRY(workspace(s, 1, al(inv_b) + al(b0_b) + al(tri_b) + al(d_b), &ws));
+    MBH_TRY(workspace(s, 1, al(inv_b) + al(x_b) + al(tri_b), &ws));
     char* base = (char*)ws;
     double* inv = (double*)base;
-    double* b0 = (double*)(base + al(inv_b));
-    T* tri = (T*)(base + al(inv_b) + al(b0_b));
-    T* d = (T*)(base + al(inv_b) + al(b0_b) + al(tri_b));
+    double* x64 = (double*)(base + al(inv_b));
+    double* tri64 = (double*)(base + al(inv_b) + al(x_b));
 
     MBH_TRY(tri_block_inverses<T>(na, A, lda, eff_lower, trans, unit, inv, s));
     if (do_ir) {
-        copy_kernel<T, double><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, B, ldb, b0, m, (double)alpha);
-        tri_copy_kernel<T, T><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0, tri,
-                                                                      na);
+        // fp32 data to fp32 accuracy: the whole solve in fp64 (fp64 copies
+        // of alpha*B and of A's triangle, the fp64 inverses, fp64 panels and
+        // fp64 products), X rounded to fp32 once.  The same products in fp32
+        // plus an fp64 residual (m^2 n flop, as many as the fp64 solve's
+        // products) and a correction solve cost more and round more often.
+        copy_kernel<T, double><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, B, ldb, x64, m, (double)alpha);
+        tri_copy_kernel<T, double><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0,
+                                                                           tri64, na);
         MBH_TRY(hipGetLastError());
+        TriSolver<double> t64{left, eff_lower, trans, m, n, tri64, na, inv, prec, s};
+        MBH_TRY(t64.solve(x64, m, 0, na));
+        copy_kernel<double, T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, x64, m, B, ldb, 1.0);
+        return hipGetLastError();
     }
     if (alpha != T(1)) MBH_TRY(scale<T>(m, n, alpha, B, ldb, s));
     TriSolver<T> ts{left, eff_lower, trans, m, n, A, lda, inv, prec, s};
-    MBH_TRY(ts.solve(B, ldb, 0, na));
-    if (!do_ir) return hipSuccess;
-
-    // R = alpha*B0 - op(T) X in fp64 (b0 becomes R); op(T) visits only its
-    // nonzero k-tiles.
-    const int resid_shape = 0;
-    if constexpr (sizeof(T) == 4) {
-        if (left)
-            MBH_TRY((gemm_f64acc<float, float, double>(trans, false, m, n, m, -1.0, tri, na, B, ldb, 1.0, b0, m, s,
-                                                       resid_shape, eff_lower ? KR_LE_ROW : KR_GE_ROW)));
-        else
-            MBH_TRY((gemm_f64acc<float, float, double>(false, trans, m, n, n, -1.0, B, ldb, tri, na, 1.0, b0, m, s,
-                                                       resid_shape, eff_lower ? KR_GE_COL : KR_LE_COL)));
-    }
-    copy_kernel<double, T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, b0, m, d, m, 1.0);
-    MBH_TRY(hipGetLastError());
-    // The correction d = op(A)^-1 R only needs a few correct digits: R is
-    // of the size of the first solve's error, so a relative error e in d
-    // adds e |R|-sized error to X, far below fp32 rounding for e ~ 1e-2.
-    // Its trailing updates therefore run as single bf16 products (fp32
-    // accumulate); the fp64 residual sets the refined accuracy.
-    TriSolver<T> tc{left, eff_lower, trans, m, n, A, lda, inv, PREC_BF16, s};
-    MBH_TRY(tc.solve(d, m, 0, na));
-    add_kernel<T><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, d, m, B, ldb);
-    return hipGetLastError();
+    return ts.solve(B, ldb, 0, na);
 }
 
 // ---------------------------------------------------------------------------

@@ -6,8 +6,8 @@ Bounds:
       |X - X_oracle| <= (e_rep + (2k+2) u)|alpha| P + 4u|beta C|
     with P the matching |A||B| product (helpers.gemm_bound);
   trsm: normwise on the solution, max|X - X_oracle| <= tol * max|X_oracle|,
-    tol = 2e-6 for fp32 data (one fp64-residual refinement step brings the
-    fp32/BF16x3 solve to fp32 rounding level), 1e-12 for fp64 data; the test
+    tol = 1.1 * 2^-24 for fp32 data (the refined entry solves in fp64 and
+    rounds X to fp32 once), 1e-12 for fp64 data; the test
     matrices have diagonal U[1,2] and off-diagonal U[-1,1]/n (well conditioned).
 """
 import itertools
@@ -59,7 +59,7 @@ def tri_operand(rng, n, upper, dtype, unit_junk=False):
     return np.asfortranarray(a).astype(dtype)
 
 
-TRSM_SIZES = [(1, 1), (5, 7), (130, 33), (300, 129), (257, 400)]
+TRSM_SIZES = [(1, 1), (5, 7), (130, 33), (300, 129), (257, 400), (1100, 70), (70, 1100)]
 
 
 @pytest.mark.parametrize("side,uplo,ta,diag", list(itertools.product("LR", "UL", "NT", "NU")))
@@ -78,14 +78,14 @@ def test_trsm_vs_oracle(side, uplo, ta, diag, prec):
         kw = {} if prec == "f64" else {"precision": prec}
         assert mb.trsm(side, uplo, ta, diag, m, n, alpha, A, na, B, m, **kw) == 0
         got = B.cpu().numpy().astype(np.float64)
-        tol = 1e-12 if prec == "f64" else 2e-6
+        tol = 1e-12 if prec == "f64" else 1.1 * 2.0 ** -24
         err = np.abs(got - want).max() / max(np.abs(want).max(), 1e-30)
         assert err <= tol, (m, n, side, uplo, ta, diag, prec, err)
 
 
 def test_trsm_refinement_reaches_fp32_level():
-    """Without refinement the BF16x3 updates leave ~1e-6-level error; one
-    fp64-residual step must bring it to fp32 rounding level."""
+    """Without refinement the BF16x3 updates leave ~1e-6-level error; the
+    refined (fp64) solve must be at fp32 rounding level."""
     rng = np.random.default_rng(22)
     m, n = 1024, 256
     a = tri_operand(rng, m, False, np.float32)
@@ -99,6 +99,37 @@ def test_trsm_refinement_reaches_fp32_level():
         errs[refine] = np.abs(B.cpu().numpy() - want).max() / np.abs(want).max()
     assert errs[1] <= 2e-7, errs
     assert errs[1] <= errs[0], errs
+
+
+@pytest.mark.parametrize("side,uplo,ta", list(itertools.product("LR", "UL", "NT")))
+@pytest.mark.parametrize("prec", ["f64", "fp32"])
+def test_trsm_ill_conditioned_triangle(side, uplo, ta, prec):
+    """Off-diagonal U[-1,1] without the 1/n scaling (diagonal U[1,2]): the
+    condition number grows roughly tenfold per 16 rows (~1e4 at n = 96,
+    ~1e8 at n = 260), so the explicit fp64 inverses of the diagonal blocks
+    hold large entries and a wrong element of them shows.  Normwise bound
+    against the fp64 oracle: fp64 data 64 kappa 2^-53; fp32 data (refined:
+    solved in fp64, X rounded to fp32 once) 2^-24 + 64 kappa 2^-53."""
+    rng = np.random.default_rng(31)
+    dt = np.float64 if prec == "f64" else np.float32
+    sizes = [96, 160, 260] if prec == "f64" else [48, 96]
+    for na in sizes:
+        a = rng.uniform(-1, 1, (na, na))
+        a[np.diag_indices(na)] = rng.uniform(1, 2, na)
+        a = np.triu(a) if uplo == "U" else np.tril(a)
+        a = np.asfortranarray(a).astype(dt)
+        kappa = np.linalg.cond(a.astype(np.float64))
+        m, n = (na, 40) if side == "L" else (40, na)
+        b = np.asfortranarray(rng.uniform(-1, 1, (m, n))).astype(dt)
+        want = b.astype(np.float64).ravel(order="F").copy()
+        assert oracle.dtrsm(side, uplo, ta, "N", m, n, 1.0, a.astype(np.float64).ravel(order="F"), na, want, m) == 0
+        A, B = dev(a.ravel(order="F")), dev(b.ravel(order="F"))
+        kw = {} if prec == "f64" else {"precision": prec}
+        assert mb.trsm(side, uplo, ta, "N", m, n, 1.0, A, na, B, m, **kw) == 0
+        got = B.cpu().numpy().astype(np.float64)
+        tol = 64 * kappa * 2.0 ** -53 + (0.0 if prec == "f64" else 2.0 ** -24)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err <= max(tol, 1e-12), (na, side, uplo, ta, prec, kappa, err, tol)
 
 
 @pytest.mark.parametrize("side,uplo,ta,diag", list(itertools.product("LR", "UL", "NT", "NU")))
@@ -335,11 +366,11 @@ def test_whole_tile_products_sampled(routine, side, uplo, trans, prec):
 
 @pytest.mark.parametrize("side,uplo,ta", [("L", "L", "N"), ("L", "U", "T"), ("R", "U", "N"), ("R", "L", "T")])
 def test_refined_strsm_whole_tiles(side, uplo, ta):
-    """strsm at 4096 (whole 128-tiles, so the fp64 refinement residual runs
-    on the direct-to-LDS kernel): against an fp64 solve of the same fp32
-    inputs, the refined solution must be within 4 u32 max|X| and well below
-    the unrefined one's error (a residual that came out zero or wrong would
-    leave the unrefined error or worse)."""
+    """strsm at 4096 (whole 128-tiles, so the refined solve's fp64 updates
+    run on the direct-to-LDS fp64 kernel and its panels cover 4 diagonal
+    blocks): against an fp64 solve of the same fp32 inputs, the refined
+    solution must be within 4 u32 max|X| and well below the unrefined one's
+    error."""
     m = n = 4096
     g = torch.Generator(device="cuda").manual_seed(43)
     a = (torch.rand(m, m, device="cuda", generator=g, dtype=torch.float64) * 2 - 1) / m
