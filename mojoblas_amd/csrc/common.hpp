@@ -122,7 +122,21 @@ __host__ __device__ inline void tri_tile_coords(int mode, int t, int T, int& tm,
     }
 }
 
-// k-range restriction for triangular operands (trmm, trsm residuals): a tile
+// How the split-precision pack reads an operand, in the coordinates (r, k) of
+// the packed operand (r: row of op(A) or column of op(B); k: the reduction
+// index): as laid out, as a triangle (trmm: zeros across the diagonal, the
+// diagonal 1 with PV_UNIT), or as a symmetric matrix of which one half is
+// stored (symm: the other half read at (k, r)).
+enum PackView {
+    PV_PLAIN = 0,
+    PV_TRI_LE = 1,  // op(X)(r, k) = 0 for k > r
+    PV_TRI_GE = 2,  // op(X)(r, k) = 0 for k < r
+    PV_SYM_LE = 3,  // (r, k) with k <= r stored; k > r read at (k, r)
+    PV_SYM_GE = 4,  // (r, k) with k >= r stored; k < r read at (k, r)
+    PV_UNIT = 8,    // with PV_TRI_*: op(X)(r, r) = 1
+};
+
+// k-range restriction for triangular operands (trmm): a tile
 // only visits the k-tiles where op(A) or op(B) can be nonzero.
 enum KRange {
     KR_ALL = 0,

@@ -30,6 +30,16 @@ hipError_t sgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha,
 hipError_t sgemm_syr2k(bool at, bool bt, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
                        const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int prec, hipStream_t s, int tri,
                        int tri_grid);
+// sgemm in a split-precision mode whose pack reads op(A) / op(B)^T through a
+// PackView (aview / bview: a triangle or one stored half of a symmetric
+// matrix), so the caller makes no full copy of that operand.  C is written
+// only after both operands are packed, so it may alias A or B.  Modes
+// without a pack pass of their own (fp32, FP8) return hipErrorNotSupported
+// with nothing queued: sgemm_has_views(prec) says which.
+bool sgemm_has_views(int prec);
+hipError_t sgemm_view(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                      int aview, const float* B, int64_t ldb, int bview, float beta, float* C, int64_t ldc, int prec,
+                      hipStream_t s, int krange = 0);
 // Bytes of pack-plane scratch that sgemm needs for a plain product (no
 // triangle grid or k-range) in precision `prec`: 0 for fp32, alpha == 0 or
 // k == 0.

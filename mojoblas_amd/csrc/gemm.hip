@@ -227,9 +227,12 @@ hipError_t packed_main(PackedGemmParams p, int64_t ntile, bool split, hipStream_
 //    whose partial sums go to slot-4 workspace and are added in chunk order
 //    by splitk_reduce_kernel (deterministic).  Not for k-range launches or a
 //    caller's scratch.
+//  * aview / bview (PackView): the pack reads op(A) / op(B)^T as a triangle
+//    or from one stored half of a symmetric matrix (trmm, symm), so that no
+//    copy of the operand is made first.
 template <int NPROD, int BT_, int WM_, int WN_>
 hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s,
-                      const Scratch* ext, bool pair2 = false) {
+                      const Scratch* ext, bool pair2 = false, int aview = PV_PLAIN, int bview = PV_PLAIN) {
     constexpr int BT = BT_, PK = 32;
     PackedGemmParams p{};
     const int64_t tm = (g.m + BT - 1) / BT, tn = (g.n + BT - 1) / BT;
@@ -237,15 +240,16 @@ hipError_t run_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid
     const int64_t ntile = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
     if (ntile <= 0 || ntile > INT_MAX || ktiles > INT_MAX) return hipErrorInvalidValue;
     const int planes = (NPROD == 3) ? 2 : 1;
-    const bool shared = g.A == g.B && g.lda == g.ldb && g.m == g.n && at != bt;
+    const bool shared = g.A == g.B && g.lda == g.ldb && g.m == g.n && at != bt && aview == PV_PLAIN &&
+                        bview == PV_PLAIN;
     const size_t a_el = (size_t)tm * BT * ktiles * PK, b_el = shared ? 0 : (size_t)tn * BT * ktiles * PK;
     void* ws = nullptr;
     hipError_t e = pack_scratch(ext, s, (a_el + b_el) * planes * sizeof(uint16_t), &ws);
     if (e != hipSuccess) return e;
     uint16_t* base = (uint16_t*)ws;
-    PackParams pa{g.A, g.lda, g.m, g.k, (int)ktiles, base, planes == 2 ? base + a_el : nullptr, g.a_vec};
+    PackParams pa{g.A, g.lda, g.m, g.k, (int)ktiles, base, planes == 2 ? base + a_el : nullptr, g.a_vec, aview};
     uint16_t* bbase = base + a_el * planes;
-    PackParams pb{g.B, g.ldb, g.n, g.k, (int)ktiles, bbase, planes == 2 ? bbase + b_el : nullptr, g.b_vec};
+    PackParams pb{g.B, g.ldb, g.n, g.k, (int)ktiles, bbase, planes == 2 ? bbase + b_el : nullptr, g.b_vec, bview};
     if (shared) {
         pb.hi = pa.hi;
         pb.lo = pa.lo;
@@ -367,15 +371,16 @@ int packed_tile(int nprod, int64_t m, int64_t n, int tri_grid, int krange) {
 
 template <int NPROD>
 hipError_t launch_packed(bool at, bool bt, const GemmParams<float>& g, int tri_grid, hipStream_t s,
-                         const Scratch* ext, bool pair2 = false) {
+                         const Scratch* ext, bool pair2 = false, int aview = PV_PLAIN, int bview = PV_PLAIN) {
     const int bt_ = packed_tile(NPROD, g.m, g.n, tri_grid, g.krange);
     if constexpr (NPROD == 8) {
+        if (aview != PV_PLAIN || bview != PV_PLAIN) return hipErrorNotSupported;
         if (bt_ == 256) return run_fp8<256, 2, 4>(at, bt, g, tri_grid, s, ext);
         return run_fp8<128, 2, 2>(at, bt, g, tri_grid, s, ext);
     } else {
-        if (bt_ == 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s, ext, pair2);
-        if (bt_ == 128) return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s, ext, pair2);
-        return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s, ext, pair2);
+        if (bt_ == 256) return run_packed<NPROD, 256, 2, 4>(at, bt, g, tri_grid, s, ext, pair2, aview, bview);
+        if (bt_ == 128) return run_packed<NPROD, 128, 2, 2>(at, bt, g, tri_grid, s, ext, pair2, aview, bview);
+        return run_packed<NPROD, 64, 2, 2>(at, bt, g, tri_grid, s, ext, pair2, aview, bview);
     }
 }
 
@@ -567,6 +572,19 @@ hipError_t sgemm_syr2k(bool at, bool bt, int64_t n, int64_t k, float alpha, cons
     hipError_t e = sgemm(at, bt, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, s, tri, tri_grid);
     if (e != hipSuccess) return e;
     return sgemm(at, bt, n, n, k, alpha, B, ldb, A, lda, 1.0f, C, ldc, prec, s, tri, tri_grid);
+}
+
+bool sgemm_has_views(int prec) { return prec == PREC_BF16X3 || prec == PREC_BF16; }
+
+hipError_t sgemm_view(bool at, bool bt, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                      int aview, const float* B, int64_t ldb, int bview, float beta, float* C, int64_t ldc, int prec,
+                      hipStream_t s, int krange) {
+    if (!sgemm_has_views(prec)) return hipErrorNotSupported;
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (alpha == 0.0f || k == 0) return scale<float>(m, n, beta, C, ldc, s);
+    GemmParams<float> p = make_params<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, krange);
+    return prec == PREC_BF16X3 ? launch_packed<3>(at, bt, p, 0, s, nullptr, false, aview, bview)
+                               : launch_packed<1>(at, bt, p, 0, s, nullptr, false, aview, bview);
 }
 
 hipError_t dgemm(bool at, bool bt, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,

@@ -35,31 +35,77 @@ inline dim3 grid_for_elems(int64_t count) {
     return dim3((unsigned)b);
 }
 
+// Square-matrix copies in 64 x 64 tiles, one tile per workgroup of 256
+// threads (grid ceil(n/64) x ceil(n/64)); thread (tx, ty) = (tid % 64,
+// tid / 64) handles rows i0 + tx of columns j0 + ty + 4q, so every global
+// access of a wave covers 64 consecutive rows of one column.
+constexpr int CT = 64;
+
+inline dim3 tile_grid(int64_t n) { return dim3((unsigned)((n + CT - 1) / CT), (unsigned)((n + CT - 1) / CT)); }
+
 // out(i,j) = element (i,j) of the `upper`/lower triangle of A (zeros outside,
-// 1 on the diagonal when unit), n x n.
+// 1 on the diagonal when unit), n x n.  Tiles outside the triangle are
+// written without reading A.
 template <typename Tin, typename Tout>
-__global__ void tri_copy_kernel(int64_t n, const Tin* A, int64_t lda, int upper, int unit, Tout* out,
-                                int64_t ldo) {
-    const int64_t total = n * n;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = idx % n, j = idx / n;
+__global__ __launch_bounds__(256) void tri_copy_kernel(int64_t n, const Tin* A, int64_t lda, int upper, int unit,
+                                                       Tout* out, int64_t ldo) {
+    const int tx = threadIdx.x % CT, ty = threadIdx.x / CT;
+    const int64_t i0 = (int64_t)blockIdx.x * CT, j0 = (int64_t)blockIdx.y * CT, i = i0 + tx;
+    if (i >= n) return;
+    const bool inside = upper ? i0 < j0 : i0 > j0, outside = upper ? i0 > j0 : i0 < j0;  // off-diagonal tiles
+#pragma unroll 4
+    for (int q = 0; q < CT / 4; ++q) {
+        const int64_t j = j0 + ty + 4 * q;
+        if (j >= n) break;
         Tout v = Tout(0);
-        if (i == j) v = unit ? Tout(1) : (Tout)A[i + j * lda];
-        else if (upper ? (i < j) : (i > j)) v = (Tout)A[i + j * lda];
+        if (inside) v = (Tout)A[i + j * lda];
+        else if (!outside) {
+            if (i == j) v = unit ? Tout(1) : (Tout)A[i + j * lda];
+            else if (upper ? i < j : i > j) v = (Tout)A[i + j * lda];
+        }
         out[i + j * ldo] = v;
     }
 }
 
-// out = A with the unreferenced triangle mirrored from the stored one.
+// out = A with the unreferenced triangle mirrored from the stored one.  A
+// mirrored tile is read as the transposed tile of A (coalesced) and turned
+// through LDS; a diagonal tile is staged whole and each element taken from
+// the stored side.
 template <typename T>
-__global__ void sym_copy_kernel(int64_t n, const T* A, int64_t lda, int upper, T* out, int64_t ldo) {
-    const int64_t total = n * n;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = idx % n, j = idx / n;
-        const bool stored = upper ? (i <= j) : (i >= j);
-        out[i + j * ldo] = stored ? A[i + j * lda] : A[j + i * lda];
+__global__ __launch_bounds__(256) void sym_copy_kernel(int64_t n, const T* A, int64_t lda, int upper, T* out,
+                                                       int64_t ldo) {
+    __shared__ T t[CT][CT + 1];
+    const int tx = threadIdx.x % CT, ty = threadIdx.x / CT;
+    const int64_t i0 = (int64_t)blockIdx.x * CT, j0 = (int64_t)blockIdx.y * CT;
+    const bool diag = i0 == j0, stored = upper ? i0 < j0 : i0 > j0;
+    if (!diag && stored) {
+        const int64_t i = i0 + tx;
+        if (i >= n) return;
+        for (int q = 0; q < CT / 4; ++q) {
+            const int64_t j = j0 + ty + 4 * q;
+            if (j >= n) break;
+            out[i + j * ldo] = A[i + j * lda];
+        }
+        return;
+    }
+    // t[a][b] = A(r0 + b, c0 + a) with (r0, c0) = (j0, i0) for a mirrored
+    // tile (so t[a][b] = out(i0 + a, j0 + b)), (i0, j0) on the diagonal.
+    const int64_t r0 = diag ? i0 : j0, c0 = diag ? j0 : i0;
+    for (int q = 0; q < CT / 4; ++q) {
+        const int a = ty + 4 * q;
+        if (r0 + tx < n && c0 + a < n) t[a][tx] = A[(r0 + tx) + (c0 + a) * lda];
+    }
+    __syncthreads();
+    const int64_t i = i0 + tx;
+    if (i >= n) return;
+    for (int q = 0; q < CT / 4; ++q) {
+        const int b = ty + 4 * q;
+        const int64_t j = j0 + b;
+        if (j >= n) break;
+        T v;
+        if (!diag) v = t[tx][b];
+        else v = (upper ? i <= j : i >= j) ? t[b][tx] : t[tx][b];  // A(i, j) : A(j, i)
+        out[i + j * ldo] = v;
     }
 }
 
@@ -571,7 +617,7 @@ hipError_t trsm(char side, char uplo, char transa, char diag, int64_t m, int64_t
         // plus an fp64 residual (m^2 n flop, as many as the fp64 solve's
         // products) and a correction solve cost more and round more often.
         copy_kernel<T, double><<<grid_for_elems(m * n), 256, 0, s>>>(m, n, B, ldb, x64, m, (double)alpha);
-        tri_copy_kernel<T, double><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0,
+        tri_copy_kernel<T, double><<<tile_grid(na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0,
                                                                            tri64, na);
         MBH_TRY(hipGetLastError());
         TriSolver<double> t64{left, eff_lower, trans, m, n, tri64, na, inv, prec, s};
@@ -598,13 +644,27 @@ hipError_t trmm(char side, char uplo, char transa, char diag, int64_t m, int64_t
     const bool unit = (diag == 'U' || diag == 'u');
     const bool eff_lower = (!upper) != trans;
     const int64_t na = left ? m : n;
+    if constexpr (sizeof(T) == 4) {
+        // Split-precision modes: the pack reads op(A) as a triangle straight
+        // from A and the product is written into B, which by then lives only
+        // in its packed planes.  Views are in the packed operand's (r, k):
+        // left, op(A)(r, k); right, op(A)(k, r).
+        if (sgemm_has_views(prec)) {
+            const int view = (eff_lower == left ? PV_TRI_LE : PV_TRI_GE) | (unit ? PV_UNIT : 0);
+            if (left)
+                return sgemm_view(trans, false, m, n, m, alpha, A, lda, view, B, ldb, PV_PLAIN, 0.0f, B, ldb, prec,
+                                  s, eff_lower ? KR_LE_ROW : KR_GE_ROW);
+            return sgemm_view(false, trans, m, n, n, alpha, B, ldb, PV_PLAIN, A, lda, view, 0.0f, B, ldb, prec, s,
+                              eff_lower ? KR_GE_COL : KR_LE_COL);
+        }
+    }
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t tri_b = (size_t)na * na * sizeof(T), w_b = (size_t)m * n * sizeof(T);
     void* ws = nullptr;
     MBH_TRY(workspace(s, 1, al(tri_b) + al(w_b), &ws));
     T* tri = (T*)ws;
     T* w = (T*)((char*)ws + al(tri_b));
-    tri_copy_kernel<T, T><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0, tri, na);
+    tri_copy_kernel<T, T><<<tile_grid(na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, unit ? 1 : 0, tri, na);
     MBH_TRY(hipGetLastError());
     if (left)
         MBH_TRY(xgemm<T>(trans, false, m, n, m, alpha, tri, na, B, ldb, T(0), w, m, prec, s, 0, 0,
@@ -655,10 +715,20 @@ hipError_t symm(char side, char uplo, int64_t m, int64_t n, T alpha, const T* A,
     const bool left = (side == 'L' || side == 'l');
     const bool upper = (uplo == 'U' || uplo == 'u');
     const int64_t na = left ? m : n;
+    if constexpr (sizeof(T) == 4) {
+        // Split-precision modes: the pack reads A's stored half and mirrors
+        // the other (left, (r, k) = A(r, k); right, (r, k) = A(k, r)).
+        if (sgemm_has_views(prec)) {
+            const int view = (upper != left) ? PV_SYM_LE : PV_SYM_GE;
+            if (left)
+                return sgemm_view(false, false, m, n, m, alpha, A, lda, view, B, ldb, PV_PLAIN, beta, C, ldc, prec, s);
+            return sgemm_view(false, false, m, n, n, alpha, B, ldb, PV_PLAIN, A, lda, view, beta, C, ldc, prec, s);
+        }
+    }
     void* ws = nullptr;
     MBH_TRY(workspace(s, 1, (size_t)na * na * sizeof(T), &ws));
     T* full = (T*)ws;
-    sym_copy_kernel<T><<<grid_for_elems(na * na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, full, na);
+    sym_copy_kernel<T><<<tile_grid(na), 256, 0, s>>>(na, A, lda, upper ? 1 : 0, full, na);
     MBH_TRY(hipGetLastError());
     if (left) return xgemm<T>(false, false, m, n, m, alpha, full, na, B, ldb, beta, C, ldc, prec, s);
     return xgemm<T>(false, false, m, n, n, alpha, B, ldb, full, na, beta, C, ldc, prec, s);

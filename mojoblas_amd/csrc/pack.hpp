@@ -46,7 +46,30 @@ struct PackParams {
     uint16_t* hi;     // packed planes
     uint16_t* lo;     // may be null (single bf16 plane)
     int vec;          // X 16-byte aligned and ld % 4 == 0 (vector loads along k)
+    int view;         // PackView: how op(X) is read (triangle, symmetric half)
 };
+
+// Classification of a 64-row x BKK-k pack block against the diagonal r = k
+// of its operand view: how its elements are loaded.
+enum PackBlockLoad : int {
+    PBL_AS_LAID = 0,  // every element read where op(X) has it
+    PBL_MIRROR = 1,   // symmetric view, every element read at (k, r)
+    PBL_SELECT = 2,   // symmetric view, block crosses the diagonal: per element
+    PBL_ZERO = 3,     // triangle view, block entirely outside the triangle
+};
+
+template <int BR, int BKK>
+__device__ inline int pack_block_load(int view, int64_t r0, int64_t k0) {
+    const bool all_le = k0 + BKK - 1 <= r0, all_ge = k0 >= r0 + BR - 1;  // k <= r / k >= r everywhere
+    const bool all_lt = k0 + BKK - 1 < r0, all_gt = k0 > r0 + BR - 1;    // strictly
+    switch (view & 7) {
+        case PV_TRI_LE: return all_gt ? PBL_ZERO : PBL_AS_LAID;
+        case PV_TRI_GE: return all_lt ? PBL_ZERO : PBL_AS_LAID;
+        case PV_SYM_LE: return all_le ? PBL_AS_LAID : all_gt ? PBL_MIRROR : PBL_SELECT;
+        case PV_SYM_GE: return all_ge ? PBL_AS_LAID : all_lt ? PBL_MIRROR : PBL_SELECT;
+        default: return PBL_AS_LAID;
+    }
+}
 
 // One 64-row x (32*NKB)-k block (r0, k0) of the split pack, staged in the
 // LDS tile `s` (64 rows of 32*NKB + 1 floats).  Every thread issues all of
@@ -57,16 +80,18 @@ struct PackParams {
 // source is read once and the planes (537 MB for both operands at 8192^2)
 // do not fit the 256 MB last-level cache anyway (pack 0.244 -> 0.236 ms at
 // 8192^3, the main kernel's time unchanged).
-template <bool KCONTIG, int PR, int PK, int NKB>
-__device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t k0, float* s) {
+//
+// Load phase: element (r, k) of op(X) into s[r][k].  KCONTIG: (r, k) lies at
+// X[k + r ld] (4 consecutive k of one row per load), else at X[r + k ld] (4
+// consecutive rows at one k).  A mirrored block of a symmetric view reads
+// (r, k) at the other address, which is this function with KCONTIG flipped.
+template <bool KCONTIG, int NKB>
+__device__ inline void pack_load_tile(const PackParams& p, int64_t r0, int64_t k0, float* s) {
     constexpr int BR = 64, BKK = 32 * NKB, S = BKK + 1, NT = 256;
     constexpr int NLD = BR * BKK / 4 / NT;  // 16-byte loads per thread
-    static_assert(PR % BR == 0 && PK == 32, "pack block must tile the gemm block");
     static_assert(NLD * 4 * NT == BR * BKK, "block not divisible by the thread block");
     const int tid = threadIdx.x;
     const MBH_GLOBAL float* const X = (const MBH_GLOBAL float*)p.X;
-    // Load q of this thread: KCONTIG -> 4 consecutive k of one row, else 4
-    // consecutive rows at one k.
     auto coords = [&](int q, int& r, int& kk) {
         const int idx = tid + q * NT;
         if (KCONTIG) {
@@ -105,11 +130,60 @@ __device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t
             else s[(r + e) * S + kk] = v[q][e];
         }
     }
-    __syncthreads();
+}
+
+// Load phase of a block of a symmetric view that crosses the diagonal: each
+// element read from the stored half (PV_SYM_LE: k <= r stored), one scalar
+// load per element in the thread mapping of pack_load_tile<KCONTIG>.
+template <bool KCONTIG, int NKB>
+__device__ inline void pack_load_select(const PackParams& p, int64_t r0, int64_t k0, float* s) {
+    constexpr int BR = 64, BKK = 32 * NKB, S = BKK + 1, NT = 256;
+    constexpr int NLD = BR * BKK / 4 / NT;
+    const int tid = threadIdx.x;
+    const bool le = (p.view & 7) == PV_SYM_LE;
+    float v[NLD][4];
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+        const int idx = tid + q * NT;
+        const int r = KCONTIG ? idx / (BKK / 4) : (idx % (BR / 4)) * 4;
+        const int kk = KCONTIG ? (idx % (BKK / 4)) * 4 : idx / (BR / 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t er = r0 + r + (KCONTIG ? 0 : e), ek = k0 + kk + (KCONTIG ? e : 0);
+            const bool stored = le ? ek <= er : ek >= er;
+            const int64_t rr = stored ? er : ek, kc = stored ? ek : er;  // read (rr, kc)
+            v[q][e] = (er < p.R && ek < p.K) ? (KCONTIG ? p.X[kc + rr * p.ld] : p.X[rr + kc * p.ld]) : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+        const int idx = tid + q * NT;
+        const int r = KCONTIG ? idx / (BKK / 4) : (idx % (BR / 4)) * 4;
+        const int kk = KCONTIG ? (idx % (BKK / 4)) * 4 : idx / (BR / 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (KCONTIG) s[r * S + kk + e] = v[q][e];
+            else s[(r + e) * S + kk] = v[q][e];
+        }
+    }
+}
+
+template <bool KCONTIG, int PR, int PK, int NKB>
+__device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t k0, float* s) {
+    constexpr int BR = 64, BKK = 32 * NKB, S = BKK + 1;
+    static_assert(PR % BR == 0 && PK == 32, "pack block must tile the gemm block");
+    const int how = p.view ? pack_block_load<BR, BKK>(p.view, r0, k0) : PBL_AS_LAID;  // uniform
+    if (how == PBL_AS_LAID) pack_load_tile<KCONTIG, NKB>(p, r0, k0, s);
+    else if (how == PBL_MIRROR) pack_load_tile<!KCONTIG, NKB>(p, r0, k0, s);
+    else if (how == PBL_SELECT) pack_load_select<KCONTIG, NKB>(p, r0, k0, s);
+    if (how != PBL_ZERO) __syncthreads();
     // Each thread: 8 consecutive k of one row per 32-k sub-block.
+    const int tid = threadIdx.x;
     const int r = tid / 4, kq = (tid % 4) * 8;
     const int64_t gr = r0 + r;
     const int64_t t = gr / PR, rr = gr % PR;
+    const int tv = p.view & 7;
+    const bool tri = tv == PV_TRI_LE || tv == PV_TRI_GE, unit = (p.view & PV_UNIT) != 0 && gr < p.R;
 #pragma unroll
     for (int sb = 0; sb < NKB; ++sb) {
         const int64_t kt = (k0 + sb * 32) / PK;  // the same for every thread of the sub-block
@@ -117,7 +191,15 @@ __device__ inline void pack_split_block(const PackParams& p, int64_t r0, int64_t
         const int64_t off = ((t * p.ktiles + kt) * PR + rr) * PK + kq;
         uint32_t h[8], l[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) split_bf16(s[r * S + sb * 32 + kq + e], h[e], l[e]);
+        for (int e = 0; e < 8; ++e) {
+            float x = how == PBL_ZERO ? 0.0f : s[r * S + sb * 32 + kq + e];
+            if (tri) {  // triangle view: zeros outside, optional unit diagonal
+                const int64_t gk = k0 + sb * 32 + kq + e;
+                if (tv == PV_TRI_LE ? gk > gr : gk < gr) x = 0.0f;
+                if (unit && gk == gr) x = 1.0f;
+            }
+            split_bf16(x, h[e], l[e]);
+        }
         __builtin_nontemporal_store(
             u32x4{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)},
             (MBH_GLOBAL u32x4*)(p.hi + off));

@@ -241,6 +241,55 @@ def test_symm_vs_oracle(side, uplo, prec):
         assert (err <= bound).all(), (m, n, side, uplo, prec, float((err / bound).max()))
 
 
+@pytest.mark.parametrize("routine,side,uplo,ta,diag",
+                         [("symm", s, u, "N", "N") for s in "LR" for u in "UL"]
+                         + [("trmm", s, u, t, d) for s in "LR" for u in "UL" for t in "NT" for d in "NU"])
+def test_split_pack_views_equal_materialized_gemm(routine, side, uplo, ta, diag):
+    """BF16x3 symm / trmm read A through the pack's operand views (the
+    symmetric half mirrored per 64x128 block, per element on blocks that
+    cross the diagonal; the triangle masked, the unit diagonal injected)
+    instead of a copy.  The same product through gemm on the materialized
+    operand (full symmetric matrix; triangle with zeros and the diagonal)
+    packs the same values, so the results are equal bit for bit (a k-range
+    only skips k-tiles of zeros).  Sizes with ragged blocks on both sides of
+    the diagonal; the unused half of A and a unit diagonal hold NaN."""
+    rng = np.random.default_rng(26)
+    for m, n in [(1000, 777), (777, 1000), (129, 65)]:
+        na = m if side == "L" else n
+        full = rng.uniform(-1, 1, (na, na))
+        keep = np.triu(np.ones((na, na), bool)) if uplo == "U" else np.tril(np.ones((na, na), bool))
+        if routine == "symm":
+            full = np.where(keep, full, full.T)  # symmetric
+            op = full
+        else:
+            tri = np.where(keep, full, 0.0)
+            if diag == "U":
+                tri[np.diag_indices(na)] = 1.0
+            op = tri if ta == "N" else tri.T
+        stored = np.where(keep, full, np.nan)
+        if routine == "trmm" and diag == "U":
+            stored[np.diag_indices(na)] = np.nan
+        a = dev(np.asfortranarray(stored).astype(np.float32).ravel(order="F"))
+        opm = dev(np.asfortranarray(op).astype(np.float32).ravel(order="F"))
+        b = dev(np.asfortranarray(rng.uniform(-1, 1, (m, n))).astype(np.float32).ravel(order="F"))
+        c0 = dev(np.asfortranarray(rng.uniform(-1, 1, (m, n))).astype(np.float32).ravel(order="F"))
+        alpha = 0.75
+        beta = 0.5 if routine == "symm" else 0.0
+        got, want = c0.clone(), c0.clone()
+        if routine == "symm":
+            assert mb.symm(side, uplo, m, n, alpha, a, na, b, m, beta, got, m, precision="bf16x3") == 0
+        else:
+            got = b.clone()
+            assert mb.trmm(side, uplo, ta, diag, m, n, alpha, a, na, got, m, precision="bf16x3") == 0
+        if side == "L":
+            assert mb.gemm("N", "N", m, n, m, alpha, opm, na, b, m, beta, want, m, precision="bf16x3") == 0
+        else:
+            assert mb.gemm("N", "N", m, n, n, alpha, b, m, opm, na, beta, want, m, precision="bf16x3") == 0
+        g, w = got.cpu().numpy(), want.cpu().numpy()
+        assert np.isfinite(g).all(), (routine, m, n, side, uplo, ta, diag)
+        assert np.array_equal(g, w), (routine, m, n, side, uplo, ta, diag, float(np.abs(g - w).max()))
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_config3_strsm_4096(prec):
     """strsm L,L,N,N m=n=4096 (config size): A lower, off-diagonal U[-1,1]/n,
