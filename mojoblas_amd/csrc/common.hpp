@@ -53,6 +53,13 @@ struct GemmArgs {
     int b_vec;      // same for B
     int tri_tiles;  // 0: full grid; 1/2: grid enumerates only lower/upper block-triangle tiles
     int krange;     // KRange
+    // Partial-round split (direct-to-LDS tile): a launch with ksplit > 1
+    // covers logical tiles tile0 .. tile0 + gridDim.x / ksplit - 1,
+    // workgroup w taking chunk w % ksplit of tile tile0 + w / ksplit's
+    // k-tiles and storing its raw sum as a 128 x 128 column-major block of
+    // TS at part + w * 128 * 128; dma_splitk_reduce_kernel finishes C.
+    int tile0, ksplit;
+    void* part;
 };
 template <typename T>
 using GemmParams = GemmArgs<T, T, T, T>;

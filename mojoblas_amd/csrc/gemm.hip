@@ -96,16 +96,64 @@ template <typename T>
 bool dma_whole_ok(const GemmParams<T>& p) {
     return p.m % 128 == 0 && p.n % 128 == 0 && p.k % DmaPanel<T, true>::BK == 0 && p.a_vec && p.b_vec;
 }
+int device_cus();
+
+// Workgroups of gemm_dma_kernel<T> resident at once on the current device
+// (CUs x occupancy; the four transposes use the same LDS and registers).
+template <typename T>
+int64_t dma_round() {
+    static std::atomic<int> occ{0};
+    if (occ.load(std::memory_order_relaxed) == 0) {
+        int r = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, gemm_dma_kernel<T, false, false>, 256, 0) != hipSuccess ||
+            r <= 0)
+            r = -1;
+        occ.store(r, std::memory_order_relaxed);
+    }
+    return (int64_t)device_cus() * std::max(occ.load(std::memory_order_relaxed), 0);
+}
+
+// The direct-to-LDS kernel over every 128 x 128 tile of p.  When the tiles
+// leave a last round of workgroups at most half full (ssyrk n = 8192,
+// k = 2048: 2080 triangle tiles on 512 resident workgroups = four rounds
+// and 32 tiles), the whole rounds run as one launch and the remaining tiles
+// are split along k into up to W / tail chunks of >= 8 k-tiles, one
+// workgroup each, whose partial sums go to slot-4 workspace and are added
+// in chunk order by dma_splitk_reduce_kernel (deterministic).  Not for
+// k-range launches, whose tiles differ in length.
 template <typename T>
 hipError_t launch_dma_whole(bool at, bool bt, GemmParams<T> p, int tri_grid, hipStream_t s) {
     dim3 grid;
     if (!grid_for<GemmParams<T>, 128, 128>(p, tri_grid, grid)) return hipErrorInvalidValue;
     const dim3 block(256);
-    if (!at && !bt) gemm_dma_kernel<T, false, false><<<grid, block, 0, s>>>(p);
-    else if (!at && bt) gemm_dma_kernel<T, false, true><<<grid, block, 0, s>>>(p);
-    else if (at && !bt) gemm_dma_kernel<T, true, false><<<grid, block, 0, s>>>(p);
-    else gemm_dma_kernel<T, true, true><<<grid, block, 0, s>>>(p);
-    return hipGetLastError();
+    auto launch = [&](dim3 g) {
+        if (!at && !bt) gemm_dma_kernel<T, false, false><<<g, block, 0, s>>>(p);
+        else if (!at && bt) gemm_dma_kernel<T, false, true><<<g, block, 0, s>>>(p);
+        else if (at && !bt) gemm_dma_kernel<T, true, false><<<g, block, 0, s>>>(p);
+        else gemm_dma_kernel<T, true, true><<<g, block, 0, s>>>(p);
+        return hipGetLastError();
+    };
+    const int64_t ntile = grid.x, W = dma_round<T>(), nk = p.k / DmaPanel<T, true>::BK;
+    p.tile0 = 0;
+    p.ksplit = 1;
+    p.part = nullptr;
+    if (p.krange == KR_ALL && W > 0 && ntile > W) {
+        const int64_t t = ntile % W, ks = std::min<int64_t>(t > 0 ? W / t : 0, nk / 8);
+        if (t > 0 && 2 * t <= W && ks >= 2) {
+            void* pw = nullptr;
+            hipError_t e = workspace(s, 4, (size_t)t * ks * 128 * 128 * sizeof(T), &pw);
+            if (e == hipErrorStreamCaptureUnsupported) return launch(grid);  // capture: no split rather than growth
+            if (e != hipSuccess) return e;
+            if ((e = launch(dim3((unsigned)(ntile - t)))) != hipSuccess) return e;
+            p.tile0 = (int)(ntile - t);
+            p.ksplit = (int)ks;
+            p.part = pw;
+            if ((e = launch(dim3((unsigned)(t * ks)))) != hipSuccess) return e;
+            dma_splitk_reduce_kernel<T><<<dim3((unsigned)t, 128 / 8), dim3(256), 0, s>>>(p);
+            return hipGetLastError();
+        }
+    }
+    return launch(grid);
 }
 
 // Split-precision path: pack op(A) and op(B)^T into bf16 planes, then run the

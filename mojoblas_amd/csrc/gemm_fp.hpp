@@ -482,9 +482,15 @@ __device__ inline void gemm_dma_tile(const GemmArgs<T, T, TC, TS>& p, void* lds)
     constexpr int BUF = PA::ELEMS + PB::ELEMS;
     // Tile enumeration, k-range and store mask as in gemm_fp_kernel
     // (triangle grids for syrk / syr2k, triangular operands for trmm).
-    int tm, tn;
-    if (p.tri_tiles) tri_tile_coords(p.tri_tiles, xcd_remap((int)blockIdx.x, (int)gridDim.x), p.tiles_m, tm, tn);
-    else tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
+    int tm, tn, chunk = 0;
+    if (p.ksplit > 1) {
+        chunk = (int)blockIdx.x % p.ksplit;
+        tile_coords_logical(p.tri_tiles, p.tile0 + (int)blockIdx.x / p.ksplit, p.tiles_m, p.tiles_n, tm, tn);
+    } else if (p.tri_tiles) {
+        tri_tile_coords(p.tri_tiles, xcd_remap((int)blockIdx.x, (int)gridDim.x), p.tiles_m, tm, tn);
+    } else {
+        tile_coords_kr(p.krange, p.tiles_m, p.tiles_n, tm, tn);
+    }
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -572,6 +578,11 @@ __device__ inline void gemm_dma_tile(const GemmArgs<T, T, TC, TS>& p, void* lds)
     };
     int kt0, kt1;
     kt_range(p.krange, tm, tn, BM, BN, BK, (int)(p.k / BK), kt0, kt1);
+    if (p.ksplit > 1) {  // this workgroup's share of the tile's k-tiles
+        const int len = kt1 - kt0, c0 = kt0 + (int)((int64_t)len * chunk / p.ksplit);
+        kt1 = kt0 + (int)((int64_t)len * (chunk + 1) / p.ksplit);
+        kt0 = c0;
+    }
     if (kt0 < kt1) MBH_DMAK(kt0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -591,6 +602,17 @@ __device__ inline void gemm_dma_tile(const GemmArgs<T, T, TC, TS>& p, void* lds)
 
     // Register r of block (i, j) holds C(m0 + wm0 + 16i + lane%16,
     // n0 + wn0 + 16j + M::col(r, lane)); the tile is interior.
+    if (p.ksplit > 1) {  // raw partial sum of the chunk, the whole block
+        MBH_GLOBAL TS* const part = (MBH_GLOBAL TS*)p.part + (size_t)blockIdx.x * BM * BN;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    part[(wm0 + 16 * i + (lane & 15)) + (wn0 + 16 * j + M::col(r, lane)) * BM] = acc[i][j][r];
+        return;
+    }
     const TS alpha = p.alpha, beta = p.beta;
     MBH_GLOBAL TC* const C = (MBH_GLOBAL TC*)p.C;
 #pragma unroll
@@ -612,6 +634,37 @@ template <typename T, bool AT, bool BT>
 __global__ __launch_bounds__(256) void gemm_dma_kernel(GemmArgs<T, T, T, T> p) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[DmaTileLds<T, T, T, AT, BT>::BYTES];
     gemm_dma_tile<T, T, T, AT, BT>(p, lds);
+}
+
+// Finishes the tiles of a split launch of gemm_dma_kernel: C = alpha * (sum
+// of the ksplit partial blocks, added in chunk order) + beta * C, under the
+// store mask.  Block (t, y): tile tile0 + t, columns 8y .. 8y + 7; a thread
+// sums four consecutive rows of one column.
+template <typename T>
+__global__ __launch_bounds__(256) void dma_splitk_reduce_kernel(GemmArgs<T, T, T, T> p) {
+    constexpr int BM = 128, BN = 128, TPC = BM / 4, CPB = 256 / TPC;
+    static_assert(TPC * CPB == 256 && BN % CPB == 0, "block shape");
+    const int t = (int)blockIdx.x;
+    int tm, tn;
+    tile_coords_logical(p.tri_tiles, p.tile0 + t, p.tiles_m, p.tiles_n, tm, tn);
+    const int i = 4 * ((int)threadIdx.x % TPC), j = CPB * (int)blockIdx.y + (int)threadIdx.x / TPC;
+    const MBH_GLOBAL T* part = (const MBH_GLOBAL T*)p.part + (size_t)t * p.ksplit * BM * BN + (size_t)j * BM + i;
+    T sum[4] = {T(0), T(0), T(0), T(0)};
+    for (int c = 0; c < p.ksplit; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sum[e] += part[(size_t)c * BM * BN + e];
+    const int64_t col = (int64_t)tn * BN + j;
+    if (col >= p.n) return;
+    MBH_GLOBAL T* const C = (MBH_GLOBAL T*)p.C;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t row = (int64_t)tm * BM + i + e;
+        if (row >= p.m || !tri_keep(p.tri, row, col)) continue;
+        MBH_GLOBAL T* cp = C + row + col * p.ldc;
+        T v = p.alpha * sum[e];
+        if (p.beta != T(0)) v += p.beta * *cp;
+        *cp = v;
+    }
 }
 
 }  // namespace mbh

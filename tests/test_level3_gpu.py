@@ -346,18 +346,21 @@ WHOLE_CASES = [("trmm", "L", "L", "N"), ("trmm", "L", "U", "N"), ("trmm", "R", "
                ("syrk", None, "L", "N"), ("syrk", None, "U", "T"), ("syr2k", None, "L", "N"), ("syr2k", None, "U", "T")]
 
 
-@pytest.mark.parametrize("routine,side,uplo,trans", WHOLE_CASES)
+@pytest.mark.parametrize("routine,side,uplo,trans,k", [c + (256,) for c in WHOLE_CASES]
+                         + [(r, None, u, t, 512) for r in ("syrk", "syr2k") for u, t in (("L", "N"), ("U", "T"))])
 @pytest.mark.parametrize("prec", ["fp32", "f64"])
-def test_whole_tile_products_sampled(routine, side, uplo, trans, prec):
+def test_whole_tile_products_sampled(routine, side, uplo, trans, k, prec):
     """4096-sized strmm / ssyrk / ssyr2k (and the fp64 forms): C is whole
     128 x 128 tiles and k a multiple of the k-tile, so the triangular grids
     (lower and upper), the four k-ranges and the store masks run on the
-    direct-to-LDS kernel.  4096 sampled entries against fp64 dot products
-    (bound (2k+2) u |alpha| |A||B| for fp32, 4k u64 for fp64); the
-    untouched triangle of C stays NaN."""
+    direct-to-LDS kernel.  With k = 512 the 528 triangle tiles leave a
+    partial last round on 512 resident workgroups, whose 16 tiles are split
+    along k into partial sums and reduced.  4096 sampled entries against
+    fp64 dot products (bound (2k+2) u |alpha| |A||B| for fp32, 4k u64 for
+    fp64); the untouched triangle of C stays NaN."""
     dt = torch.float64 if prec == "f64" else torch.float32
     g = torch.Generator(device="cuda").manual_seed(41)
-    n, k = 4096, 256
+    n = 4096
     rng = np.random.default_rng(42)
     kw = {} if prec == "f64" else {"precision": prec}
     alpha = 0.5
