@@ -267,22 +267,29 @@ int mbh_trsv_status(void* stream, int* failed);
 /* ---- variable-size batched gemm (no counterpart in the reference) --------
  * count independent problems C_p := alpha*A_p*B_p + beta*C_p (each with the
  * semantics of gemm("N","N", m_p, n_p, k_p, ...), gemm.mojo:56-124), run as
- * persistent kernels that pull C tiles from work queues (128x128 tiles
- * first, then 64x64; per tile class one queue per XCD holding whole
- * problems, drained by that XCD's workgroups first, then by any).
+ * persistent kernels, 128x128 tiles first, then 64x64.  When every problem
+ * with k > 0 has at least four elements in each operand, each workgroup
+ * walks a fixed list of C tiles (problems spread over the XCDs, then over
+ * each XCD's workgroups, longest-processing-time first) with four waves
+ * loading the operands two k-tiles ahead and four running the MFMAs;
+ * otherwise workgroups pull tiles from work queues (per tile class one queue
+ * per XCD holding whole problems, drained by that XCD's workgroups first,
+ * then by any).  Both give the same results.
  *  plan_create : host arrays of shapes and leading dimensions; validates every
  *                problem (MBH_E_ARG names the first bad one), assigns tile
- *                classes from the tuning table, splits the problems over the
- *                queues by longest-processing-time, orders each queue's tiles
- *                longest-first and stores the plan in HBM (allocates; once
- *                per shape list).
+ *                classes from the tuning table, builds the workgroup lists or
+ *                queues and stores the plan in HBM (allocates; once per shape
+ *                list).  MBH_VBATCH_KERNEL=lean in the environment selects
+ *                the work-queue kernels for every plan (for comparisons).
  *  exec        : device arrays of count device pointers; allocates nothing and
  *                never synchronises the host, so it can be captured in a
  *                hipGraph and replayed.  precision MBH_PREC_FP32 (fp32 MFMA) or
  *                MBH_PREC_BF16 (bf16 operands packed on the fly, fp32
  *                accumulate).  Calls sharing one plan must not overlap.
  *  plan_info   : out[8] = {count, tiles, workgroups, flops, problems on
- *                64x64 tiles, on 128x128 tiles, pack blocks, precision}.
+ *                64x64 tiles, on 128x128 tiles, kernel (1: warp-specialized
+ *                tiles on fixed per-workgroup lists, 0: work queues),
+ *                precision}.
  *  set_tuning  : for one precision, the class table (36 ints,
  *                [bucket(m)][bucket(n)], buckets <=16,32,64,128,256,larger;
  *                0 = 64x64 tiles, 1 = 128x128; NULL = compiled default) and

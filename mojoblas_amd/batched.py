@@ -45,7 +45,7 @@ class VBatchPlan:
     def info(self):
         out = np.zeros(8, np.float64)
         _check(_lib.lib().mbh_vbatch_plan_info(self._h, ctypes.c_void_p(out.ctypes.data)), "mbh_vbatch_plan_info")
-        keys = ["count", "tiles", "workgroups", "flops", "problems_64", "problems_128", "pack_blocks", "precision"]
+        keys = ["count", "tiles", "workgroups", "flops", "problems_64", "problems_128", "kernel", "precision"]
         return dict(zip(keys, out.tolist()))
 
     def exec(self, alpha, a_ptrs, b_ptrs, beta, c_ptrs, stream=None):

@@ -178,6 +178,16 @@ __device__ inline float4 vb_load4(const MBH_GLOBAL float* base, unsigned off) {
     return v;
 }
 
+// 16 bytes to / from a dword-aligned global address (one dwordx4 access).
+__device__ inline f32x4 vb_ld16(const MBH_GLOBAL float* p) {
+    f32x4 v;
+    __builtin_memcpy(&v, (const MBH_GLOBAL unsigned char*)p, 16);
+    return v;
+}
+__device__ inline void vb_st16(MBH_GLOBAL float* p, const f32x4& v) {
+    __builtin_memcpy((MBH_GLOBAL unsigned char*)p, &v, 16);
+}
+
 // v shifted down by s elements: out[e] = v[min(e + s, 3)], as a two-stage
 // shifter of selects (shift by 2, then by 1; s >= 3 shifts by 3).
 __device__ inline void vb_shift4(const float4& v, int s, float* out) {
@@ -531,6 +541,462 @@ __global__ __launch_bounds__(256, (VbLean<BF16, T>::MINWG)) void vb_gemm_lean_ke
     }
 }
 
+// ---- warp-specialized tiles -------------------------------------------------
+// One record per work item (a T x T tile of one problem), written at exec
+// time by vb_ws_fill_kernel from the plan's item list and the caller's
+// pointer arrays, so a workgroup reads an item with one 64-byte scalar load.
+struct VbWsRec {
+    const float* A;
+    const float* B;
+    float* C;
+    int m, n, K, lda, ldb, ldc, m0, n0, nkt, pad;
+};
+static_assert(sizeof(VbWsRec) == 64, "one 64-byte record per item");
+
+// What the loader waves tell the MFMA waves about the k-tile they staged.
+struct VbWsHdr {
+    float* Ct;      // C tile origin, C + m0 + n0 ldc
+    int ldc, mr, nr;  // rows / columns of C from the tile origin to the problem's edge
+    int nk;         // k values of the k-tile below K (0 for k = 0)
+    int last;       // the item's last k-tile: its C tile is stored after it
+    int end;        // no k-tile: the workgroup's list is done
+};
+
+__global__ void vb_ws_fill_kernel(const VbProblem* __restrict__ probs, const VbItem* __restrict__ items, int nitems,
+                                  const float* const* A, const float* const* B, float* const* C,
+                                  VbWsRec* __restrict__ recs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    const VbItem w = items[i];
+    const VbProblem pr = probs[w.prob];
+    const int T = vb_tile(pr.cls);
+    VbWsRec r;
+    r.A = A[w.prob];
+    r.B = B[w.prob];
+    r.C = C[w.prob];
+    r.m = (int)pr.m;
+    r.n = (int)pr.n;
+    r.K = (int)pr.k;
+    r.lda = (int)pr.lda;
+    r.ldb = (int)pr.ldb;
+    r.ldc = (int)pr.ldc;
+    r.m0 = (w.tile / pr.tiles_n) * T;
+    r.n0 = (w.tile % pr.tiles_n) * T;
+    r.nkt = max(1, (int)((pr.k + 31) / 32));
+    r.pad = 0;
+    recs[i] = r;
+}
+
+// T x T tiles on 512 threads: waves 0-3 run the MFMAs (2 x 2 waves of
+// T/2 x T/2, the lean tile's LDS layouts and fragment reads) and store C;
+// waves 4-7 load the operands, two 32-deep k-tiles ahead in two register
+// sets, and stage each k-tile into LDS (rounded to bf16 in BF16 mode) with a
+// header for the MFMA waves.  The waves that compute never wait for a load,
+// and a load has two k-steps to arrive.  Barriers order the two roles:
+//   step s: loaders issue the loads of step s + 2 (set s % 2) and stage step
+//           s + 1 (set (s + 1) % 2) into LDS buffer (s + 1) % 2; MFMA waves
+//           run step s from buffer s % 2; barrier.  At an item's last
+//           k-tile the MFMA waves pass a second barrier first and send the C
+//           tile out through buffer s % 2 (16-byte stores).
+// 128 registers per lane, four waves per SIMD: two workgroups per CU.
+// Every step issues the same loads (those a tile does not need read 16 bytes
+// of the record array or of A's first elements), so the loader's wait for
+// set (s + 1) % 2 is a fixed count that leaves step s + 2's loads in flight.
+// A workgroup walks its own list of items, recs[wsoff[b] .. wsoff[b + 1])
+// for block b, the list fixed by the plan (longest-processing-time over the
+// XCDs, then over each XCD's workgroups); the loaders read each record one
+// item ahead.  Every item takes at least one k-step (k = 0: a zero k-tile,
+// so C := beta C).
+template <bool BF16, int T>
+__global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __restrict__ recs,
+                                                             const int* __restrict__ wsoff, float alpha,
+                                                             float beta) {
+    typedef VbLean<BF16, T> L;
+    constexpr int NQ = L::NQ, TW = L::TW, SAF = L::SAF, SBF = L::SBF, SAH = L::SAH;
+    // A stage holds one k-tile, or at an item's end the MFMA waves' C blocks
+    // on their way out (4 KB per wave).
+    constexpr int SB = L::BUF > 16384 ? L::BUF : 16384;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[2 * SB];
+    __shared__ VbWsHdr hdr[2];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool loader = wave >= 4;
+    MBH_LDS unsigned char* const L0 = (MBH_LDS unsigned char*)lds;
+    auto lbuf = [&](int b) { return L0 + b * SB; };
+
+    // ---- loader waves ----
+    // A (m-contiguous columns): T/4 lanes cover a k-column's T rows, four
+    // rows each, so a wave's 16-byte load reads whole 4T-byte column runs
+    // (at any 4-byte alignment a run of 512 bytes touches 5 cache lines, a
+    // run of 128 bytes 2); load q of a lane is at k = akg + (1024/T) q.
+    // B (k-contiguous columns): lane group g8 holds k = 4 g8 .. + 3 of
+    // column cb + 32 q (the k-tile's 128-byte run of each column).
+    constexpr int ARG = T / 4, AKG = 1024 / T;
+    const int ltid = tid & 255;
+    const int arg = ltid % ARG, akg = ltid / ARG;  // A: row group, first k
+    const int g8 = lane & 7;                       // B: k group
+    const int cb = ltid >> 3;                      // B: column within a 32-column load
+    struct Set {
+        float4 va[NQ], vb[NQ];
+        int sa[NQ], sb[NQ];
+        int qb, krem, lda, last, end, ldc, mr, nr;
+        MBH_GLOBAL float* Ct;
+    };
+    const MBH_GLOBAL float* const nullp = (const MBH_GLOBAL float*)recs;
+    // s_waitcnt vmcnt(2 NQ) (expcnt and lgkmcnt at their maxima: no wait)
+    constexpr int VMWAIT = ((2 * NQ) & 15) | (((2 * NQ) >> 4) << 14) | (7 << 4) | (15 << 8);
+    int j = wsoff[blockIdx.x];
+    const int jend = wsoff[blockIdx.x + 1];
+    bool done = j >= jend;
+    int kt = 0;
+    VbWsRec cur{}, nxt{};
+    if (loader) {
+        if (!done) cur = recs[j];
+        if (j + 1 < jend) nxt = recs[j + 1];
+    }
+    auto nload = [&](int extent) { return min(NQ, (extent + 31) >> 5); };
+    // The next k-tile of the list into S: loads issued, not waited for.  S's
+    // previous loads completed before S was staged; the wait for all but the
+    // other set's 2 NQ loads only says so to the compiler (which would
+    // otherwise wait for every load in flight before reusing S's registers).
+    auto fetch = [&](Set& S) {
+        __builtin_amdgcn_s_waitcnt(VMWAIT);
+        if (!done && kt == cur.nkt) {
+            ++j;
+            if (j >= jend) {
+                done = true;
+            } else {
+                cur = nxt;
+                kt = 0;
+                if (j + 1 < jend) nxt = recs[j + 1];
+            }
+        }
+        const bool live = !done && cur.K > 0;
+        const int k0 = 32 * kt;
+        S.end = done;
+        S.qb = nload(cur.n - cur.n0);
+        S.krem = cur.K - k0;
+        S.lda = cur.lda;
+        S.last = kt == cur.nkt - 1;
+        S.ldc = cur.ldc;
+        S.mr = cur.m - cur.m0;
+        S.nr = cur.n - cur.n0;
+        S.Ct = (MBH_GLOBAL float*)cur.C + (cur.m0 + (int64_t)cur.n0 * cur.ldc);
+        // last element of each operand (k > 0: at least four elements each)
+        const int alast = (cur.K - 1) * cur.lda + cur.m - 1, blast = (cur.n - 1) * cur.ldb + cur.K - 1;
+        const int kk = k0 + 4 * g8;
+        const bool arow_ok = live && 4 * arg < S.mr;  // lanes past the edge read A[0 .. 3]
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const bool oka = arow_ok, okb = live && q < S.qb;
+            const int ia = cur.m0 + 4 * arg + (k0 + akg + AKG * q) * cur.lda, ja = min(ia, alast - 3);
+            S.sa[q] = ia - ja;
+            S.va[q] = vb_load4(live ? (const MBH_GLOBAL float*)cur.A : nullp, oka ? 4u * (unsigned)ja : 0u);
+            const int ib = kk + min(cur.n0 + cb + 32 * q, cur.n - 1) * cur.ldb, jb = min(ib, blast - 3);
+            S.sb[q] = ib - jb;
+            S.vb[q] = vb_load4(okb ? (const MBH_GLOBAL float*)cur.B : nullp, okb ? 4u * (unsigned)jb : 0u);
+        }
+        if (!done) ++kt;
+    };
+    // S's k-tile into LDS buffer b and its header.  S's loads are the older
+    // of the two sets in flight: one wait for all but the newer set's 2 NQ
+    // loads (s_waitcnt with only vmcnt set), whatever loads the staging then
+    // reads, so no later reuse of S's registers waits again.
+    auto stage = [&](int b, const Set& S) {
+        MBH_LDS unsigned char* buf = lbuf(b);
+        __builtin_amdgcn_s_waitcnt(VMWAIT);
+        if (!S.end) {
+            float ra[4 * NQ], rb[4 * NQ];
+            if (S.krem > 32 && S.lda >= 3) {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    ra[4 * q] = S.va[q].x, ra[4 * q + 1] = S.va[q].y, ra[4 * q + 2] = S.va[q].z, ra[4 * q + 3] = S.va[q].w;
+                    rb[4 * q] = S.vb[q].x, rb[4 * q + 1] = S.vb[q].y, rb[4 * q + 2] = S.vb[q].z, rb[4 * q + 3] = S.vb[q].w;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    float a4[4], b4[4];
+                    vb_shift4(S.va[q], S.sa[q], a4);
+                    vb_shift4(S.vb[q], S.sb[q], b4);
+                    const bool kva = akg + AKG * q < S.krem;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        ra[4 * q + e] = kva ? a4[e] : 0.0f;
+                        rb[4 * q + e] = 4 * g8 + e < S.krem ? b4[e] : 0.0f;
+                    }
+                }
+            }
+            if constexpr (BF16) {
+                typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+                MBH_LDS unsigned char* arow = buf + akg * SAH + 8 * arg;
+                MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + L::A_BYTES);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    *(MBH_LDS u32x2*)(arow + AKG * SAH * q) = u32x2{bf16x2_rn_bits(ra[4 * q], ra[4 * q + 1]),
+                                                                    bf16x2_rn_bits(ra[4 * q + 2], ra[4 * q + 3])};
+                    if (q < S.qb) {
+                        const int jj = cb + 32 * q, k4 = 4 * g8;
+                        *(MBH_LDS u32x2*)(b16 + jj * 32 + (((k4 >> 3) ^ lds_swz(jj)) << 3) + (k4 & 7)) =
+                            u32x2{bf16x2_rn_bits(rb[4 * q], rb[4 * q + 1]), bf16x2_rn_bits(rb[4 * q + 2], rb[4 * q + 3])};
+                    }
+                }
+            } else {
+                MBH_LDS float* as = (MBH_LDS float*)buf + akg * SAF + 4 * arg;
+                MBH_LDS float* bs = (MBH_LDS float*)(buf + L::A_BYTES) + cb * SBF + 4 * g8;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    *(MBH_LDS f32x4*)(as + AKG * SAF * q) = f32x4{ra[4 * q], ra[4 * q + 1], ra[4 * q + 2], ra[4 * q + 3]};
+                    if (q < S.qb)
+                        *(MBH_LDS f32x4*)(bs + 32 * q * SBF) = f32x4{rb[4 * q], rb[4 * q + 1], rb[4 * q + 2], rb[4 * q + 3]};
+                }
+            }
+        }
+        if (ltid == 0) {
+            VbWsHdr h;
+            h.Ct = (float*)S.Ct;
+            h.ldc = S.ldc;
+            h.mr = S.mr;
+            h.nr = S.nr;
+            h.nk = min(32, max(0, S.krem));
+            h.last = S.end ? 0 : S.last;
+            h.end = S.end;
+            hdr[b] = h;
+        }
+    };
+
+    // ---- MFMA waves ----
+    const int wm0 = (wave & 1) * (T / 2), wn0 = ((wave >> 1) & 1) * (T / 2);
+    const int lr = lane & 31, lh = lane >> 5;
+    // MFMAs of one k-tile over the wave's BI x BJ blocks below the tile's
+    // edge (compile-time subsets, so no MFMA is predicated); only the MFMA
+    // steps holding k < K run (nk: k values of the k-tile below K).
+    auto compute = [&](f32x16 (&acc)[TW][TW], const MBH_LDS unsigned char* buf, int nk, auto bi_c, auto bj_c) {
+        constexpr int BI = decltype(bi_c)::value, BJ = decltype(bj_c)::value;
+        if constexpr (BI == 0 || BJ == 0) {
+            return;
+        } else if constexpr (BF16) {
+            typedef short v4i16 __attribute__((ext_vector_type(4)));
+            const int G = lane >> 4, t = lane & 15;
+            const MBH_LDS unsigned char* ab =
+                buf + (8 * (G >> 1) + (t >> 2)) * SAH + 2 * (wm0 + 16 * (G & 1) + 4 * (t & 3));
+            const MBH_LDS u32x4* b4 = (const MBH_LDS u32x4*)(buf + L::A_BYTES);
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                if (ks > 0 && nk <= 16) break;
+                bf16x8 ah[BI], bh[BJ];
+#pragma unroll
+                for (int i = 0; i < BI; ++i) {
+                    const MBH_LDS unsigned char* p = ab + 16 * ks * SAH + 64 * i;
+                    const v4i16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((MBH_LDS v4i16*)p);
+                    const v4i16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((MBH_LDS v4i16*)(p + 4 * SAH));
+                    ah[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+                }
+#pragma unroll
+                for (int jj = 0; jj < BJ; ++jj) {
+                    const int row = wn0 + 32 * jj + lr;
+                    bh[jj] = __builtin_bit_cast(bf16x8, (u32x4)b4[row * 4 + ((2 * ks + lh) ^ lds_swz(row))]);
+                }
+#pragma unroll
+                for (int i = 0; i < BI; ++i)
+#pragma unroll
+                    for (int jj = 0; jj < BJ; ++jj)
+                        acc[i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[jj], ah[i], acc[i][jj], 0, 0, 0);
+            }
+        } else {
+            // 8-deep k-blocks c (v_mfma_f32_32x32x2f32 over k = 8c + 4h + e,
+            // lane half h), the fragments of block c + 1 read before block
+            // c's MFMAs (always: rows past K hold zeros), MFMAs only for
+            // blocks holding k < K.
+            const MBH_LDS float* as = (const MBH_LDS float*)buf + 4 * lh * SAF + wm0 + lr;
+            const MBH_LDS float* bs = (const MBH_LDS float*)(buf + L::A_BYTES) + (wn0 + lr) * SBF + 4 * lh;
+            const int nc = nk >= 32 ? 4 : (nk + 7) >> 3;
+            f32x4 bq[2][BJ];
+            float af[2][4][BI];
+            auto frag = [&](int c, f32x4 (&q)[BJ], float (&a)[4][BI]) {
+#pragma unroll
+                for (int jj = 0; jj < BJ; ++jj) q[jj] = *(const MBH_LDS f32x4*)(bs + 32 * jj * SBF + 8 * c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int i = 0; i < BI; ++i) a[e][i] = as[(8 * c + e) * SAF + 32 * i];
+            };
+            frag(0, bq[0], af[0]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (c < 3) frag(c + 1, bq[(c + 1) & 1], af[(c + 1) & 1]);
+                __builtin_amdgcn_sched_barrier(0);
+                if (c < nc) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+#pragma unroll
+                        for (int i = 0; i < BI; ++i)
+#pragma unroll
+                            for (int jj = 0; jj < BJ; ++jj)
+                                acc[i][jj] = __builtin_amdgcn_mfma_f32_32x32x2f32(bq[c & 1][jj][e], af[c & 1][e][i],
+                                                                                  acc[i][jj], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    };
+    // C^T accumulators: register r of block (i, jj), lane l holds C(wm0 +
+    // 32i + l%32, wn0 + 32jj + c(r) + 4(l/32)) from the tile origin Ct, c(r)
+    // = 8(r/4) + r%4.  Each block goes out through the wave's 4 KB of the
+    // stage buffer the item's last k-tile came from (every MFMA wave is past
+    // it): written as [column][32 rows], read back as four rows of one column
+    // per lane, and stored 16 bytes at a time (8 lanes per 128-byte column
+    // segment); both LDS passes take one base address per lane and constant
+    // offsets.  mr /
+    // nr: rows / columns left for the wave; blocks past the edge store
+    // nothing, partial rows and columns store per element.
+    auto epilogue = [&](const f32x16 (&acc)[TW][TW], MBH_LDS unsigned char* buf, MBH_GLOBAL float* Ct, int ldc,
+                        int mr, int nr, auto bi_c, auto bj_c, auto beta_c) {
+        constexpr int BI = decltype(bi_c)::value, BJ = decltype(bj_c)::value;
+        constexpr bool BETA = decltype(beta_c)::value;  // beta != 0: C is read
+        MBH_LDS float* reg = (MBH_LDS float*)(buf + 4096 * wave);
+        MBH_LDS float* wbase = reg + 128 * lh + lr;  // + 32 c(r): the lane's writes
+        const int r4 = 4 * (lane & 7), cl = lane >> 3;
+        const MBH_LDS float* rbase = reg + 32 * cl + r4;  // + 256 t: the lane's reads
+#pragma unroll
+        for (int i = 0; i < BI; ++i) {
+#pragma unroll
+            for (int jj = 0; jj < BJ; ++jj) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    wbase[32 * (8 * (r >> 2) + (r & 3))] = acc[i][jj][r];
+                const int rows = mr - 32 * i - r4;  // rows of the lane's four below the edge
+                auto ok = [&](int t) { return 32 * jj + 8 * t + cl < nr && rows > 0; };
+                auto cp = [&](int t) {
+                    return Ct + (wm0 + 32 * i + r4) + (int64_t)(wn0 + 32 * jj + 8 * t + cl) * ldc;
+                };
+                // beta != 0: the block's C loads all issue before its stores
+                f32x4 c[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    c[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                    if (BETA && ok(t)) {
+                        if (rows >= 4) {
+                            c[t] = vb_ld16(cp(t));
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 3; ++e)
+                                if (e < rows) c[t][e] = cp(t)[e];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    f32x4 v = *(const MBH_LDS f32x4*)(rbase + 256 * t);
+                    v = BETA ? alpha * v + beta * c[t] : alpha * v;
+                    if (!ok(t)) continue;
+                    if (rows >= 4) {
+                        vb_st16(cp(t), v);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 3; ++e)
+                            if (e < rows) cp(t)[e] = v[e];
+                    }
+                }
+            }
+        }
+    };
+    // One item on the MFMA waves, with the wave's blocks fixed for the item:
+    // its k-steps from LDS buffer b on (one barrier each), its C tile after
+    // the last; returns the buffer of the next step.  The accumulators live
+    // only within an item.
+    auto run_item = [&](int b, auto bi_c, auto bj_c, auto beta_c) {
+        f32x16 acc[TW][TW];
+#pragma unroll
+        for (int i = 0; i < TW; ++i)
+#pragma unroll
+            for (int jj = 0; jj < TW; ++jj)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[i][jj][r] = 0.0f;
+        for (;;) {
+            const int nk = __builtin_amdgcn_readfirstlane(hdr[b].nk);
+            const int last = __builtin_amdgcn_readfirstlane(hdr[b].last);
+            compute(acc, lbuf(b), nk, bi_c, bj_c);
+            if (last) {
+                const uint64_t cbits = (uint64_t)hdr[b].Ct;
+                const unsigned clo = __builtin_amdgcn_readfirstlane((unsigned)cbits);
+                const unsigned chi = __builtin_amdgcn_readfirstlane((unsigned)(cbits >> 32));
+                MBH_GLOBAL float* Ct = (MBH_GLOBAL float*)(((uint64_t)chi << 32) | clo);
+                const int ldc = __builtin_amdgcn_readfirstlane(hdr[b].ldc);
+                const int mr = __builtin_amdgcn_readfirstlane(hdr[b].mr) - wm0;
+                const int nr = __builtin_amdgcn_readfirstlane(hdr[b].nr) - wn0;
+                __syncthreads();  // every MFMA wave is done with buffer b
+                epilogue(acc, lbuf(b), Ct, ldc, mr, nr, bi_c, bj_c, beta_c);
+            }
+            __syncthreads();
+            b ^= 1;
+            if (last) return b;
+        }
+    };
+
+    // The two roles run separate loops (so neither holds the other's
+    // registers) with the same barriers: one after the prologue and one per
+    // step, steps taken in pairs; both leave at the first even step whose
+    // header says end (a list that ends on an odd step takes one empty step).
+    // The loader's loop has no exit between its two halves, so the compiler
+    // sees each set's loads in flight across exactly one half.
+    if (loader) {
+        Set S0, S1;
+        fetch(S0);
+        fetch(S1);
+        stage(0, S0);
+        __syncthreads();
+        // (steps that end an item take a second barrier: the MFMA waves'
+        // C blocks pass through the stage buffer the step read)
+        while (!__builtin_amdgcn_readfirstlane(hdr[0].end)) {
+            fetch(S0);
+            stage(1, S1);
+            if (__builtin_amdgcn_readfirstlane(hdr[0].last)) __syncthreads();
+            __syncthreads();
+            fetch(S1);
+            stage(0, S0);
+            if (__builtin_amdgcn_readfirstlane(hdr[1].last)) __syncthreads();
+            __syncthreads();
+        }
+    } else {
+        typedef std::integral_constant<int, 0> zero;
+        typedef std::integral_constant<int, 1> one;
+        typedef std::integral_constant<int, TW> tw;
+        // The walk, in two copies: beta == 0 (C never read) and beta != 0.
+        auto walk = [&](auto beta_c) {
+            for (int b = 0;;) {
+                if (__builtin_amdgcn_readfirstlane(hdr[b].end)) {
+                    if (b == 0) break;
+                    __syncthreads();  // the empty odd step
+                    b = 0;
+                    continue;
+                }
+                // blocks of the wave below the item's edge (bf16: all of
+                // them, its MFMAs cost little; those past the edge are never
+                // stored)
+                const int mr = __builtin_amdgcn_readfirstlane(hdr[b].mr) - wm0;
+                const int nr = __builtin_amdgcn_readfirstlane(hdr[b].nr) - wn0;
+                const int bi = min(TW, max(0, (mr + 31) >> 5)), bj = min(TW, max(0, (nr + 31) >> 5));
+                if (BF16 || (bi == TW && bj == TW))
+                    b = run_item(b, tw(), tw(), beta_c);
+                else if (bi == 0 || bj == 0)
+                    b = run_item(b, zero(), zero(), beta_c);
+                else if (TW == 2 && bi == 1 && bj == 2)
+                    b = run_item(b, one(), tw(), beta_c);
+                else if (TW == 2 && bi == 2 && bj == 1)
+                    b = run_item(b, tw(), one(), beta_c);
+                else
+                    b = run_item(b, one(), one(), beta_c);
+            }
+        };
+        __syncthreads();
+        if (beta == 0.0f)
+            walk(std::false_type());
+        else
+            walk(std::true_type());
+    }
+}
+
 // A problem can take the lean tile when its element offsets (with the
 // k-tile and tile overhang the tile computes before clamping) fit in int32
 // and the byte offsets of the elements it loads fit in 32 bits.
@@ -539,6 +1005,15 @@ bool vb_lean_ok(const VbProblem& q) {
     return q.m < l30 && q.n < l30 && q.k < l30 && q.lda * (q.k + 128) + q.m + 128 < l31 &&
            q.ldb * (q.n + 128) + q.k + 128 < l31 && q.ldc * (q.n + 128) + q.m + 128 < l31 &&
            q.lda * q.k < l30 && q.ldb * q.n < l30;  // byte offsets of loaded elements fit 32 bits
+}
+
+// The warp-specialized tile also needs four elements in each operand of a
+// problem with k > 0 (its 16-byte loads are clamped to the operand's last
+// four elements).
+bool vb_ws_ok(const VbProblem& q) {
+    if (!vb_lean_ok(q)) return false;
+    if (q.m == 0 || q.n == 0 || q.k == 0) return true;
+    return (q.k - 1) * q.lda + q.m >= 4 && (q.n - 1) * q.ldb + q.k >= 4;
 }
 
 // MAXCLS: highest tile class compiled in.  The 64x64-only variant needs far
@@ -734,6 +1209,12 @@ struct VbPlan {
     // Lean path: items [0, nitems1) are 128x128 tiles (counters and queue
     // offsets of class 0, grid grid1), the rest 64x64 tiles (class 1, grid).
     int nitems1 = 0, grid1 = 0;
+    // Warp-specialized path: items in per-workgroup lists (the 128x128 lists
+    // first), wsoff[0 .. grid1] and wsoff[grid1 + 1 .. grid1 + 1 + grid] the
+    // absolute list offsets of the two launches, recs one record per item.
+    bool ws = false;
+    int* wsoff = nullptr;
+    VbWsRec* recs = nullptr;
 };
 
 }  // namespace mbh
@@ -810,6 +1291,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         double cost;
     };
     int64_t ntiles = 0;
+    bool all_ws = true;
     for (int64_t p = 0; p < count; ++p) {
         VbProblem& q = probs[(size_t)p];
         q.m = m[p];
@@ -826,6 +1308,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         if (q.m == 0 || q.n == 0) continue;
         plan->class_count[q.cls]++;
         if (!vb_lean_ok(q)) plan->lean = false;
+        if (!vb_ws_ok(q)) all_ws = false;
         ntiles += tm * tn;
     }
     if (ntiles > (int64_t)INT32_MAX) {
@@ -864,7 +1347,82 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
     std::vector<VbItem> ditems;
     ditems.reserve((size_t)ntiles);
     std::vector<int> qoff(2 * (VB_QUEUES + 1), 0);
-    for (int c = 0; c < 2; ++c) {
+    int dev = 0, ncu = 256;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+    }
+    const bool bf = precision == MBH_PREC_BF16;
+    // Warp-specialized path unless MBH_VBATCH_KERNEL=lean asks for the
+    // lean one (both give the same results; the switch is for comparisons).
+    const char* kenv = getenv("MBH_VBATCH_KERNEL");
+    plan->ws = use_lean && all_ws && !(kenv && strcmp(kenv, "lean") == 0);
+    std::vector<int> wsoff;
+    if (plan->ws) {
+        // A persistent grid of the workgroups that stay resident (at most
+        // wgs per CU); each workgroup gets a fixed list.  Cost of an item:
+        // its k-steps plus about one for the C tile and the item switch.
+        auto ws_cost = [&](int p) { return (double)std::max<int64_t>(1, (probs[(size_t)p].k + 31) / 32) + 1.0; };
+        for (int c = 0; c < 2; ++c) {
+            const void* kfn = c == 0 ? (bf ? (const void*)vb_gemm_ws_kernel<true, 128> : (const void*)vb_gemm_ws_kernel<false, 128>)
+                                     : (bf ? (const void*)vb_gemm_ws_kernel<true, 64> : (const void*)vb_gemm_ws_kernel<false, 64>);
+            int resident = 0, w = wgs;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kfn, 512, 0) == hipSuccess && resident > 0)
+                w = std::min(w, resident);
+            const int G = (int)std::min<int64_t>((int64_t)queues[c].size(), (int64_t)ncu * std::max(w, 1));
+            (c == 0 ? plan->grid1 : plan->grid) = G;
+            // Problems onto the XCDs (workgroup b runs on XCD b % 8), longest
+            // first onto the least loaded; then each XCD's items, longest
+            // first, onto its least loaded workgroup.
+            const int X = std::min(VB_QUEUES, std::max(G, 1));
+            std::vector<std::vector<Item>> xitems(X);
+            {
+                struct Job {
+                    size_t first;
+                    int64_t tiles;
+                    double cost;
+                };
+                std::vector<Job> jobs;
+                for (size_t i = 0; i < queues[c].size();) {  // the tiles of a problem are consecutive
+                    const int p = queues[c][i].prob;
+                    const int64_t nt = tiles_of(p);
+                    jobs.push_back({i, nt, (double)nt * ws_cost(p)});
+                    i += (size_t)nt;
+                }
+                std::stable_sort(jobs.begin(), jobs.end(), [](const Job& a, const Job& b) { return a.cost > b.cost; });
+                std::vector<double> xload((size_t)X, 0.0);
+                for (const Job& jb : jobs) {
+                    const int x = (int)(std::min_element(xload.begin(), xload.end()) - xload.begin());
+                    xload[(size_t)x] += jb.cost;
+                    for (int64_t t = 0; t < jb.tiles; ++t) {
+                        Item it = queues[c][jb.first + (size_t)t];
+                        it.cost = ws_cost(it.prob);
+                        xitems[(size_t)x].push_back(it);
+                    }
+                }
+            }
+            std::vector<std::vector<VbItem>> lists((size_t)G);
+            for (int x = 0; x < X && G > 0; ++x) {
+                std::vector<int> members;
+                for (int b = x; b < G; b += VB_QUEUES) members.push_back(b);
+                std::vector<double> wl(members.size(), 0.0);
+                std::stable_sort(xitems[(size_t)x].begin(), xitems[(size_t)x].end(),
+                                 [](const Item& a, const Item& b) { return a.cost > b.cost; });
+                for (const Item& it : xitems[(size_t)x]) {
+                    const size_t u = (size_t)(std::min_element(wl.begin(), wl.end()) - wl.begin());
+                    wl[u] += it.cost;
+                    lists[(size_t)members[u]].push_back({it.prob, it.tile});
+                }
+            }
+            for (int b = 0; b < G; ++b) {
+                wsoff.push_back((int)ditems.size());
+                for (const VbItem& it : lists[(size_t)b]) ditems.push_back(it);
+            }
+            wsoff.push_back((int)ditems.size());
+        }
+    }
+    for (int c = 0; c < 2 && !plan->ws; ++c) {
         if (!use_lean) {
             longest_first(queues[c]);
             for (const Item& it : queues[c]) ditems.push_back({it.prob, it.tile});
@@ -901,12 +1459,6 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
     }
     plan->nitems = (int)ditems.size();
     plan->nitems1 = use_lean ? (int)queues[0].size() : 0;
-    int dev = 0, ncu = 256;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
-    }
     // A persistent grid never asks for more workgroups per CU than the kernel
     // it launches can keep resident (late workgroups would only find the
     // queue drained).
@@ -916,8 +1468,9 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
             w = std::min(w, resident);
         return std::min(nit, ncu * w);  // 0 for an empty queue (no launch)
     };
-    const bool bf = precision == MBH_PREC_BF16;
-    if (use_lean) {
+    if (plan->ws) {
+        // grids set with the lists
+    } else if (use_lean) {
         plan->grid1 = grid_for(bf ? (const void*)vb_gemm_lean_kernel<true, 128> : (const void*)vb_gemm_lean_kernel<false, 128>,
                                plan->nitems1);
         plan->grid = grid_for(bf ? (const void*)vb_gemm_lean_kernel<true, 64> : (const void*)vb_gemm_lean_kernel<false, 64>,
@@ -953,6 +1506,15 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
             hipSuccess)
             return fail(e, "vbatch plan: items copy");
     }
+    if (plan->ws) {
+        if ((e = hipMalloc(&plan->wsoff, wsoff.size() * sizeof(int))) != hipSuccess)
+            return fail(e, "vbatch plan: workgroup lists");
+        if ((e = hipMemcpy(plan->wsoff, wsoff.data(), wsoff.size() * sizeof(int), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return fail(e, "vbatch plan: workgroup lists copy");
+        if (!ditems.empty() && (e = hipMalloc(&plan->recs, sizeof(VbWsRec) * ditems.size())) != hipSuccess)
+            return fail(e, "vbatch plan: item records");
+    }
     *out = (mbh_vbatch_plan)plan;
     return 0;
 }
@@ -969,7 +1531,7 @@ int mbh_vbatch_plan_info(mbh_vbatch_plan handle, double* info) {
     info[3] = plan->flops;
     info[4] = (double)plan->class_count[0];
     info[5] = (double)plan->class_count[1];
-    info[6] = 0.0;  // conversion pass work units (none: bf16 staging is fused)
+    info[6] = plan->ws ? 1.0 : 0.0;  // 1: warp-specialized tiles on fixed lists, 0: work queues
     info[7] = (double)plan->prec;
     return 0;
 }
@@ -991,9 +1553,31 @@ int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* co
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : hip_fail(e, "vbatched scale");
     }
-    vb_reset_kernel<<<1, 64, 0, s>>>(plan->ctr);
     const dim3 grid((unsigned)plan->grid), block(256);
     const bool mixed = plan->class_count[1] > 0;
+    if (plan->ws) {
+        const bool bf = plan->prec == MBH_PREC_BF16;
+        vb_ws_fill_kernel<<<dim3((unsigned)((plan->nitems + 255) / 256)), 256, 0, s>>>(
+            plan->probs, plan->items, plan->nitems, a, b, c, plan->recs);
+        const int* off0 = plan->wsoff + plan->grid1 + 1;
+        if (plan->grid1 > 0) {
+            if (bf)
+                vb_gemm_ws_kernel<true, 128><<<dim3((unsigned)plan->grid1), 512, 0, s>>>(plan->recs, plan->wsoff,
+                                                                                          alpha, beta);
+            else
+                vb_gemm_ws_kernel<false, 128><<<dim3((unsigned)plan->grid1), 512, 0, s>>>(plan->recs, plan->wsoff,
+                                                                                           alpha, beta);
+        }
+        if (plan->grid > 0) {
+            if (bf)
+                vb_gemm_ws_kernel<true, 64><<<grid, 512, 0, s>>>(plan->recs, off0, alpha, beta);
+            else
+                vb_gemm_ws_kernel<false, 64><<<grid, 512, 0, s>>>(plan->recs, off0, alpha, beta);
+        }
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? 0 : hip_fail(e, "vbatched exec");
+    }
+    vb_reset_kernel<<<1, 64, 0, s>>>(plan->ctr);
     if (plan->lean) {
         // 128x128 items first (the longer ones), then the 64x64 queue.
         const bool bf = plan->prec == MBH_PREC_BF16;
@@ -1044,6 +1628,8 @@ int mbh_vbatch_plan_destroy(mbh_vbatch_plan handle) {
     if (plan->items) (void)hipFree(plan->items);
     if (plan->ctr) (void)hipFree(plan->ctr);
     if (plan->qoff) (void)hipFree(plan->qoff);
+    if (plan->wsoff) (void)hipFree(plan->wsoff);
+    if (plan->recs) (void)hipFree(plan->recs);
     delete plan;
     return e == hipSuccess ? 0 : hip_fail(e, "mbh_vbatch_plan_destroy");
 }

@@ -7,7 +7,7 @@
 #define VB_BUCKETS 6
 
 // [0] = MBH_PREC_FP32, [1] = MBH_PREC_BF16
-static const int VB_DEFAULT_WGS_PER_CU[2] = {6, 4};
+static const int VB_DEFAULT_WGS_PER_CU[2] = {8, 6};
 
 static const int VB_DEFAULT_CLASS[2][VB_BUCKETS][VB_BUCKETS] = {{  // FP32
     {0, 0, 0, 0, 0, 0},
@@ -17,10 +17,10 @@ static const int VB_DEFAULT_CLASS[2][VB_BUCKETS][VB_BUCKETS] = {{  // FP32
     {0, 0, 0, 1, 1, 1},
     {0, 0, 0, 1, 1, 1},
 }, {  // BF16
-    {1, 1, 1, 1, 1, 1},
-    {1, 1, 1, 1, 1, 1},
-    {1, 1, 1, 1, 1, 1},
-    {1, 1, 1, 1, 1, 1},
-    {1, 1, 1, 1, 1, 1},
-    {1, 1, 1, 1, 1, 1},
+    {0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 1, 1, 1},
+    {0, 0, 0, 1, 1, 1},
+    {0, 0, 0, 1, 1, 1},
 }};

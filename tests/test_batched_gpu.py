@@ -75,6 +75,28 @@ def _cases(tables):
     return [pytest.param(p, (t, p), id="%s-%s" % (p, t)) for p in ("fp32", "bf16") for t in tables]
 
 
+# Kernel path: "ws" = warp-specialized tiles on fixed per-workgroup lists
+# (the default when every problem with k > 0 has at least four elements in
+# each operand), "lean" = the work-queue tiles (MBH_VBATCH_KERNEL=lean, and
+# the path for batches holding smaller operands).
+@pytest.fixture(params=["ws", "lean"])
+def path(request, monkeypatch):
+    if request.param == "lean":
+        monkeypatch.setenv("MBH_VBATCH_KERNEL", "lean")
+    else:
+        monkeypatch.delenv("MBH_VBATCH_KERNEL", raising=False)
+    return request.param
+
+
+def _ws_k(k, path):
+    """k >= 4 where k > 0, so that a ws batch is eligible for the ws kernel."""
+    return np.where((k > 0) & (k < 4), 4, k) if path == "ws" else k
+
+
+def _assert_path(plan, path):
+    assert plan.info()["kernel"] == (1.0 if path == "ws" else 0.0), plan.info()
+
+
 @pytest.mark.parametrize("prec,table", _cases(TABLES), indirect=["table"])
 def test_ragged_batch_vs_oracle(prec, table):
     rng = np.random.default_rng(40)
@@ -96,8 +118,36 @@ def test_ragged_batch_vs_oracle(prec, table):
     plan.close()
 
 
+@pytest.mark.parametrize("prec,table", _cases(("default", "all64", "all128")), indirect=["table"])
+def test_ragged_ws_lists_vs_oracle(prec, table):
+    """The warp-specialized path on a ragged batch (m, n from 1, k from 4,
+    empty and k == 0 problems), alpha, beta != 0, every problem in full;
+    more problems than workgroups on some lists and empty lists on others."""
+    rng = np.random.default_rng(46)
+    count = 700
+    m = np.floor(np.exp(rng.uniform(0, np.log(400), count))).astype(np.int64)
+    n = np.floor(np.exp(rng.uniform(0, np.log(400), count))).astype(np.int64)
+    k = np.floor(np.exp(rng.uniform(np.log(4), np.log(400), count))).astype(np.int64)
+    m[3], n[5], k[7], k[11] = 0, 0, 0, 0
+    g = torch.Generator(device="cuda").manual_seed(47)
+    bufs, offs, ptrs = batched.pack_problems(torch, m, n, k, gen=g, fill="normal")
+    bufs[2].uniform_(-1, 1, generator=g)
+    c0 = bufs[2].cpu().numpy().copy()
+    plan = batched.VBatchPlan(m, n, k, precision=prec)
+    assert plan.info()["kernel"] == 1.0
+    plan.exec(1.25, ptrs[0], ptrs[1], -0.5, ptrs[2])
+    torch.cuda.synchronize()
+    _check_problems(prec, m, n, k, bufs, offs, 1.25, -0.5, c0, range(count))
+    # a second exec of the same plan (the records are rewritten each exec)
+    bufs[2].copy_(torch.from_numpy(c0))
+    plan.exec(1.25, ptrs[0], ptrs[1], -0.5, ptrs[2])
+    torch.cuda.synchronize()
+    _check_problems(prec, m, n, k, bufs, offs, 1.25, -0.5, c0, range(count))
+    plan.close()
+
+
 @pytest.mark.parametrize("prec,table", _cases(("all64", "all128")), indirect=["table"])
-def test_tile_edges(prec, table):
+def test_tile_edges(prec, table, path):
     """Every m and n around the 32-row blocks and the 64 / 128 tiles (a load,
     MFMA block or store is skipped when it lies wholly past the edge), k
     around the 8- and 16-deep steps and the 32-deep k-tiles, operands of
@@ -109,11 +159,13 @@ def test_tile_edges(prec, table):
     n = np.array([edges[(7 * i + 3) % len(edges)] for i in range(160)], np.int64)
     k = np.array([ks[(5 * i + 1) % len(ks)] for i in range(160)], np.int64)
     rng.shuffle(k)
+    k = _ws_k(k, path)
     g = torch.Generator(device="cuda").manual_seed(45)
     bufs, offs, ptrs = batched.pack_problems(torch, m, n, k, gen=g, fill="normal")
     bufs[2].uniform_(-1, 1, generator=g)
     c0 = bufs[2].cpu().numpy().copy()
     plan = batched.VBatchPlan(m, n, k, precision=prec)
+    _assert_path(plan, path)
     plan.exec(0.75, ptrs[0], ptrs[1], 1.5, ptrs[2])
     torch.cuda.synchronize()
     _check_problems(prec, m, n, k, bufs, offs, 0.75, 1.5, c0, range(len(m)))
@@ -126,7 +178,7 @@ def test_tile_edges(prec, table):
 
 
 @pytest.mark.parametrize("prec,table", _cases(("default", "all64", "mixed")), indirect=["table"])
-def test_config4_4096_problems(prec, table):
+def test_config4_4096_problems(prec, table, path):
     """Config size: 4096 N,N problems, m, n, k log-uniform in [16, 512],
     U[-1,1], alpha=1, beta=0, ld = rows.  Every problem is priced on 8 sampled
     entries by the oracle, every 64th problem in full."""
@@ -136,6 +188,7 @@ def test_config4_4096_problems(prec, table):
     bufs[2].fill_(float("nan"))  # beta == 0: C is never read
     c0 = np.zeros(bufs[2].numel(), np.float32)
     plan = batched.VBatchPlan(m, n, k, precision=prec)
+    _assert_path(plan, path)
     plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2])
     torch.cuda.synchronize()
     _check_problems(prec, m, n, k, bufs, offs, 1.0, 0.0, c0, range(len(m)), sample=8)
@@ -144,11 +197,12 @@ def test_config4_4096_problems(prec, table):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_graph_capture_replays_match_eager(prec):
+def test_graph_capture_replays_match_eager(prec, path):
     m, n, k = batched.config4_shapes(count=512, seed=7)
     g = torch.Generator(device="cuda").manual_seed(43)
     bufs, offs, ptrs = batched.pack_problems(torch, m, n, k, gen=g)
     plan = batched.VBatchPlan(m, n, k, precision=prec)
+    _assert_path(plan, path)
     plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2])
     torch.cuda.synchronize()
     eager = bufs[2].clone()
@@ -174,14 +228,14 @@ def test_graph_capture_replays_match_eager(prec):
 
 
 @pytest.mark.parametrize("prec,table", _cases(("all64", "all128", "mixed")), indirect=["table"])
-def test_padded_leading_dimensions(prec, table):
+def test_padded_leading_dimensions(prec, table, path):
     """lda > m, ldb > k, ldc > m with odd paddings (columns off 16-byte
     alignment); the padding rows of C must stay untouched."""
     rng = np.random.default_rng(44)
     count = 40
     m = rng.integers(1, 200, count)
     n = rng.integers(1, 200, count)
-    k = rng.integers(1, 200, count)
+    k = _ws_k(rng.integers(1, 200, count), path)
     lda, ldb, ldc = m + rng.integers(1, 6, count), k + rng.integers(1, 6, count), m + rng.integers(1, 6, count)
     g = torch.Generator(device="cuda").manual_seed(45)
     As = [torch.rand(int(lda[p] * k[p]), device="cuda", generator=g) * 2 - 1 for p in range(count)]
@@ -190,6 +244,7 @@ def test_padded_leading_dimensions(prec, table):
     C0 = [c.cpu().numpy().copy() for c in Cs]
     ptr = lambda ts: torch.tensor([t.data_ptr() for t in ts], dtype=torch.int64, device="cuda")
     plan = batched.VBatchPlan(m, n, k, lda=lda, ldb=ldb, ldc=ldc, precision=prec)
+    _assert_path(plan, path)
     alpha, beta = 0.75, 1.5
     plan.exec(alpha, ptr(As), ptr(Bs), beta, ptr(Cs))
     torch.cuda.synchronize()
