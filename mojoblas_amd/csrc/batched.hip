@@ -766,7 +766,11 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
     };
 
     // ---- MFMA waves ----
-    const int wm0 = (wave & 1) * (T / 2), wn0 = ((wave >> 1) & 1) * (T / 2);
+    // The wave's part of the current item's C tile starts at (wm0, wn0): its
+    // quadrant, or for an fp32 128 x 128 tile with at most 64 rows (columns)
+    // below the edge a 32-column (32-row) strip, so that the four waves share
+    // the tile's blocks evenly (they step together).
+    int wm0 = (wave & 1) * (T / 2), wn0 = ((wave >> 1) & 1) * (T / 2);
     const int lr = lane & 31, lh = lane >> 5;
     // MFMAs of one k-tile over the wave's BI x BJ blocks below the tile's
     // edge (compile-time subsets, so no MFMA is predicated); only the MFMA
@@ -974,9 +978,22 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
                 // blocks of the wave below the item's edge (bf16: all of
                 // them, its MFMAs cost little; those past the edge are never
                 // stored)
-                const int mr = __builtin_amdgcn_readfirstlane(hdr[b].mr) - wm0;
-                const int nr = __builtin_amdgcn_readfirstlane(hdr[b].nr) - wn0;
-                const int bi = min(TW, max(0, (mr + 31) >> 5)), bj = min(TW, max(0, (nr + 31) >> 5));
+                const int tmr = __builtin_amdgcn_readfirstlane(hdr[b].mr);
+                const int tnr = __builtin_amdgcn_readfirstlane(hdr[b].nr);
+                if (!BF16 && TW == 2 && tmr <= 64 && tnr > 64) {
+                    wm0 = 0;
+                    wn0 = 32 * wave;
+                } else if (!BF16 && TW == 2 && tnr <= 64 && tmr > 64) {
+                    wm0 = 32 * wave;
+                    wn0 = 0;
+                } else {
+                    wm0 = (wave & 1) * (T / 2);
+                    wn0 = ((wave >> 1) & 1) * (T / 2);
+                }
+                const int strip_r = !BF16 && TW == 2 && tmr <= 64 && tnr > 64 ? 1 : TW;  // block-columns at most
+                const int strip_c = !BF16 && TW == 2 && tnr <= 64 && tmr > 64 ? 1 : TW;  // block-rows at most
+                const int bi = min(strip_c, max(0, (tmr - wm0 + 31) >> 5));
+                const int bj = min(strip_r, max(0, (tnr - wn0 + 31) >> 5));
                 if (BF16 || (bi == TW && bj == TW))
                     b = run_item(b, tw(), tw(), beta_c);
                 else if (bi == 0 || bj == 0)
