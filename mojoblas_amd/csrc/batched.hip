@@ -1378,9 +1378,36 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
     std::vector<int> wsoff;
     if (plan->ws) {
         // A persistent grid of the workgroups that stay resident (at most
-        // wgs per CU); each workgroup gets a fixed list.  Cost of an item:
-        // its k-steps plus about one for the C tile and the item switch.
-        auto ws_cost = [&](int p) { return (double)std::max<int64_t>(1, (probs[(size_t)p].k + 31) / 32) + 1.0; };
+        // wgs per CU); each workgroup gets a fixed list, balanced by a cost
+        // per tile in units of one 32-deep k-step's fixed work (barrier,
+        // headers, staging): bf16, its k-steps plus one for the C tile (a
+        // step's time varies little with the panels it stages: weighting
+        // the staged bytes measured slower); fp32, its k-steps plus the
+        // busiest MFMA wave's 32 x 32 blocks times the tile's 8-deep
+        // k-blocks (four blocks of four a k-step weigh about two units) and
+        // a share for the C tile.
+        auto ws_tile_cost = [&](int p, int tile) {
+            const VbProblem& q = probs[(size_t)p];
+            const int T = vb_tile(q.cls);
+            const int64_t mr = std::min<int64_t>(T, q.m - (int64_t)(tile / q.tiles_n) * T);
+            const int64_t nr = std::min<int64_t>(T, q.n - (int64_t)(tile % q.tiles_n) * T);
+            const double steps = (double)std::max<int64_t>(1, (q.k + 31) / 32);
+            const double br = (double)((mr + 31) / 32), bc = (double)((nr + 31) / 32);
+            if (bf) return steps + 1.0;
+            double bmax;
+            if (T == 128 && mr <= 64 && nr > 64)
+                bmax = br;
+            else if (T == 128 && nr <= 64 && mr > 64)
+                bmax = bc;
+            else
+                bmax = std::min(br, T / 64.0) * std::min(bc, T / 64.0);
+            return steps + bmax * (double)((q.k + 7) / 8) / 8.0 + bmax / 4.0;
+        };
+        auto ws_cost = [&](int p) {
+            double c = 0.0;
+            for (int64_t t = 0; t < tiles_of(p); ++t) c += ws_tile_cost(p, (int)t);
+            return c;
+        };
         for (int c = 0; c < 2; ++c) {
             const void* kfn = c == 0 ? (bf ? (const void*)vb_gemm_ws_kernel<true, 128> : (const void*)vb_gemm_ws_kernel<false, 128>)
                                      : (bf ? (const void*)vb_gemm_ws_kernel<true, 64> : (const void*)vb_gemm_ws_kernel<false, 64>);
@@ -1404,7 +1431,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
                 for (size_t i = 0; i < queues[c].size();) {  // the tiles of a problem are consecutive
                     const int p = queues[c][i].prob;
                     const int64_t nt = tiles_of(p);
-                    jobs.push_back({i, nt, (double)nt * ws_cost(p)});
+                    jobs.push_back({i, nt, ws_cost(p)});
                     i += (size_t)nt;
                 }
                 std::stable_sort(jobs.begin(), jobs.end(), [](const Job& a, const Job& b) { return a.cost > b.cost; });
@@ -1414,7 +1441,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
                     xload[(size_t)x] += jb.cost;
                     for (int64_t t = 0; t < jb.tiles; ++t) {
                         Item it = queues[c][jb.first + (size_t)t];
-                        it.cost = ws_cost(it.prob);
+                        it.cost = ws_tile_cost(it.prob, it.tile);
                         xitems[(size_t)x].push_back(it);
                     }
                 }
