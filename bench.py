@@ -395,6 +395,11 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(tf_main / peak, 4),
             "traffic": traffic,
+            "traffic_source": ("profiles/gemm_packed_bf16x3_8192_traffic.txt: rocprofv3 --pmc FETCH_SIZE and "
+                               "WRITE_SIZE passes of this kernel at this size (FETCH_SIZE x2.000, WRITE_SIZE x1.000 "
+                               "from a 1 GiB device copy in the same process); this run reads no counters"
+                               if traffic is not None else None),
+            "algorithmic_bytes": 2 * 2 * N * N * 2 + N * N * 4,
             "kernel": "gemm_packed_kernel<3,256,256,2,4>",
             "kernel_ms": round(main_ms, 4),
             "pack_ms_per_step": round(pack_ms, 4),

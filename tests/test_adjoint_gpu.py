@@ -49,10 +49,19 @@ def test_gemm_backward_vs_fp64_reference(shape, prec):
         assert (err <= bound).all(), (shape, prec, float((err / bound).max()))
 
 
-def test_gemm_backward_vs_fp64_finite_differences():
-    """Config size m = n = k = 2048: 64 random directions (V, U), h = 1e-6."""
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_gemm_backward_vs_fp64_finite_differences(prec):
+    """Config size m = n = k = 2048: 64 random directions (V, U), h = 1e-6,
+    |fd - <dA, V> - <dB, U>| <= 1e-5 sum(|dA||V| + |dB||U|) for both
+    precisions.  The tolerance is statistical, not a worst case: a
+    gradient entry's rounding error (fp32 accumulation; BF16x3 also about
+    2^-17 |a b| per product) is a sum of k random-signed terms, about
+    2^-17 sqrt(k) ~ 3e-4 here against entries of size ~sqrt(k) ~ 45, and the
+    directional sum over 4M entries adds them with random signs (~1 in all,
+    against a tolerance ~1.5e3); a wrong gradient (a transposed or dropped
+    term) moves <dA, V> by about its own size (~1e5)."""
     m = n = k = 2048
-    A, B, W, _ = _gemm_case(m, n, k, "fp32", 51)
+    A, B, W, _ = _gemm_case(m, n, k, prec, 51)
     A64, B64, W64 = (t.detach().double() for t in (A, B, W))
     f = lambda a, b: (W64 * (a @ b)).sum()
     g = torch.Generator(device="cuda").manual_seed(52)
@@ -64,7 +73,7 @@ def test_gemm_backward_vs_fp64_finite_differences():
         fd = (f(A64 + h * V, B64 + h * U) - f(A64 - h * V, B64 - h * U)) / (2 * h)
         ours = (dA * V).sum() + (dB * U).sum()
         scale = (dA.abs() * V.abs()).sum() + (dB.abs() * U.abs()).sum()
-        assert abs(float(fd - ours)) <= 1e-5 * float(scale), (float(fd), float(ours), float(scale))
+        assert abs(float(fd - ours)) <= 1e-5 * float(scale), (prec, float(fd), float(ours), float(scale))
 
 
 def _tri(n, lower, seed, unit=False):
