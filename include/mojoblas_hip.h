@@ -301,6 +301,16 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
                            mbh_vbatch_plan* plan);
 int mbh_sgemm_vbatched_exec(mbh_vbatch_plan plan, float alpha, const float* const* a,
                             const float* const* b, float beta, float* const* c, void* stream);
+/* exec with bf16 operands (no counterpart in the reference): a[p] / b[p]
+ * point at bf16 bit patterns (uint16, column-major with the plan's lda /
+ * ldb, any 2-byte alignment), C stays fp32.  Needs a MBH_PREC_BF16 plan on
+ * the warp-specialized kernels (plan_info kernel = 1) whose problems with
+ * k > 0 have at least 8 elements per operand (MBH_E_ARG otherwise).  The
+ * results are bit-identical to mbh_sgemm_vbatched_exec on fp32 operands
+ * that round (nearest even) to the same bf16 values; half the operand bytes
+ * are read.  Same capture and overlap rules as exec. */
+int mbh_bgemm_vbatched_exec(mbh_vbatch_plan plan, float alpha, const void* const* a, const void* const* b,
+                            float beta, float* const* c, void* stream);
 int mbh_vbatch_plan_info(mbh_vbatch_plan plan, double* out);
 int mbh_vbatch_plan_destroy(mbh_vbatch_plan plan);
 int mbh_vbatch_set_tuning(int precision, const int* class_table, int wgs_per_cu);

@@ -72,6 +72,7 @@ SIGNATURES = {
     "mbh_trsv_status": ([vp, ctypes.POINTER(i32)], i32),
     "mbh_vbatch_plan_create": ([i64, vp, vp, vp, vp, vp, vp, i32, vp], i32),
     "mbh_sgemm_vbatched_exec": ([vp, f32, vp, vp, f32, vp, vp], i32),
+    "mbh_bgemm_vbatched_exec": ([vp, f32, vp, vp, f32, vp, vp], i32),
     "mbh_vbatch_plan_info": ([vp, vp], i32),
     "mbh_vbatch_plan_destroy": ([vp], i32),
     "mbh_vbatch_set_tuning": ([i32, vp, i32], i32),

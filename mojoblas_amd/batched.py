@@ -48,18 +48,24 @@ class VBatchPlan:
         keys = ["count", "tiles", "workgroups", "flops", "problems_64", "problems_128", "kernel", "precision"]
         return dict(zip(keys, out.tolist()))
 
-    def exec(self, alpha, a_ptrs, b_ptrs, beta, c_ptrs, stream=None):
+    def exec(self, alpha, a_ptrs, b_ptrs, beta, c_ptrs, stream=None, operands="fp32"):
+        """operands "fp32": A_p / B_p are fp32 (mbh_sgemm_vbatched_exec);
+        "bf16": they are bf16 (mbh_bgemm_vbatched_exec, a "bf16" plan only).
+        C_p is fp32 either way."""
         for t in (a_ptrs, b_ptrs, c_ptrs):
             if not (t.is_cuda and str(t.dtype) == "torch.int64" and t.numel() == self.count):
                 raise TypeError("pointer arrays must be int64 CUDA tensors with one entry per problem")
+        if operands not in ("fp32", "bf16"):
+            raise ValueError("operands must be 'fp32' or 'bf16'")
         if stream is None:
             import torch
 
             stream = torch.cuda.current_stream().cuda_stream
-        rc = _lib.lib().mbh_sgemm_vbatched_exec(self._h, alpha, ctypes.c_void_p(a_ptrs.data_ptr()),
-                                                ctypes.c_void_p(b_ptrs.data_ptr()), beta,
-                                                ctypes.c_void_p(c_ptrs.data_ptr()), ctypes.c_void_p(stream))
-        _check(rc, "mbh_sgemm_vbatched_exec")
+        name = "mbh_sgemm_vbatched_exec" if operands == "fp32" else "mbh_bgemm_vbatched_exec"
+        rc = getattr(_lib.lib(), name)(self._h, alpha, ctypes.c_void_p(a_ptrs.data_ptr()),
+                                       ctypes.c_void_p(b_ptrs.data_ptr()), beta, ctypes.c_void_p(c_ptrs.data_ptr()),
+                                       ctypes.c_void_p(stream))
+        _check(rc, name)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -109,6 +115,18 @@ def pack_problems(torch, sizes_m, sizes_n, sizes_k, gen=None, device="cuda", fil
     C = torch.zeros(max(ct, 1), device=device)
     ptr = lambda T, off: torch.from_numpy(T.data_ptr() + 4 * off).to(device)
     return (A, B, C), (aoff, boff, coff), (ptr(A, aoff), ptr(B, boff), ptr(C, coff))
+
+
+def to_bf16(torch, buffers, offsets):
+    """bf16 copies (nearest even) of the flat fp32 operand buffers of
+    pack_problems and their pointer arrays (same element offsets, so the
+    operands keep their leading dimensions)."""
+    out, ptrs = [], []
+    for T, off in zip(buffers, offsets):
+        H = T.to(torch.bfloat16)
+        out.append(H)
+        ptrs.append(torch.from_numpy(H.data_ptr() + 2 * np.asarray(off, np.int64)).to(T.device))
+    return out, ptrs
 
 
 def config4_shapes(count=4096, seed=2024, lo=16, hi=512):

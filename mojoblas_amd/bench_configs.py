@@ -175,30 +175,36 @@ def cfg4(torch, mb, reps):
     m, n, k = batched.config4_shapes()
     g = torch.Generator(device="cuda").manual_seed(5)
     bufs, offs, ptrs = batched.pack_problems(torch, m, n, k, gen=g)
+    _, p16 = batched.to_bf16(torch, bufs[:2], offs[:2])
     flops = float(np.sum(2.0 * m * n * k))
-    for prec in ("fp32", "bf16"):
-        # bf16 mode reads fp32 operands once in its pack pass, then bf16 panels
-        byt = float(np.sum(4.0 * (m * k + k * n) + 4.0 * m * n))
+    # (precision, operand storage): fp32; bf16 MFMAs on the fp32 operands
+    # (rounded as they are staged); bf16 MFMAs on bf16 operands (the U[-1,1]
+    # operands rounded once, outside the timed region)
+    for prec, opnd in (("fp32", "fp32"), ("bf16", "fp32"), ("bf16", "bf16")):
+        a, b = (ptrs[0], ptrs[1]) if opnd == "fp32" else p16
+        s = 4.0 if opnd == "fp32" else 2.0
+        byt = float(np.sum(s * (m * k + k * n) + 4.0 * m * n))
         ai = flops / byt
         peak = PEAK["fp32"] if prec == "fp32" else PEAK["bf16"]
         roof = min(peak, ai * HBM / 1e12)
         plan = batched.VBatchPlan(m, n, k, precision=prec)
-        ms = timed(torch, lambda: plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2]), reps)
+        call = lambda: plan.exec(1.0, a, b, 0.0, ptrs[2], operands=opnd)
+        ms = timed(torch, call, reps)
         graph = torch.cuda.CUDAGraph()
         st = torch.cuda.Stream()
         st.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(st):
-            plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2])
+            call()
         torch.cuda.current_stream().wait_stream(st)
         torch.cuda.synchronize()
         with torch.cuda.graph(graph):
-            plan.exec(1.0, ptrs[0], ptrs[1], 0.0, ptrs[2])
+            call()
         ms_g = timed(torch, graph.replay, reps)
         info = plan.info()
         emit({"config": "cfg4", "routine": "4096 variable-size NN gemms, m,n,k log-uniform [16,512]",
-              "precision": prec, "ms_eager": ms, "ms_graph": ms_g, "tflops_graph": flops / ms_g / 1e9,
-              "flops": flops, "bytes_min": byt, "arith_intensity": ai, "roofline_tflops": roof,
-              "roofline_frac": flops / ms_g / 1e9 / roof, "plan": info})
+              "precision": prec, "operands": opnd, "ms_eager": ms, "ms_graph": ms_g,
+              "tflops_graph": flops / ms_g / 1e9, "flops": flops, "bytes_min": byt, "arith_intensity": ai,
+              "roofline_tflops": roof, "roofline_frac": flops / ms_g / 1e9 / roof, "plan": info})
         plan.close()
 
 

@@ -188,6 +188,38 @@ __device__ inline void vb_st16(MBH_GLOBAL float* p, const f32x4& v) {
     __builtin_memcpy((MBH_GLOBAL unsigned char*)p, &v, 16);
 }
 
+// 16 / 4 bytes at byte offset off (a multiple of 4) from a uniform base.
+__device__ inline u32x4 vb_ld16w(const MBH_GLOBAL unsigned char* base, unsigned off) {
+    u32x4 v;
+    __builtin_memcpy(&v, (const MBH_GLOBAL unsigned*)(base + off), 16);
+    return v;
+}
+__device__ inline unsigned vb_ld4w(const MBH_GLOBAL unsigned char* base, unsigned off) {
+    return *(const MBH_GLOBAL unsigned*)(base + off);
+}
+
+// Eight 16-bit elements from the five dwords d, d4 that hold them: dwords
+// d..d4 shifted down by 16 bits when bit 3 of f is set (the first element in
+// a dword's upper half), else d itself.
+__device__ inline u32x4 vb_join8(const u32x4& d, unsigned d4, int f) {
+    const unsigned sh = (f & 8) ? 16u : 0u;
+    return u32x4{__builtin_amdgcn_alignbit(d.y, d.x, sh), __builtin_amdgcn_alignbit(d.z, d.y, sh),
+                 __builtin_amdgcn_alignbit(d.w, d.z, sh), __builtin_amdgcn_alignbit(d4, d.w, sh)};
+}
+
+// Eight 16-bit elements shifted down by s (0 .. 7): element e of the result
+// is element e + s of w for e + s <= 7 (the others are left unspecified).
+__device__ inline u32x4 vb_shift8(const u32x4& w, int s) {
+    const int dq = s >> 1;
+    const unsigned t0 = dq == 0 ? w.x : dq == 1 ? w.y : dq == 2 ? w.z : w.w;
+    const unsigned t1 = dq == 0 ? w.y : dq == 1 ? w.z : w.w;
+    const unsigned t2 = dq == 0 ? w.z : w.w;
+    const unsigned t3 = w.w;
+    const unsigned sh = (s & 1) ? 16u : 0u;
+    return u32x4{__builtin_amdgcn_alignbit(t1, t0, sh), __builtin_amdgcn_alignbit(t2, t1, sh),
+                 __builtin_amdgcn_alignbit(t3, t2, sh), __builtin_amdgcn_alignbit(t3, t3, sh)};
+}
+
 // v shifted down by s elements: out[e] = v[min(e + s, 3)], as a two-stage
 // shifter of selects (shift by 2, then by 1; s >= 3 shifts by 3).
 __device__ inline void vb_shift4(const float4& v, int s, float* out) {
@@ -553,6 +585,19 @@ struct VbWsRec {
 };
 static_assert(sizeof(VbWsRec) == 64, "one 64-byte record per item");
 
+// Record j, moved into scalar registers (the index is wave-uniform; the
+// compiler loads it through the vector cache, since the kernel's C stores
+// could alias it as far as it knows).
+__device__ inline VbWsRec vb_rec(const VbWsRec* __restrict__ recs, int j) {
+    unsigned w[16];
+    __builtin_memcpy(w, recs + j, 64);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    VbWsRec r;
+    __builtin_memcpy(&r, w, 64);
+    return r;
+}
+
 // What the loader waves tell the MFMA waves about the k-tile they staged.
 struct VbWsHdr {
     float* Ct;      // C tile origin, C + m0 + n0 ldc
@@ -607,10 +652,17 @@ __global__ void vb_ws_fill_kernel(const VbProblem* __restrict__ probs, const VbI
 // XCDs, then over each XCD's workgroups); the loaders read each record one
 // item ahead.  Every item takes at least one k-step (k = 0: a zero k-tile,
 // so C := beta C).
-template <bool BF16, int T>
+// SRC16 (BF16 only): the operands are stored as bf16 in HBM (16-bit bit
+// patterns, any 2-byte alignment), so each load moves eight elements: a
+// dword-aligned 16-byte load plus the dword after it, funnel-shifted by 16
+// bits when the first element sits in a dword's upper half.  The LDS
+// layouts, the MFMA waves and the results are those of BF16 on the fp32
+// operands that round to the same bf16 values.
+template <bool BF16, int T, bool SRC16 = false>
 __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __restrict__ recs,
                                                              const int* __restrict__ wsoff, float alpha,
                                                              float beta) {
+    static_assert(BF16 || !SRC16, "16-bit operands feed the bf16 tiles");
     typedef VbLean<BF16, T> L;
     constexpr int NQ = L::NQ, TW = L::TW, SAF = L::SAF, SBF = L::SBF, SAH = L::SAH;
     // A stage holds one k-tile, or at an item's end the MFMA waves' C blocks
@@ -636,9 +688,18 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
     const int arg = ltid % ARG, akg = ltid / ARG;  // A: row group, first k
     const int g8 = lane & 7;                       // B: k group
     const int cb = ltid >> 3;                      // B: column within a 32-column load
+    // SRC16: eight elements per load.  A: T/8 lanes per k-column (eight rows
+    // each), 2048/T k-columns per load; B: lane group g4 holds k = 8 g4 ..
+    // + 7 of column cb16 + 64 q.  NQ16 loads per operand, each a 16-byte
+    // load and a 4-byte one, so a set issues 2 NQ loads as in fp32.
+    constexpr int NQ16 = T / 64, ARG16 = T / 8, AKG16 = 2048 / T;
+    const int arg16 = ltid % ARG16, akg16 = ltid / ARG16;
+    const int g4 = lane & 3, cb16 = ltid >> 2;
     struct Set {
         float4 va[NQ], vb[NQ];
         int sa[NQ], sb[NQ];
+        u32x4 ha[NQ16], hb[NQ16];    // SRC16: dwords 0-3 of each 8-element load
+        unsigned ha4[NQ16], hb4[NQ16];  // SRC16: dword 4 (dword 3 again when aligned)
         int qb, krem, lda, last, end, ldc, mr, nr;
         MBH_GLOBAL float* Ct;
     };
@@ -651,8 +712,8 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
     int kt = 0;
     VbWsRec cur{}, nxt{};
     if (loader) {
-        if (!done) cur = recs[j];
-        if (j + 1 < jend) nxt = recs[j + 1];
+        if (!done) cur = vb_rec(recs, j);
+        if (j + 1 < jend) nxt = vb_rec(recs, j + 1);
     }
     auto nload = [&](int extent) { return min(NQ, (extent + 31) >> 5); };
     // The next k-tile of the list into S: loads issued, not waited for.  S's
@@ -668,7 +729,7 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
             } else {
                 cur = nxt;
                 kt = 0;
-                if (j + 1 < jend) nxt = recs[j + 1];
+                if (j + 1 < jend) nxt = vb_rec(recs, j + 1);
             }
         }
         const bool live = !done && cur.K > 0;
@@ -684,6 +745,37 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         S.Ct = (MBH_GLOBAL float*)cur.C + (cur.m0 + (int64_t)cur.n0 * cur.ldc);
         // last element of each operand (k > 0: at least four elements each)
         const int alast = (cur.K - 1) * cur.lda + cur.m - 1, blast = (cur.n - 1) * cur.ldb + cur.K - 1;
+        if constexpr (SRC16) {
+            // byte offsets from the operand's dword-aligned base (lanes past
+            // the edge read elements 0 .. 7, idle lanes the record array);
+            // an 8-element group past the operand's end is clamped to its
+            // last eight elements (every operand of a problem with k > 0
+            // has at least eight).  sa / sb: element shift | odd-half flag << 3.
+            const uintptr_t pa = live ? (uintptr_t)cur.A : (uintptr_t)nullp;
+            const uintptr_t pb = live ? (uintptr_t)cur.B : (uintptr_t)nullp;
+            const MBH_GLOBAL unsigned char* a4 = (const MBH_GLOBAL unsigned char*)(pa & ~(uintptr_t)3);
+            const MBH_GLOBAL unsigned char* b4 = (const MBH_GLOBAL unsigned char*)(pb & ~(uintptr_t)3);
+            const unsigned ua = (unsigned)pa & 2u, ub = (unsigned)pb & 2u;
+            S.qb = min(NQ16, (S.nr + 63) >> 6);
+            const bool arow_ok = live && 8 * arg16 < S.mr;
+#pragma unroll
+            for (int q = 0; q < NQ16; ++q) {
+                const int ia = cur.m0 + 8 * arg16 + (k0 + akg16 + AKG16 * q) * cur.lda, ja = min(ia, alast - 7);
+                const unsigned boa = arow_ok ? ua + 2u * (unsigned)ja : ua;
+                S.sa[q] = (arow_ok ? ia - ja : 0) | (int)((boa & 2u) << 2);
+                S.ha[q] = vb_ld16w(a4, boa & ~3u);
+                S.ha4[q] = vb_ld4w(a4, (boa & ~3u) + ((boa & 2u) ? 16u : 12u));
+                const bool okb = live && q < S.qb;
+                const int ib = k0 + 8 * g4 + min(cur.n0 + cb16 + 64 * q, cur.n - 1) * cur.ldb, jb = min(ib, blast - 7);
+                const unsigned bob = okb ? ub + 2u * (unsigned)jb : 0u;
+                S.sb[q] = (okb ? ib - jb : 0) | (int)((bob & 2u) << 2);
+                S.hb[q] = vb_ld16w(okb ? b4 : (const MBH_GLOBAL unsigned char*)nullp, bob & ~3u);
+                S.hb4[q] = vb_ld4w(okb ? b4 : (const MBH_GLOBAL unsigned char*)nullp,
+                                   (bob & ~3u) + ((bob & 2u) ? 16u : 12u));
+            }
+            if (!done) ++kt;
+            return;
+        }
         const int kk = k0 + 4 * g8;
         const bool arow_ok = live && 4 * arg < S.mr;  // lanes past the edge read A[0 .. 3]
 #pragma unroll
@@ -705,7 +797,30 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
     auto stage = [&](int b, const Set& S) {
         MBH_LDS unsigned char* buf = lbuf(b);
         __builtin_amdgcn_s_waitcnt(VMWAIT);
-        if (!S.end) {
+        if (SRC16 && !S.end) {
+            MBH_LDS unsigned char* arow = buf + akg16 * SAH + 16 * arg16;
+            MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + L::A_BYTES);
+            const bool whole = S.krem > 32 && S.lda >= 7;  // no clamped group, no k past K
+#pragma unroll
+            for (int q = 0; q < NQ16; ++q) {
+                u32x4 wa = vb_join8(S.ha[q], S.ha4[q], S.sa[q]), wb = vb_join8(S.hb[q], S.hb4[q], S.sb[q]);
+                if (!whole) {
+                    wa = vb_shift8(wa, S.sa[q] & 7);
+                    wb = vb_shift8(wb, S.sb[q] & 7);
+                    if (akg16 + AKG16 * q >= S.krem) wa = u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int k = 8 * g4 + 2 * i;
+                        wb[i] &= (k < S.krem ? 0xffffu : 0u) | (k + 1 < S.krem ? 0xffff0000u : 0u);
+                    }
+                }
+                *(MBH_LDS u32x4*)(arow + AKG16 * SAH * q) = wa;
+                if (q < S.qb) {
+                    const int jj = cb16 + 64 * q;
+                    *(MBH_LDS u32x4*)(b16 + jj * 32 + ((g4 ^ lds_swz(jj)) << 3)) = wb;
+                }
+            }
+        } else if (!S.end) {
             float ra[4 * NQ], rb[4 * NQ];
             if (S.krem > 32 && S.lda >= 3) {
 #pragma unroll
@@ -1232,11 +1347,43 @@ struct VbPlan {
     bool ws = false;
     int* wsoff = nullptr;
     VbWsRec* recs = nullptr;
+    // First problem with k > 0 and an operand of fewer than eight elements
+    // (-1: none); bf16-stored operands need eight.
+    int64_t first_short16 = -1;
 };
 
 }  // namespace mbh
 
 using namespace mbh;
+
+namespace {
+
+// The warp-specialized launches of one exec: the record fill, then the
+// 128x128 and 64x64 classes.
+template <bool BF16, bool SRC16>
+int vb_ws_exec(const VbPlan* plan, float alpha, const float* const* a, const float* const* b, float beta,
+               float* const* c, hipStream_t s) {
+    vb_ws_fill_kernel<<<dim3((unsigned)((plan->nitems + 255) / 256)), 256, 0, s>>>(plan->probs, plan->items,
+                                                                                   plan->nitems, a, b, c, plan->recs);
+    if (plan->grid1 > 0)
+        vb_gemm_ws_kernel<BF16, 128, SRC16><<<dim3((unsigned)plan->grid1), 512, 0, s>>>(plan->recs, plan->wsoff,
+                                                                                         alpha, beta);
+    if (plan->grid > 0)
+        vb_gemm_ws_kernel<BF16, 64, SRC16><<<dim3((unsigned)plan->grid), 512, 0, s>>>(
+            plan->recs, plan->wsoff + plan->grid1 + 1, alpha, beta);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "vbatched exec");
+}
+
+int vb_scale_exec(const VbPlan* plan, float beta, float* const* c, hipStream_t s) {
+    if (beta == 1.0f) return 0;
+    vb_scale_kernel<<<dim3((unsigned)std::min<int64_t>(plan->count, 65535)), 256, 0, s>>>(plan->probs,
+                                                                                           (int)plan->count, beta, c);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "vbatched scale");
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1326,6 +1473,8 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         plan->class_count[q.cls]++;
         if (!vb_lean_ok(q)) plan->lean = false;
         if (!vb_ws_ok(q)) all_ws = false;
+        if (plan->first_short16 < 0 && q.k > 0 && ((q.k - 1) * q.lda + q.m < 8 || (q.n - 1) * q.ldb + q.k < 8))
+            plan->first_short16 = p;
         ntiles += tm * tn;
     }
     if (ntiles > (int64_t)INT32_MAX) {
@@ -1580,6 +1729,33 @@ int mbh_vbatch_plan_info(mbh_vbatch_plan handle, double* info) {
     return 0;
 }
 
+int mbh_bgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const void* const* a, const void* const* b,
+                            float beta, float* const* c, void* stream) {
+    const VbPlan* plan = (const VbPlan*)handle;
+    if (!plan) {
+        set_error("mbh_bgemm_vbatched_exec: null plan");
+        return MBH_E_ARG;
+    }
+    if (plan->prec != MBH_PREC_BF16) {
+        set_error("mbh_bgemm_vbatched_exec: the plan's precision is %d (bf16 operands need a BF16 plan)", plan->prec);
+        return MBH_E_PRECISION;
+    }
+    if (plan->count == 0 || plan->nitems == 0) return 0;
+    if (!plan->ws) {
+        set_error("mbh_bgemm_vbatched_exec: the plan runs the work-queue kernels (an operand under four elements, "
+                  "offsets past 32 bits or MBH_VBATCH_KERNEL=lean), which read fp32 operands only");
+        return MBH_E_ARG;
+    }
+    if (plan->first_short16 >= 0) {
+        set_error("mbh_bgemm_vbatched_exec: problem %lld has an operand of fewer than 8 elements",
+                  (long long)plan->first_short16);
+        return MBH_E_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (alpha == 0.0f) return vb_scale_exec(plan, beta, c, s);
+    return vb_ws_exec<true, true>(plan, alpha, (const float* const*)a, (const float* const*)b, beta, c, s);
+}
+
 int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* const* a, const float* const* b,
                             float beta, float* const* c, void* stream) {
     const VbPlan* plan = (const VbPlan*)handle;
@@ -1589,37 +1765,12 @@ int mbh_sgemm_vbatched_exec(mbh_vbatch_plan handle, float alpha, const float* co
     }
     if (plan->count == 0 || plan->nitems == 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    if (alpha == 0.0f) {
-        if (beta == 1.0f) return 0;
-        vb_scale_kernel<<<dim3((unsigned)std::min<int64_t>(plan->count, 65535)), 256, 0, s>>>(plan->probs,
-                                                                                               (int)plan->count,
-                                                                                               beta, c);
-        hipError_t e = hipGetLastError();
-        return e == hipSuccess ? 0 : hip_fail(e, "vbatched scale");
-    }
+    if (alpha == 0.0f) return vb_scale_exec(plan, beta, c, s);
     const dim3 grid((unsigned)plan->grid), block(256);
     const bool mixed = plan->class_count[1] > 0;
     if (plan->ws) {
-        const bool bf = plan->prec == MBH_PREC_BF16;
-        vb_ws_fill_kernel<<<dim3((unsigned)((plan->nitems + 255) / 256)), 256, 0, s>>>(
-            plan->probs, plan->items, plan->nitems, a, b, c, plan->recs);
-        const int* off0 = plan->wsoff + plan->grid1 + 1;
-        if (plan->grid1 > 0) {
-            if (bf)
-                vb_gemm_ws_kernel<true, 128><<<dim3((unsigned)plan->grid1), 512, 0, s>>>(plan->recs, plan->wsoff,
-                                                                                          alpha, beta);
-            else
-                vb_gemm_ws_kernel<false, 128><<<dim3((unsigned)plan->grid1), 512, 0, s>>>(plan->recs, plan->wsoff,
-                                                                                           alpha, beta);
-        }
-        if (plan->grid > 0) {
-            if (bf)
-                vb_gemm_ws_kernel<true, 64><<<grid, 512, 0, s>>>(plan->recs, off0, alpha, beta);
-            else
-                vb_gemm_ws_kernel<false, 64><<<grid, 512, 0, s>>>(plan->recs, off0, alpha, beta);
-        }
-        hipError_t e = hipGetLastError();
-        return e == hipSuccess ? 0 : hip_fail(e, "vbatched exec");
+        return plan->prec == MBH_PREC_BF16 ? vb_ws_exec<true, false>(plan, alpha, a, b, beta, c, s)
+                                           : vb_ws_exec<false, false>(plan, alpha, a, b, beta, c, s);
     }
     vb_reset_kernel<<<1, 64, 0, s>>>(plan->ctr);
     if (plan->lean) {
