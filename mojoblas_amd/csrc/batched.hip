@@ -188,6 +188,15 @@ __device__ inline void vb_st16(MBH_GLOBAL float* p, const f32x4& v) {
     __builtin_memcpy((MBH_GLOBAL unsigned char*)p, &v, 16);
 }
 
+// a * b (low 32 bits) as its own v_mul_lo_u32.  Written as a multiply-add,
+// the compiler may form v_mad_u64_u32 whose unused high addend is a register
+// that a load in flight will write, and then wait for that load.
+__device__ inline int vb_mul(int a, int b) {
+    int r;
+    asm("v_mul_lo_u32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // 16 / 4 bytes at byte offset off (a multiple of 4) from a uniform base.
 __device__ inline u32x4 vb_ld16w(const MBH_GLOBAL unsigned char* base, unsigned off) {
     u32x4 v;
@@ -585,14 +594,18 @@ struct VbWsRec {
 };
 static_assert(sizeof(VbWsRec) == 64, "one 64-byte record per item");
 
-// Record j, moved into scalar registers (the index is wave-uniform; the
-// compiler loads it through the vector cache, since the kernel's C stores
-// could alias it as far as it knows).
+// Record j (a wave-uniform index) by a scalar load: the records are written
+// by the fill kernel before the tile kernel starts and only read by it, so
+// they are read through the constant address space.  (Through the vector
+// cache the loads would share the vmcnt counter with the operand loads, and
+// moving them to scalar registers would wait for every operand load in
+// flight.)
 __device__ inline VbWsRec vb_rec(const VbWsRec* __restrict__ recs, int j) {
+    typedef const __attribute__((address_space(4))) unsigned* cu32_t;
+    const cu32_t p = (cu32_t)(recs + j);
     unsigned w[16];
-    __builtin_memcpy(w, recs + j, 64);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    for (int i = 0; i < 16; ++i) w[i] = p[i];
     VbWsRec r;
     __builtin_memcpy(&r, w, 64);
     return r;
@@ -760,13 +773,14 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
             const bool arow_ok = live && 8 * arg16 < S.mr;
 #pragma unroll
             for (int q = 0; q < NQ16; ++q) {
-                const int ia = cur.m0 + 8 * arg16 + (k0 + akg16 + AKG16 * q) * cur.lda, ja = min(ia, alast - 7);
+                const int ia = cur.m0 + 8 * arg16 + vb_mul(k0 + akg16 + AKG16 * q, cur.lda), ja = min(ia, alast - 7);
                 const unsigned boa = arow_ok ? ua + 2u * (unsigned)ja : ua;
                 S.sa[q] = (arow_ok ? ia - ja : 0) | (int)((boa & 2u) << 2);
                 S.ha[q] = vb_ld16w(a4, boa & ~3u);
                 S.ha4[q] = vb_ld4w(a4, (boa & ~3u) + ((boa & 2u) ? 16u : 12u));
                 const bool okb = live && q < S.qb;
-                const int ib = k0 + 8 * g4 + min(cur.n0 + cb16 + 64 * q, cur.n - 1) * cur.ldb, jb = min(ib, blast - 7);
+                const int ib = k0 + 8 * g4 + vb_mul(min(cur.n0 + cb16 + 64 * q, cur.n - 1), cur.ldb),
+                          jb = min(ib, blast - 7);
                 const unsigned bob = okb ? ub + 2u * (unsigned)jb : 0u;
                 S.sb[q] = (okb ? ib - jb : 0) | (int)((bob & 2u) << 2);
                 S.hb[q] = vb_ld16w(okb ? b4 : (const MBH_GLOBAL unsigned char*)nullp, bob & ~3u);
