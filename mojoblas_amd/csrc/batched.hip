@@ -713,7 +713,10 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         int sa[NQ], sb[NQ];
         u32x4 ha[NQ16], hb[NQ16];    // SRC16: dwords 0-3 of each 8-element load
         unsigned ha4[NQ16], hb4[NQ16];  // SRC16: dword 4 (dword 3 again when aligned)
-        int qb, krem, lda, last, end, ldc, mr, nr;
+        // whole: every k below K and no group past the operand's end (any
+        // k-tile but an item's last, given lda >= 3, or >= 7 for 16-bit
+        // operands), so the k-tile is staged without shifts or masks
+        int qb, krem, whole, last, end, ldc, mr, nr;
         MBH_GLOBAL float* Ct;
     };
     const MBH_GLOBAL float* const nullp = (const MBH_GLOBAL float*)recs;
@@ -748,14 +751,15 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         const bool live = !done && cur.K > 0;
         const int k0 = 32 * kt;
         S.end = done;
-        S.qb = nload(cur.n - cur.n0);
+        S.qb = SRC16 ? min(NQ16, (cur.n - cur.n0 + 63) >> 6) : nload(cur.n - cur.n0);
         S.krem = cur.K - k0;
-        S.lda = cur.lda;
+        S.whole = live && S.krem > 32 && cur.lda >= (SRC16 ? 7 : 3);
         S.last = kt == cur.nkt - 1;
         S.ldc = cur.ldc;
         S.mr = cur.m - cur.m0;
         S.nr = cur.n - cur.n0;
         S.Ct = (MBH_GLOBAL float*)cur.C + (cur.m0 + (int64_t)cur.n0 * cur.ldc);
+        if (!done) ++kt;
         // last element of each operand (k > 0: at least four elements each)
         const int alast = (cur.K - 1) * cur.lda + cur.m - 1, blast = (cur.n - 1) * cur.ldb + cur.K - 1;
         if constexpr (SRC16) {
@@ -769,7 +773,6 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
             const MBH_GLOBAL unsigned char* a4 = (const MBH_GLOBAL unsigned char*)(pa & ~(uintptr_t)3);
             const MBH_GLOBAL unsigned char* b4 = (const MBH_GLOBAL unsigned char*)(pb & ~(uintptr_t)3);
             const unsigned ua = (unsigned)pa & 2u, ub = (unsigned)pb & 2u;
-            S.qb = min(NQ16, (S.nr + 63) >> 6);
             const bool arow_ok = live && 8 * arg16 < S.mr;
 #pragma unroll
             for (int q = 0; q < NQ16; ++q) {
@@ -787,7 +790,6 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
                 S.hb4[q] = vb_ld4w(okb ? b4 : (const MBH_GLOBAL unsigned char*)nullp,
                                    (bob & ~3u) + ((bob & 2u) ? 16u : 12u));
             }
-            if (!done) ++kt;
             return;
         }
         const int kk = k0 + 4 * g8;
@@ -802,7 +804,6 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
             S.sb[q] = ib - jb;
             S.vb[q] = vb_load4(okb ? (const MBH_GLOBAL float*)cur.B : nullp, okb ? 4u * (unsigned)jb : 0u);
         }
-        if (!done) ++kt;
     };
     // S's k-tile into LDS buffer b and its header.  S's loads are the older
     // of the two sets in flight: one wait for all but the newer set's 2 NQ
@@ -814,11 +815,10 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         if (SRC16 && !S.end) {
             MBH_LDS unsigned char* arow = buf + akg16 * SAH + 16 * arg16;
             MBH_LDS uint16_t* b16 = (MBH_LDS uint16_t*)(buf + L::A_BYTES);
-            const bool whole = S.krem > 32 && S.lda >= 7;  // no clamped group, no k past K
 #pragma unroll
             for (int q = 0; q < NQ16; ++q) {
                 u32x4 wa = vb_join8(S.ha[q], S.ha4[q], S.sa[q]), wb = vb_join8(S.hb[q], S.hb4[q], S.sb[q]);
-                if (!whole) {
+                if (!S.whole) {  // a clamped group, or k past K
                     wa = vb_shift8(wa, S.sa[q] & 7);
                     wb = vb_shift8(wb, S.sb[q] & 7);
                     if (akg16 + AKG16 * q >= S.krem) wa = u32x4{0u, 0u, 0u, 0u};
@@ -836,7 +836,7 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
             }
         } else if (!S.end) {
             float ra[4 * NQ], rb[4 * NQ];
-            if (S.krem > 32 && S.lda >= 3) {
+            if (S.whole) {
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
                     ra[4 * q] = S.va[q].x, ra[4 * q + 1] = S.va[q].y, ra[4 * q + 2] = S.va[q].z, ra[4 * q + 3] = S.va[q].w;
@@ -1078,17 +1078,23 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         fetch(S0);
         fetch(S1);
         stage(0, S0);
+        // the end / last flags of the headers in buffers 0 and 1, as staged
+        // (kept in scalar registers rather than read back from LDS)
+        int end0 = S0.end, last0 = S0.end ? 0 : S0.last, last1;
         __syncthreads();
         // (steps that end an item take a second barrier: the MFMA waves'
         // C blocks pass through the stage buffer the step read)
-        while (!__builtin_amdgcn_readfirstlane(hdr[0].end)) {
+        while (!end0) {
             fetch(S0);
             stage(1, S1);
-            if (__builtin_amdgcn_readfirstlane(hdr[0].last)) __syncthreads();
+            last1 = S1.end ? 0 : S1.last;
+            if (last0) __syncthreads();
             __syncthreads();
             fetch(S1);
             stage(0, S0);
-            if (__builtin_amdgcn_readfirstlane(hdr[1].last)) __syncthreads();
+            end0 = S0.end;
+            last0 = S0.end ? 0 : S0.last;
+            if (last1) __syncthreads();
             __syncthreads();
         }
     } else {
