@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -593,6 +594,9 @@ struct VbWsRec {
     int m, n, K, lda, ldb, ldc, m0, n0, nkt, pad;
 };
 static_assert(sizeof(VbWsRec) == 64, "one 64-byte record per item");
+static_assert(offsetof(VbWsRec, C) == 16 && offsetof(VbWsRec, m) == 24 && offsetof(VbWsRec, K) == 32 &&
+                  offsetof(VbWsRec, ldc) == 44 && offsetof(VbWsRec, m0) == 48 && offsetof(VbWsRec, nkt) == 56,
+              "vb_tile_rec reads the record by dword");
 
 // Record j (a wave-uniform index) by a scalar load: the records are written
 // by the fill kernel before the tile kernel starts and only read by it, so
@@ -600,6 +604,27 @@ static_assert(sizeof(VbWsRec) == 64, "one 64-byte record per item");
 // cache the loads would share the vmcnt counter with the operand loads, and
 // moving them to scalar registers would wait for every operand load in
 // flight.)
+// The part of record j the MFMA waves use: the C tile and the k-steps.
+struct VbWsTile {
+    float* C;
+    int m, n, K, ldc, m0, n0, nkt;
+};
+__device__ inline VbWsTile vb_tile_rec(const VbWsRec* __restrict__ recs, int j) {
+    typedef const __attribute__((address_space(4))) unsigned* cu32_t;
+    const cu32_t p = (cu32_t)(recs + j);
+    VbWsTile t;
+    const uint64_t c = (uint64_t)p[4] | ((uint64_t)p[5] << 32);
+    t.C = (float*)c;
+    t.m = (int)p[6];
+    t.n = (int)p[7];
+    t.K = (int)p[8];
+    t.ldc = (int)p[11];
+    t.m0 = (int)p[12];
+    t.n0 = (int)p[13];
+    t.nkt = (int)p[14];
+    return t;
+}
+
 __device__ inline VbWsRec vb_rec(const VbWsRec* __restrict__ recs, int j) {
     typedef const __attribute__((address_space(4))) unsigned* cu32_t;
     const cu32_t p = (cu32_t)(recs + j);
@@ -610,15 +635,6 @@ __device__ inline VbWsRec vb_rec(const VbWsRec* __restrict__ recs, int j) {
     __builtin_memcpy(&r, w, 64);
     return r;
 }
-
-// What the loader waves tell the MFMA waves about the k-tile they staged.
-struct VbWsHdr {
-    float* Ct;      // C tile origin, C + m0 + n0 ldc
-    int ldc, mr, nr;  // rows / columns of C from the tile origin to the problem's edge
-    int nk;         // k values of the k-tile below K (0 for k = 0)
-    int last;       // the item's last k-tile: its C tile is stored after it
-    int end;        // no k-tile: the workgroup's list is done
-};
 
 __global__ void vb_ws_fill_kernel(const VbProblem* __restrict__ probs, const VbItem* __restrict__ items, int nitems,
                                   const float* const* A, const float* const* B, float* const* C,
@@ -648,8 +664,9 @@ __global__ void vb_ws_fill_kernel(const VbProblem* __restrict__ probs, const VbI
 // T x T tiles on 512 threads: waves 0-3 run the MFMAs (2 x 2 waves of
 // T/2 x T/2, the lean tile's LDS layouts and fragment reads) and store C;
 // waves 4-7 load the operands, two 32-deep k-tiles ahead in two register
-// sets, and stage each k-tile into LDS (rounded to bf16 in BF16 mode) with a
-// header for the MFMA waves.  The waves that compute never wait for a load,
+// sets, and stage each k-tile into LDS (rounded to bf16 in BF16 mode).  Both
+// roles walk the workgroup's item list from the same records, so the LDS
+// k-tiles are all they pass.  The waves that compute never wait for a load,
 // and a load has two k-steps to arrive.  Barriers order the two roles:
 //   step s: loaders issue the loads of step s + 2 (set s % 2) and stage step
 //           s + 1 (set (s + 1) % 2) into LDS buffer (s + 1) % 2; MFMA waves
@@ -682,7 +699,6 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
     // on their way out (4 KB per wave).
     constexpr int SB = L::BUF > 16384 ? L::BUF : 16384;
     __shared__ __attribute__((aligned(16))) unsigned char lds[2 * SB];
-    __shared__ VbWsHdr hdr[2];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool loader = wave >= 4;
@@ -716,8 +732,7 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         // whole: every k below K and no group past the operand's end (any
         // k-tile but an item's last, given lda >= 3, or >= 7 for 16-bit
         // operands), so the k-tile is staged without shifts or masks
-        int qb, krem, whole, last, end, ldc, mr, nr;
-        MBH_GLOBAL float* Ct;
+        int qb, krem, whole, last, end, mr;
     };
     const MBH_GLOBAL float* const nullp = (const MBH_GLOBAL float*)recs;
     // s_waitcnt vmcnt(2 NQ) (expcnt and lgkmcnt at their maxima: no wait)
@@ -755,10 +770,7 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         S.krem = cur.K - k0;
         S.whole = live && S.krem > 32 && cur.lda >= (SRC16 ? 7 : 3);
         S.last = kt == cur.nkt - 1;
-        S.ldc = cur.ldc;
         S.mr = cur.m - cur.m0;
-        S.nr = cur.n - cur.n0;
-        S.Ct = (MBH_GLOBAL float*)cur.C + (cur.m0 + (int64_t)cur.n0 * cur.ldc);
         if (!done) ++kt;
         // last element of each operand (k > 0: at least four elements each)
         const int alast = (cur.K - 1) * cur.lda + cur.m - 1, blast = (cur.n - 1) * cur.ldb + cur.K - 1;
@@ -805,7 +817,7 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
             S.vb[q] = vb_load4(okb ? (const MBH_GLOBAL float*)cur.B : nullp, okb ? 4u * (unsigned)jb : 0u);
         }
     };
-    // S's k-tile into LDS buffer b and its header.  S's loads are the older
+    // S's k-tile into LDS buffer b.  S's loads are the older
     // of the two sets in flight: one wait for all but the newer set's 2 NQ
     // loads (s_waitcnt with only vmcnt set), whatever loads the staging then
     // reads, so no later reuse of S's registers waits again.
@@ -880,17 +892,6 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
                         *(MBH_LDS f32x4*)(bs + 32 * q * SBF) = f32x4{rb[4 * q], rb[4 * q + 1], rb[4 * q + 2], rb[4 * q + 3]};
                 }
             }
-        }
-        if (ltid == 0) {
-            VbWsHdr h;
-            h.Ct = (float*)S.Ct;
-            h.ldc = S.ldc;
-            h.mr = S.mr;
-            h.nr = S.nr;
-            h.nk = min(32, max(0, S.krem));
-            h.last = S.end ? 0 : S.last;
-            h.end = S.end;
-            hdr[b] = h;
         }
     };
 
@@ -1038,28 +1039,31 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
     // its k-steps from LDS buffer b on (one barrier each), its C tile after
     // the last; returns the buffer of the next step.  The accumulators live
     // only within an item.
-    auto run_item = [&](int b, auto bi_c, auto bj_c, auto beta_c) {
+    // The MFMA waves walk the same list as the loaders, from the same
+    // records (a scalar load of the C tile's fields per item), so they know
+    // each step's k-depth and the C tile without reading anything the
+    // loaders write but the k-tiles themselves.
+    auto run_item = [&](int b, const VbWsTile& r, auto bi_c, auto bj_c, auto beta_c) {
         f32x16 acc[TW][TW];
 #pragma unroll
         for (int i = 0; i < TW; ++i)
 #pragma unroll
             for (int jj = 0; jj < TW; ++jj)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) acc[i][jj][r] = 0.0f;
-        for (;;) {
-            const int nk = __builtin_amdgcn_readfirstlane(hdr[b].nk);
-            const int last = __builtin_amdgcn_readfirstlane(hdr[b].last);
+                for (int e = 0; e < 16; ++e) acc[i][jj][e] = 0.0f;
+        for (int kt = 0;; ++kt) {
+            const int nk = min(32, max(0, r.K - 32 * kt));
+            const bool last = kt == r.nkt - 1;
             compute(acc, lbuf(b), nk, bi_c, bj_c);
             if (last) {
-                const uint64_t cbits = (uint64_t)hdr[b].Ct;
-                const unsigned clo = __builtin_amdgcn_readfirstlane((unsigned)cbits);
-                const unsigned chi = __builtin_amdgcn_readfirstlane((unsigned)(cbits >> 32));
-                MBH_GLOBAL float* Ct = (MBH_GLOBAL float*)(((uint64_t)chi << 32) | clo);
-                const int ldc = __builtin_amdgcn_readfirstlane(hdr[b].ldc);
-                const int mr = __builtin_amdgcn_readfirstlane(hdr[b].mr) - wm0;
-                const int nr = __builtin_amdgcn_readfirstlane(hdr[b].nr) - wn0;
+                // the C tile's origin and extent, opaque to the compiler here
+                // so that the epilogue's per-lane addresses are formed here
+                // and not held in registers across the k-loop
+                uint64_t cb = (uint64_t)((MBH_GLOBAL float*)r.C + (r.m0 + (int64_t)r.n0 * r.ldc));
+                int ldc = r.ldc, mr = r.m - r.m0 - wm0, nr = r.n - r.n0 - wn0;
+                asm volatile("" : "+s"(cb), "+s"(ldc), "+s"(mr), "+s"(nr));
                 __syncthreads();  // every MFMA wave is done with buffer b
-                epilogue(acc, lbuf(b), Ct, ldc, mr, nr, bi_c, bj_c, beta_c);
+                epilogue(acc, lbuf(b), (MBH_GLOBAL float*)cb, ldc, mr, nr, bi_c, bj_c, beta_c);
             }
             __syncthreads();
             b ^= 1;
@@ -1069,8 +1073,8 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
 
     // The two roles run separate loops (so neither holds the other's
     // registers) with the same barriers: one after the prologue and one per
-    // step, steps taken in pairs; both leave at the first even step whose
-    // header says end (a list that ends on an odd step takes one empty step).
+    // step, steps taken in pairs; both leave at the first even step past the
+    // end of the list (a list that ends on an odd step takes one empty step).
     // The loader's loop has no exit between its two halves, so the compiler
     // sees each set's loads in flight across exactly one half.
     if (loader) {
@@ -1078,8 +1082,7 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         fetch(S0);
         fetch(S1);
         stage(0, S0);
-        // the end / last flags of the headers in buffers 0 and 1, as staged
-        // (kept in scalar registers rather than read back from LDS)
+        // end / last flags of the k-tiles staged into buffers 0 and 1
         int end0 = S0.end, last0 = S0.end ? 0 : S0.last, last1;
         __syncthreads();
         // (steps that end an item take a second barrier: the MFMA waves'
@@ -1103,18 +1106,21 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
         typedef std::integral_constant<int, TW> tw;
         // The walk, in two copies: beta == 0 (C never read) and beta != 0.
         auto walk = [&](auto beta_c) {
+            int jm = wsoff[blockIdx.x];
+            const int jmend = wsoff[blockIdx.x + 1];
             for (int b = 0;;) {
-                if (__builtin_amdgcn_readfirstlane(hdr[b].end)) {
+                if (jm >= jmend) {
                     if (b == 0) break;
                     __syncthreads();  // the empty odd step
                     b = 0;
                     continue;
                 }
+                const VbWsTile r = vb_tile_rec(recs, jm++);
                 // blocks of the wave below the item's edge (bf16: all of
                 // them, its MFMAs cost little; those past the edge are never
                 // stored)
-                const int tmr = __builtin_amdgcn_readfirstlane(hdr[b].mr);
-                const int tnr = __builtin_amdgcn_readfirstlane(hdr[b].nr);
+                const int tmr = r.m - r.m0;
+                const int tnr = r.n - r.n0;
                 if (!BF16 && TW == 2 && tmr <= 64 && tnr > 64) {
                     wm0 = 0;
                     wn0 = 32 * wave;
@@ -1130,15 +1136,15 @@ __global__ __launch_bounds__(512, 4) void vb_gemm_ws_kernel(const VbWsRec* __res
                 const int bi = min(strip_c, max(0, (tmr - wm0 + 31) >> 5));
                 const int bj = min(strip_r, max(0, (tnr - wn0 + 31) >> 5));
                 if (BF16 || (bi == TW && bj == TW))
-                    b = run_item(b, tw(), tw(), beta_c);
+                    b = run_item(b, r, tw(), tw(), beta_c);
                 else if (bi == 0 || bj == 0)
-                    b = run_item(b, zero(), zero(), beta_c);
+                    b = run_item(b, r, zero(), zero(), beta_c);
                 else if (TW == 2 && bi == 1 && bj == 2)
-                    b = run_item(b, one(), tw(), beta_c);
+                    b = run_item(b, r, one(), tw(), beta_c);
                 else if (TW == 2 && bi == 2 && bj == 1)
-                    b = run_item(b, tw(), one(), beta_c);
+                    b = run_item(b, r, tw(), one(), beta_c);
                 else
-                    b = run_item(b, one(), one(), beta_c);
+                    b = run_item(b, r, one(), one(), beta_c);
             }
         };
         __syncthreads();
@@ -1549,7 +1555,7 @@ int mbh_vbatch_plan_create(int64_t count, const int64_t* m, const int64_t* n, co
         // A persistent grid of the workgroups that stay resident (at most
         // wgs per CU); each workgroup gets a fixed list, balanced by a cost
         // per tile in units of one 32-deep k-step's fixed work (barrier,
-        // headers, staging): bf16, its k-steps plus one for the C tile (a
+        // staging): bf16, its k-steps plus one for the C tile (a
         // step's time varies little with the panels it stages: weighting
         // the staged bytes measured slower); fp32, its k-steps plus the
         // busiest MFMA wave's 32 x 32 blocks times the tile's 8-deep
