@@ -223,3 +223,28 @@ def test_bf16_operand_exec_rejections(monkeypatch):
     with pytest.raises(mb.HipBlasError, match="work-queue kernels"):
         plan.exec(1.0, p, p, 0.0, p, operands="bf16")
     plan.close()
+
+
+def test_alpha_zero_and_empty_bf16_operands():
+    """alpha == 0 scales C by beta without reading A or B (beta == 1 leaves C
+    untouched); a batch whose problems are all empty or k == 0 runs."""
+    m = np.array([16, 9, 0, 33], np.int64)
+    n = np.array([16, 0, 5, 17], np.int64)
+    k = np.array([16, 8, 8, 0], np.int64)
+    g = torch.Generator(device="cuda").manual_seed(77)
+    b16, (A32, B32, C), offs, p16, p32 = _pack16(m, n, k, g)
+    c0 = (torch.rand(C.numel(), device="cuda", generator=g) * 2 - 1).cpu().numpy()
+    plan = batched.VBatchPlan(m, n, k, precision="bf16")
+    C.copy_(torch.from_numpy(c0))
+    plan.exec(0.0, p16[0], p16[1], 1.0, p32[2], operands="bf16")
+    torch.cuda.synchronize()
+    assert np.array_equal(C.cpu().numpy(), c0)
+    plan.exec(0.0, p16[0], p16[1], -2.0, p32[2], operands="bf16")
+    torch.cuda.synchronize()
+    _check_problems("fp32", m, n, k, (A32, B32, C), offs, 0.0, -2.0, c0, range(len(m)))
+    # k == 0 problem with alpha != 0: C := beta C
+    C.copy_(torch.from_numpy(c0))
+    plan.exec(1.5, p16[0], p16[1], 0.5, p32[2], operands="bf16")
+    torch.cuda.synchronize()
+    _check_problems("fp32", m, n, k, (A32, B32, C), offs, 1.5, 0.5, c0, range(len(m)))
+    plan.close()
