@@ -288,8 +288,9 @@ int mbh_trsv_status(void* stream, int* failed);
  *                accumulate).  Calls sharing one plan must not overlap.
  *  plan_info   : out[8] = {count, tiles, workgroups, flops, problems on
  *                64x64 tiles, on 128x128 tiles, kernel (1: warp-specialized
- *                tiles on fixed per-workgroup lists, 0: work queues),
- *                precision}.
+ *                tiles on fixed per-workgroup lists, 0: work queues, 2: the
+ *                two-class kernels with 64-bit offsets, taken when an
+ *                operand spans 2^30 elements or more), precision}.
  *  set_tuning  : for one precision, the class table (36 ints,
  *                [bucket(m)][bucket(n)], buckets <=16,32,64,128,256,larger;
  *                0 = 64x64 tiles, 1 = 128x128; NULL = compiled default) and

@@ -1750,7 +1750,9 @@ int mbh_vbatch_plan_info(mbh_vbatch_plan handle, double* info) {
     info[3] = plan->flops;
     info[4] = (double)plan->class_count[0];
     info[5] = (double)plan->class_count[1];
-    info[6] = plan->ws ? 1.0 : 0.0;  // 1: warp-specialized tiles on fixed lists, 0: work queues
+    // 1: warp-specialized tiles on fixed lists, 0: work queues, 2: the
+    // two-class kernels with 64-bit offsets
+    info[6] = plan->ws ? 1.0 : plan->lean ? 0.0 : 2.0;
     info[7] = (double)plan->prec;
     return 0;
 }

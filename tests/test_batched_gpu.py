@@ -273,3 +273,34 @@ def test_padded_leading_dimensions(prec, table, path):
 def test_invalid_problem_is_rejected_before_gpu_work():
     with pytest.raises(mb.HipBlasError, match="problem 1 fails gemm argument 8"):
         batched.VBatchPlan([4, 4], [4, 4], [4, 4], lda=[4, 3])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_operand_of_2_30_elements_takes_64bit_offsets(prec):
+    """A problem whose A spans 2^30 elements (lda = 2^20, k = 1024: 4 GiB)
+    does not fit the 32-bit offsets of the other kernels, so the plan runs
+    the two-class kernels with 64-bit offsets (plan_info kernel = 2); both
+    problems are checked in full against the fp64 oracle bound."""
+    m, n, k = np.array([64, 100]), np.array([64, 70]), np.array([1024, 50])
+    lda = np.array([1 << 20, 100])
+    g = torch.Generator(device="cuda").manual_seed(48)
+    A = [torch.rand(int((k[p] - 1) * lda[p] + m[p]), device="cuda", generator=g) * 2 - 1 for p in range(2)]
+    B = [torch.rand(int(k[p] * n[p]), device="cuda", generator=g) * 2 - 1 for p in range(2)]
+    C = [torch.rand(int(m[p] * n[p]), device="cuda", generator=g) * 2 - 1 for p in range(2)]
+    C0 = [c.cpu().numpy().astype(np.float64) for c in C]
+    ptr = lambda ts: torch.tensor([t.data_ptr() for t in ts], dtype=torch.int64, device="cuda")
+    plan = batched.VBatchPlan(m, n, k, lda=lda, precision=prec)
+    assert plan.info()["kernel"] == 2.0, plan.info()
+    plan.exec(0.75, ptr(A), ptr(B), -1.25, ptr(C))
+    torch.cuda.synchronize()
+    for p in range(2):
+        a = torch.as_strided(A[p], (int(m[p]), int(k[p])), (1, int(lda[p]))).cpu().numpy().astype(np.float64)
+        b = B[p].cpu().numpy().astype(np.float64).reshape(int(n[p]), int(k[p])).T
+        want = 0.75 * (a @ b) - 1.25 * C0[p].reshape(int(n[p]), int(m[p])).T
+        got = C[p].cpu().numpy().astype(np.float64).reshape(int(n[p]), int(m[p])).T
+        e = E_REP[prec] + (2.0 * int(k[p]) + 2.0) * H.U32
+        bound = e * 0.75 * (np.abs(a) @ np.abs(b)) + 4 * H.U32 * 1.25 * np.abs(C0[p].reshape(int(n[p]), int(m[p])).T)
+        assert (np.abs(got - want) <= bound + 1e-30).all(), (prec, p, float((np.abs(got - want) / bound).max()))
+    plan.close()
+    del A
+    torch.cuda.empty_cache()
