@@ -17,6 +17,7 @@ from HIP events recorded on the stream the kernels are launched on
 printed on stdout; progress goes to stderr.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -357,15 +358,27 @@ def main():
     main_ms, pack_ms = kernel_ms[("bf16x3", "NN")]
     tf_main = FLOPS / main_ms / 1e9
     peak = PEAK_BF16 / 3.0
-    traffic = None
+    traffic, traffic_note = None, None
     # HBM bytes per launch of the main kernel, from the two rocprofv3 PMC
-    # passes (FETCH_SIZE, WRITE_SIZE) calibrated against a 1 GiB copy.
-    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_packed_bf16x3_8192_traffic.txt")
+    # passes (FETCH_SIZE, WRITE_SIZE) calibrated against a 1 GiB copy; used
+    # only while the kernel's sources hash to what was profiled.
+    root = os.path.dirname(os.path.abspath(__file__))
+    tpath = os.path.join(root, "profiles", "gemm_packed_bf16x3_8192_traffic.txt")
     if os.path.exists(tpath):
+        rec = {}
         with open(tpath) as f:
             for ln in f:
-                if ln.startswith("traffic_bytes_per_launch:"):
-                    traffic = int(ln.split(":")[1])
+                key, sep, val = ln.partition(":")
+                if sep and not ln.startswith(" "):
+                    rec[key.strip()] = val.strip()
+        h = hashlib.sha256()
+        for src in rec.get("kernel_sources", "").split():
+            with open(os.path.join(root, src), "rb") as f:
+                h.update(f.read())
+        if rec.get("kernel_sources") and h.hexdigest() == rec.get("kernel_sources_sha256"):
+            traffic = int(rec["traffic_bytes_per_launch"])
+        else:
+            traffic_note = "stale: the kernel's sources changed since the counters were collected"
     line = {
         "metric": "SGEMM TFLOP/s @8192^3: BF16x3-MFMA vs plain FP32 at stated FP64-oracle error",
         "value": round(tf_bf, 3),
@@ -397,8 +410,8 @@ def main():
             "traffic": traffic,
             "traffic_source": ("profiles/gemm_packed_bf16x3_8192_traffic.txt: rocprofv3 --pmc FETCH_SIZE and "
                                "WRITE_SIZE passes of this kernel at this size (FETCH_SIZE x2.000, WRITE_SIZE x1.000 "
-                               "from a 1 GiB device copy in the same process); this run reads no counters"
-                               if traffic is not None else None),
+                               "from a 1 GiB device copy in the same process), kernel sources checked by sha256; "
+                               "this run reads no counters" if traffic is not None else traffic_note),
             "algorithmic_bytes": 2 * 2 * N * N * 2 + N * N * 4,
             "kernel": "gemm_packed_kernel<3,256,256,2,4>",
             "kernel_ms": round(main_ms, 4),
