@@ -249,6 +249,34 @@ def cpu_baseline(args):
     return rec
 
 
+
+def committed_traffic(root, name="gemm_packed_bf16x3_8192_traffic.txt"):
+    """HBM bytes per launch of the BF16x3 main kernel at 8192^3 from the
+    committed profile (two rocprofv3 PMC passes, FETCH_SIZE and WRITE_SIZE,
+    calibrated against a 1 GiB copy), and None in its place when the file is
+    missing or the kernel's sources no longer hash to what was profiled.
+    Returns (bytes or None, note or None)."""
+    tpath = os.path.join(root, "profiles", name)
+    if not os.path.exists(tpath):
+        return None, None
+    rec = {}
+    with open(tpath) as f:
+        for ln in f:
+            key, sep, val = ln.partition(":")
+            if sep and not ln.startswith(" "):
+                rec[key.strip()] = val.strip()
+    srcs = rec.get("kernel_sources", "").split()
+    h = hashlib.sha256()
+    for src in srcs:
+        path = os.path.join(root, src)
+        if not os.path.exists(path):
+            return None, "stale: %s is missing" % src
+        with open(path, "rb") as f:
+            h.update(f.read())
+    if srcs and h.hexdigest() == rec.get("kernel_sources_sha256") and "traffic_bytes_per_launch" in rec:
+        return int(rec["traffic_bytes_per_launch"]), None
+    return None, "stale: the kernel's sources changed since the counters were collected"
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,27 +386,7 @@ def main():
     main_ms, pack_ms = kernel_ms[("bf16x3", "NN")]
     tf_main = FLOPS / main_ms / 1e9
     peak = PEAK_BF16 / 3.0
-    traffic, traffic_note = None, None
-    # HBM bytes per launch of the main kernel, from the two rocprofv3 PMC
-    # passes (FETCH_SIZE, WRITE_SIZE) calibrated against a 1 GiB copy; used
-    # only while the kernel's sources hash to what was profiled.
-    root = os.path.dirname(os.path.abspath(__file__))
-    tpath = os.path.join(root, "profiles", "gemm_packed_bf16x3_8192_traffic.txt")
-    if os.path.exists(tpath):
-        rec = {}
-        with open(tpath) as f:
-            for ln in f:
-                key, sep, val = ln.partition(":")
-                if sep and not ln.startswith(" "):
-                    rec[key.strip()] = val.strip()
-        h = hashlib.sha256()
-        for src in rec.get("kernel_sources", "").split():
-            with open(os.path.join(root, src), "rb") as f:
-                h.update(f.read())
-        if rec.get("kernel_sources") and h.hexdigest() == rec.get("kernel_sources_sha256"):
-            traffic = int(rec["traffic_bytes_per_launch"])
-        else:
-            traffic_note = "stale: the kernel's sources changed since the counters were collected"
+    traffic, traffic_note = committed_traffic(os.path.dirname(os.path.abspath(__file__)))
     line = {
         "metric": "SGEMM TFLOP/s @8192^3: BF16x3-MFMA vs plain FP32 at stated FP64-oracle error",
         "value": round(tf_bf, 3),
