@@ -5,13 +5,18 @@
 // semantics of gemm("N", "N", ...), src/level3/gemm.mojo:56-124.)
 //
 // Plan (host, once per shape list): validate, give every problem a tile class
-// from the tuning table, enumerate its C tiles as work items, order the items
-// longest-first (by BM*BN*k) and copy everything to HBM.
+// from the tuning table, enumerate its C tiles as work items and copy them to
+// HBM, either as one fixed list per workgroup of each class's persistent grid
+// (balanced longest-processing-time first over the XCDs and their
+// workgroups) or, for batches the warp-specialized tiles cannot take, as
+// work queues.
 // Exec (device only, no allocation, no host synchronisation, capturable in a
-// hipGraph): a one-thread kernel resets the work counter, then the persistent
-// kernel's workgroups pull items from the counter until the list is
-// exhausted.  In bf16-input mode the fp32 operands are rounded to bf16 as
-// they are staged into LDS, so no separate conversion pass runs.
+// hipGraph): a fill kernel writes one record per item from the caller's
+// pointer arrays and the warp-specialized kernels walk their lists; or a
+// one-thread kernel resets the queue counters and the work-queue kernels'
+// workgroups claim items until the queues are drained.  In bf16 mode fp32
+// operands are rounded to bf16 as they are staged into LDS (no conversion
+// pass), and bf16-stored operands are staged as they are.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
