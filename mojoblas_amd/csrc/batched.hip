@@ -603,13 +603,8 @@ static_assert(offsetof(VbWsRec, C) == 16 && offsetof(VbWsRec, m) == 24 && offset
                   offsetof(VbWsRec, ldc) == 44 && offsetof(VbWsRec, m0) == 48 && offsetof(VbWsRec, nkt) == 56,
               "vb_tile_rec reads the record by dword");
 
-// Record j (a wave-uniform index) by a scalar load: the records are written
-// by the fill kernel before the tile kernel starts and only read by it, so
-// they are read through the constant address space.  (Through the vector
-// cache the loads would share the vmcnt counter with the operand loads, and
-// moving them to scalar registers would wait for every operand load in
-// flight.)
-// The part of record j the MFMA waves use: the C tile and the k-steps.
+// The part of record j the MFMA waves use (the C tile and the k-steps), by
+// scalar loads as in vb_rec below.
 struct VbWsTile {
     float* C;
     int m, n, K, ldc, m0, n0, nkt;
@@ -630,6 +625,12 @@ __device__ inline VbWsTile vb_tile_rec(const VbWsRec* __restrict__ recs, int j) 
     return t;
 }
 
+// Record j (a wave-uniform index) by a scalar load: the records are written
+// by the fill kernel before the tile kernel starts and only read by it, so
+// they are read through the constant address space.  (Through the vector
+// cache the loads would share the vmcnt counter with the operand loads, and
+// moving them to scalar registers would wait for every operand load in
+// flight.)
 __device__ inline VbWsRec vb_rec(const VbWsRec* __restrict__ recs, int j) {
     typedef const __attribute__((address_space(4))) unsigned* cu32_t;
     const cu32_t p = (cu32_t)(recs + j);
