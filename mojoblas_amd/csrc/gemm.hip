@@ -369,11 +369,17 @@ hipError_t run_fp8(bool at, bool bt, const GemmParams<float>& g, int tri_grid, h
     TimingMark tmk;
     tmk.active = timing_on();
     timing_mark(tmk, 0, s);
-    dim3 ga((unsigned)(tm * BT / 128), (unsigned)kblocks), gb((unsigned)(tn * BT / 128), (unsigned)kblocks);
-    if (at) pack_fp8_kernel<true, BT><<<ga, 256, 0, s>>>(pa);
-    else pack_fp8_kernel<false, BT><<<ga, 256, 0, s>>>(pa);
-    if (!bt) pack_fp8_kernel<true, BT><<<gb, 256, 0, s>>>(pb);
-    else pack_fp8_kernel<false, BT><<<gb, 256, 0, s>>>(pb);
+    // One launch packs both operands in 128x128 blocks, op(A) first: op(A)
+    // is k-contiguous iff A is transposed, op(B)^T iff B is not.
+    const int64_t gx_a = tm * BT / 128, gx_b = tn * BT / 128;
+    const int64_t nblk_a = gx_a * kblocks, nblk = nblk_a + gx_b * kblocks;
+    if (nblk > INT_MAX) return hipErrorInvalidValue;
+    const dim3 pgrid((unsigned)nblk);
+    const int xa = (int)gx_a, na_ = (int)nblk_a, xb = (int)gx_b;
+    if (at && !bt) pack_fp8_kernel<true, true, BT><<<pgrid, 256, 0, s>>>(pa, pb, xa, na_, xb);
+    else if (at && bt) pack_fp8_kernel<true, false, BT><<<pgrid, 256, 0, s>>>(pa, pb, xa, na_, xb);
+    else if (!at && !bt) pack_fp8_kernel<false, true, BT><<<pgrid, 256, 0, s>>>(pa, pb, xa, na_, xb);
+    else pack_fp8_kernel<false, false, BT><<<pgrid, 256, 0, s>>>(pa, pb, xa, na_, xb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     PackedGemmParams p{};
     p.m = g.m;

@@ -341,12 +341,13 @@ __device__ inline uint32_t fp8_pack4(float a, float b, float c, float d) {
     return (uint32_t)w;
 }
 
+// One 128-row x 128-k block (bx, by) of op(X); `red` is four floats of the
+// workgroup's LDS.
 template <bool KCONTIG, int PR>
-__global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params p) {
+__device__ inline void pack_fp8_block(const PackFp8Params& p, int64_t bx, int64_t by, float* red) {
     constexpr int T = 128, PK = 64;
-    __shared__ float red[4];
     const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * T, k0 = (int64_t)blockIdx.y * T;
+    const int64_t r0 = bx * T, k0 = by * T;
     const bool full = p.vec && r0 + T <= p.R && k0 + T <= p.K;  // no edge: unguarded 16-byte loads
     // KCONTIG ((r, k) at X[k + r ld]): v[j] = X(gr0 + 8j, gk0 .. gk0 + 3),
     //   16 rows from gr0 = r0 + tid/32 (stride 8) at the 4 consecutive k from
@@ -410,6 +411,22 @@ __global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params p) {
             w.w = fp8_pack4(v[12][e4] * inv, v[13][e4] * inv, v[14][e4] * inv, v[15][e4] * inv);
             *reinterpret_cast<uint4*>(dst(gr0 + e4, gk0)) = w;
         }
+    }
+}
+
+// Both operands of one FP8 gemm in one launch: workgroups [0, nblk_a) pack
+// op(A) (gx_a row blocks x k blocks), the rest op(B)^T.  KA / KB: k is the
+// contiguous index of op(A) / op(B)^T.
+template <bool KA, bool KB, int PR>
+__global__ __launch_bounds__(256) void pack_fp8_kernel(PackFp8Params a, PackFp8Params b, int gx_a, int nblk_a,
+                                                       int gx_b) {
+    __shared__ float red[4];
+    int w = (int)blockIdx.x;
+    if (w < nblk_a) {
+        pack_fp8_block<KA, PR>(a, w % gx_a, w / gx_a, red);
+    } else {
+        w -= nblk_a;
+        pack_fp8_block<KB, PR>(b, w % gx_b, w / gx_b, red);
     }
 }
 
